@@ -1,0 +1,225 @@
+/*
+ * vlasov_hip.h -- C ABI of libvlasov_hip.so, the gfx950 implementation of the
+ * reference's per-step pseudo-particle loop.
+ *
+ * The boundary is the reference's device layer: every entry point below
+ * replaces one method of `DeviceAccelerator` / `DeviceMesh`
+ * (src/accelerator.jl, src/devicemesh.jl) or one buffer operation of
+ * `DualBuffer` (src/accelerator.jl:28-68), and is cited there.  Plain
+ * pointers and sizes only.  Host arrays use the reference's Julia
+ * column-major layout: a grid is `[i + n*j + n*n*k]`, a matrix
+ * `[row + n*col]`, the force buffer `(3, N)` and the particle array a
+ * `PackedParticle{Float32}` record of 48 bytes
+ * {int32 knode[3]; float delta[3]; int32 pknode[3]; float pdelta[3]}
+ * (src/particles.jl:86-91).  Cell indices written by the library are the
+ * reference's 1-based ones.
+ *
+ * Conventions: every function returns 0 on success and a negative code on
+ * failure, with a message for the calling thread in vh_last_error().  Sizes
+ * are checked before any launch (the reference throws ArgumentError /
+ * DimensionMismatch before launching, src/accelerator.jl:583-586,613).
+ * Launches are asynchronous on the context's stream; an entry point that
+ * returns a host scalar synchronises first.  Handles are not thread-safe.
+ */
+#ifndef VLASOV_HIP_H
+#define VLASOV_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VH_OK 0
+#define VH_EINVAL (-1)   /* bad argument (ArgumentError / DimensionMismatch) */
+#define VH_EHIP (-2)     /* HIP runtime error */
+#define VH_ESTATE (-3)   /* call out of order (e.g. forces before a sort) */
+
+typedef struct vh_context vh_context;
+typedef struct vh_mesh vh_mesh;
+typedef struct vh_accel vh_accel;
+
+/* Grids held by a mesh, for vh_mesh_grid_download / _upload. */
+enum { VH_GRID_RHO = 0, VH_GRID_PHI = 1, VH_GRID_CSOL = 2 };
+
+/* Law of the projectile <-> pseudo-electron interaction inside the field
+ * kernel.  VH_SOFT_DEVICE reproduces the reference's fused device kernel:
+ * Gaussian kernel of width scale(softening) for the in-stencil particles and
+ * the projectile's total force, positions rebuilt from the cell
+ * (src/kernels.jl:425-443, src/accelerator.jl:628-632), the projectile's own
+ * law for the out-of-stencil fallback (src/accelerator.jl:719-724).
+ * VH_SOFT_BALL applies the CPU path's uniform-ball law everywhere
+ * (src/projectile.jl:79-80, 203-232). */
+enum { VH_SOFT_DEVICE = 0, VH_SOFT_BALL = 1 };
+
+/* Message of the last failure on this thread ("" if none). */
+const char *vh_last_error(void);
+/* Library version string. */
+const char *vh_version(void);
+/* Number of visible GPUs (does not create a context). */
+int vh_device_count(int *count);
+
+/* --- context: one device, one stream ------------------------------------ */
+int vh_context_create(int device, vh_context **out);
+int vh_context_destroy(vh_context *ctx);
+/* Waits for every launch of the context (the synchronisation `download!`
+ * performs first, src/accelerator.jl:64-68). */
+int vh_sync(vh_context *ctx);
+
+/* --- DeviceMesh (src/devicemesh.jl:21-81) --------------------------------
+ * Built from the FP64 tables of a SplineMesh; three identical axes.
+ *   n        collocation points per axis (= 2 nk), ni = n - 2
+ *   colloc   n            knots nk
+ *   lap      n x n        full D = S'' S^-1 (src/collocation.jl:94)
+ *   M, Minv  ni x ni      eigenbasis of the interior D (src/tensorsolver.jl:128-137)
+ *   lam      ni           its eigenvalues
+ *   Sinv     n x n        inverse collocation matrix (src/collocation.jl:53)
+ *   dual     3 x n        S^-T m_k, k = 0,1,2 (src/mesh.jl:61-62), column k*n
+ *   duallen  n            dual lengths (src/deposition.jl:185-195)
+ *   loc_x0, loc_invwidth, loc_cells[nbin]  LocateTable (src/splines.jl:362-376),
+ *            1-based cells as the reference stores them.
+ * Replaces `DeviceMesh(backend, E, mesh)` (src/devicemesh.jl:49-81). */
+int vh_mesh_create(vh_context *ctx, int n, int nk,
+                   const double *colloc, const double *knots,
+                   const double *lap, const double *M, const double *Minv,
+                   const double *lam, const double *Sinv,
+                   const double *dual, const double *duallen,
+                   double loc_x0, double loc_invwidth,
+                   const int32_t *loc_cells, int nbin, vh_mesh **out);
+int vh_mesh_destroy(vh_mesh *m);
+/* `Array(dev.rho[l])` etc. (src/simulation.jl:544-552): n^3 floats. */
+int vh_mesh_grid_download(vh_mesh *m, int which, float *host);
+int vh_mesh_grid_upload(vh_mesh *m, int which, const float *host);
+
+/* poisson!(phi, rho, dm) single level with multipole faces
+ * (src/devicemesh.jl:148-151). */
+int vh_poisson(vh_mesh *m);
+/* poisson!(phis, rhos, dms) two levels, coarse first, fine faces read from
+ * the coarse spline (src/devicemesh.jl:160-169; src/poisson.jl:427-436).
+ * Leaves csol of the coarse mesh = S^-1 phi_coarse as a by-product. */
+int vh_poisson_nested(vh_mesh *fine, vh_mesh *coarse);
+/* spline_coefficients!(csol, phi, dm): csol = (S^-1 x S^-1 x S^-1) phi
+ * (src/devicemesh.jl:126-127; src/deposition.jl:354-360). */
+int vh_spline_coefficients(vh_mesh *m);
+/* effective_potential!(csol, rho, dm, jel): csol += S^-1^3 (V_xc(rho) +
+ * V_jel(r)) (src/devicemesh.jl:171-183; src/meanfield.jl:116-147). */
+int vh_effective_potential(vh_mesh *m, double nions, double radius);
+/* total_charge(rho, dm) in FP64 (src/devicemesh.jl:112-113). */
+int vh_total_charge(vh_mesh *m, double *q);
+/* multipole(rho, dm): q, center[3], quadrupole[6] in FP64
+ * (src/devicemesh.jl:115-117; src/poisson.jl:152-189). */
+int vh_multipole(vh_mesh *m, double out10[10]);
+
+/* --- DeviceAccelerator (src/accelerator.jl:126-185, 281-331) -------------
+ *   npart      particles;  nk knots of the uniform fine axis, n = 2 nk
+ *   knots      the nk knots (x0 = knots[0], h = (knots[nk-1]-knots[0])/(nk-1),
+ *              non-uniform axes refused, src/accelerator.jl:262-268)
+ *   spacing    GaussianSmoothing.spacing (src/fields.jl:165)
+ *   overlap, gradient  10 x ncol, nodes 8 x ncol: GaussianSmoothing tables
+ *              (src/fields.jl:44-53), ncol = nbdt + 1
+ * Replaces `DeviceAccelerator(backend, E, fine, sm, npart, n; packed=true)`. */
+int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const double *knots,
+                    double spacing, int nbdt, const double *overlap,
+                    const double *gradient, const double *nodes, vh_accel **out);
+int vh_accel_destroy(vh_accel *a);
+
+/* upload!/download! of the particle buffer (src/accelerator.jl:48, 64-68):
+ * npart PackedParticle{Float32} records, in the device's current (sorted)
+ * order. */
+int vh_particles_upload(vh_accel *a, const void *records);
+int vh_particles_download(vh_accel *a, void *records);
+/* download!(acc.force): (3, npart) floats, slot order (src/accelerator.jl:647). */
+int vh_forces_download(vh_accel *a, float *host);
+int vh_forces_upload(vh_accel *a, const float *host);
+
+/* _sort_and_list!(acc) for a resident cloud (src/accelerator.jl:466-503):
+ * stable counting sort of the records by clamped cell key
+ * (src/kernels.jl:902-908), occupied cells and bounds, and the list of
+ * particles whose 10^3 stencil leaves the fine grid (src/kernels.jl:1041-1058).
+ * The records are moved; the order within a cell is the previous order
+ * (stable, as cellsort!, src/sorting.jl:163-204).  Outputs may be NULL. */
+int vh_sort(vh_accel *a, int32_t *n_occupied, int32_t *outcount);
+/* The occupied-cell list (1-based keys, increasing), bounds (length n_occ+1,
+ * exclusive prefix) and outlist (1-based sorted slots, increasing) of the
+ * last sort.  Arrays must hold n_occupied, n_occupied+1 and outcount
+ * entries. */
+int vh_sort_lists(vh_accel *a, int32_t *occupied, int32_t *bounds, int32_t *outlist);
+/* _fill_columns!: table columns (3, npart) 1-based and nout, the particles
+ * outside [knots[2] + h/2, knots[end-1] - h/2] (src/accelerator.jl:516-529;
+ * src/kernels.jl:754-779). */
+int vh_columns(vh_accel *a, int32_t *cols, int32_t *nout);
+
+/* deposit_smoothed!(rho, acc, mesh, sm, positions; charge) -> nout, resident
+ * (src/accelerator.jl:531-565): sort, columns, sorted 8^3 deposit with no
+ * float atomics, FP64 total charge, renormalisation; rho of `fine`. */
+int vh_deposit_smoothed(vh_accel *a, vh_mesh *fine, double charge, int32_t *nout);
+/* deposit_cic!(rho, dm, acc, npart, charge, x0f, hf) (src/devicemesh.jl:93-106):
+ * trilinear deposit onto the stretched coarse grid, per-cell tiles combined
+ * through exact 64-bit fixed-point sums (deterministic). */
+int vh_deposit_cic(vh_accel *a, vh_mesh *coarse, double charge);
+
+/* forces!(cloud, acc, fine, csol_f, coarse, csol_c, sm; projectile, packed)
+ * (src/accelerator.jl:607-733): smoothed 10^3 field of the fine csol,
+ * fused projectile, and the out-of-stencil fallback on the device
+ * (coarse spline_field + the projectile's own law, src/fields.jl:358-393).
+ * `proj` = {x, y, z, charge, scale} of the projectile or NULL; `soft_mode`
+ * VH_SOFT_*; `ball_radius` is the radius of the projectile's own ball law
+ * used by the fallback (<= 0: Gaussian law of width scale in the fallback).
+ * red[4] (may be NULL) receives the reduction of src/kernels.jl:449-469:
+ * sum of the per-particle reaction forces and of the pair potential. */
+int vh_forces(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight,
+              const double *proj, int soft_mode, double ball_radius,
+              double red[4], int32_t *nout);
+
+/* step!(cloud, dt, acc; rcmax) -> (kinetic, escaped, angular[3])
+ * (src/accelerator.jl:580-605): packed position Verlet, diagnostics summed
+ * in FP64. diag may be NULL. */
+int vh_verlet(vh_accel *a, double dt, double mass, double rcmax, double diag[5]);
+
+/* enclosed_charge(cloud, proj, radius) (src/projectile.jl:243-247): number of
+ * particles closer than `radius` to `center` (multiply by the weight). */
+int vh_enclosed_count(vh_accel *a, const double center[3], double radius, int64_t *count);
+
+/* --- timing ---------------------------------------------------------------
+ * HIP events recorded on the context's stream around each stage of the step
+ * (and around the field kernel alone), for per-stage ms/step as
+ * scripts/profil_device.jl reports them.  No counterpart in the reference,
+ * which times stages with a device synchronisation after each. */
+enum {
+  VH_ST_SORT = 0,       /* counting sort, occupied cells, outlist */
+  VH_ST_DEPOSIT_FINE,   /* smoothed 8^3 deposit + charge + renormalisation */
+  VH_ST_DEPOSIT_CIC,    /* coarse cloud-in-cell */
+  VH_ST_POISSON,        /* nested Poisson (incl. S^-1 of the coarse phi) */
+  VH_ST_SPLINE,         /* S^-1 of the fine phi */
+  VH_ST_MEANFIELD,      /* effective potential, both levels */
+  VH_ST_FIELD,          /* field kernel + reduction + fallback */
+  VH_ST_VERLET,         /* packed Verlet + diagnostics */
+  VH_ST_FIELD_KERNEL,   /* the smoothed-field kernel launch alone */
+  VH_ST_DEPOSIT_KERNEL, /* the per-cell deposit tile kernel alone */
+  VH_ST_COUNT
+};
+int vh_profile_enable(vh_context *ctx, int on);
+/* Synchronises, adds the elapsed time of every recorded region to ms[stage]
+ * and the number of regions to count[stage] (arrays of VH_ST_COUNT), and
+ * clears the record. */
+int vh_profile_read(vh_context *ctx, double *ms, int64_t *count);
+
+/* --- host-side sampling (native, no device) -------------------------------
+ * Ran2 of the thesis code, IQ1 = 3668 unless consistent != 0
+ * (src/random.jl:288-347), and the two Thomas-Fermi samplers:
+ * sample_thomas_fermi(::RadialProfile) (src/initial.jl:94-113) and
+ * sample_thomas_fermi(::PotentialProfile) (src/initial.jl:204-229).
+ * pos, mom: npart x 3 doubles, particle-major ((3, N) column-major). */
+int vh_ran2_sequence(int seed, int consistent, int64_t count, float *out);
+int vh_sample_radial(const double *quantiles, int nq, const double *density, int nrho,
+                     double rmax, int64_t npart, double weight, int seed,
+                     double *pos, double *mom);
+int vh_sample_rejection(const double *grid, const double *potential, int ngrid,
+                        double rmax, double pmax, double fermi, int64_t npart,
+                        double weight, int seed, double *pos, double *mom);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VLASOV_HIP_H */

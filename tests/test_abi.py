@@ -1,0 +1,132 @@
+"""The C ABI library loads and exports every symbol the header declares; its
+host-only entry points (ran2 and the Thomas-Fermi samplers) are pinned by the
+reference's known answers.  No device call is made here."""
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, load_package
+
+va = load_package()
+HEADER = ROOT / "include" / "vlasov_hip.h"
+
+
+def header_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vh_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = va._lib.load()
+    names = header_functions()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(names) == sorted(va._lib.EXPORTED)
+    assert b"gfx950" in lib.vh_version()
+
+
+def test_library_is_gfx950_code_object():
+    so = (ROOT / "vlasov.jl_amd" / "libvlasov_hip.so").read_bytes()
+    assert b"amdgcn-amd-amdhsa--gfx950" in so
+
+
+def test_host_entry_points_refuse_bad_arguments():
+    with pytest.raises(va.VlasovHIPError, match="bad argument"):
+        va._lib.call("vh_sample_radial", np.zeros(1), 1, np.zeros(2), 2, 1.0, 5, 1.0, -1,
+                     np.zeros(15), np.zeros(15))
+
+
+def make_pot_dat(path: Path):
+    """pot.dat as ref/fortran98/make_pot.py writes it from rhoinit.dat
+    (V = -p_F^2 / 2 with p_F = (3 pi^2 rho)^(1/3), 7 significant digits)."""
+    tok = (va.physics.DATA_DIR / "rhoinit.dat").read_text().split()
+    n, rmax = int(tok[0]), float(tok[1])
+    rho = [float(v) for v in tok[2:2 + n]]
+    coef = (3.0 * math.pi ** 2) ** (1 / 3)
+    pF = [coef * r ** (1 / 3) for r in rho]
+    ng, pmax, ef = n - 1, max(pF), 0.0
+    lines = [f"{ng:12d}", f"  {rmax:.14E}  {pmax:.14E}  {ef:.14E}"]
+    for i in range(ng + 1):
+        lines.append(f"{i * rmax / ng:14.6E}{0.0:14.6E}{ef - 0.5 * pF[i] ** 2:14.6E}")
+    path.write_text("\n".join(lines) + "\n")
+
+
+def test_ran2_and_rejection_sampler_known_answers(tmp_path):
+    """Pseudo-particle 109 of the 1998 Fortran run, bit for bit
+    (test/oracle.jl:410-441): r and |p| go through no transcendental
+    function, so equality proves the ran2 stream was consumed rejection for
+    rejection as the oracle consumed it."""
+    pot = tmp_path / "pot.dat"
+    make_pot_dat(pot)
+    prof = va.physics.read_potential_profile(pot)
+    assert prof.rmax == 35.0 and prof.fermi == 0.0
+    assert abs(prof.potential[0] - -0.11552990) < 1e-9 and prof.potential[-1] == 0.0
+    npart, nbelec = 20_000, 196.0
+    pos, mom = va.sample_thomas_fermi(prof, npart, nbelec / npart)
+    r = math.sqrt(sum(x * x for x in pos[108]))
+    p = math.sqrt(sum(x * x for x in mom[108]))
+    assert r == 18.535201405644546
+    # |p| is recomputed from components that went through cos/sin: glibc and
+    # Julia's libm differ by an ulp there, so |p| is held to 2 ulp
+    assert abs(p - 3.7169869799140247e-3) <= 2 * np.spacing(3.7169869799140247e-3)
+    rel = lambda a, b: np.linalg.norm(np.array(a) - np.array(b)) / np.linalg.norm(b)
+    assert rel(pos[108], [11.908010207778275, 13.479312950281439, 4.4789626508412743]) < 1e-12
+    assert rel(mom[108], [1.1166910985229179e-3, 9.4449849564513094e-4, -3.4171502441441093e-3]) < 1e-12
+    assert math.isclose(pos[0][0], 20.103859205517573, rel_tol=1e-12)
+    assert math.isclose(mom[0][0], -3.3382689538331450e-3, rel_tol=1e-12)
+
+
+def test_ran2_statistics_and_variants():
+    """test/runtests.jl:27-51."""
+    a = va.physics.ran2_sequence(50, -1)
+    assert a.dtype == np.float32 and np.all((a >= 0) & (a < 1))
+    assert np.array_equal(a, va.physics.ran2_sequence(50, -1))
+    assert not np.array_equal(a, va.physics.ran2_sequence(50, -7))
+    assert not np.array_equal(va.physics.ran2_sequence(100, -1),
+                              va.physics.ran2_sequence(100, -1, consistent=True))
+    for consistent in (False, True):
+        v = va.physics.ran2_sequence(100_000, -3, consistent).astype(np.float64)
+        assert abs(v.mean() - 0.5) < 0.01
+        assert abs(((v - v.mean()) ** 2).mean() - 1 / 12) < 0.002
+
+
+def test_radial_sampler_statistics():
+    """Uniform ball profile: (r/R)^3 and (p/p_F)^3 uniform, isotropic
+    directions, reproducible (test/runtests.jl:979-1013)."""
+    R, nq = 10.0, 1001
+    q = np.array([R * np.cbrt((j) / (nq - 1)) for j in range(nq)])
+    rmax = 12.0
+    dens = np.array([1e-3 if r <= R else 0.0 for r in np.linspace(0, rmax, nq)])
+    prof = va.RadialProfile(q, dens, rmax)
+    npart, w = 20_000, 100.0 / 20_000
+    pos, mom = va.sample_thomas_fermi(prof, npart, w)
+    r = np.linalg.norm(pos, axis=1)
+    assert r.max() <= R + 1e-9 and abs(np.mean((r / R) ** 3) - 0.5) < 0.02
+    mu = pos[:, 2] / r
+    assert abs(mu.mean()) < 0.02 and abs((mu ** 2).mean() - 1 / 3) < 0.02
+    pf = va.physics.FERMI_COEFFICIENT * np.cbrt(1e-3)
+    pn = np.linalg.norm(mom, axis=1) / w
+    assert pn.max() <= pf * (1 + 1e-9) and abs(np.mean((pn / pf) ** 3) - 0.5) < 0.02
+    p2, m2 = va.sample_thomas_fermi(prof, 100, w)
+    p3, m3 = va.sample_thomas_fermi(prof, 100, w)
+    assert np.array_equal(p2, p3) and np.array_equal(m2, m3)
+
+
+def test_packed_split_round_trip():
+    """PackedPositions: k exact, |delta| <= h/2, x0 + (k-1)h + delta == x to
+    Float32 resolution of delta (src/particles.jl:115-158)."""
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-100, 100, (1000, 3))
+    x0, h = -78.0, 2.4375
+    k, d = va.physics.split(x, x0, h)
+    assert np.all(np.abs(d) <= h / 2 + 1e-6)
+    back = va.physics.unsplit(k, d, x0, h)
+    assert np.max(np.abs(back - x)) < 1e-6
+    # ties round to even, as Julia's round
+    k2, _ = va.physics.split(np.array([x0 + 0.5 * h, x0 + 1.5 * h]), x0, h)
+    assert list(k2) == [1, 3]

@@ -1,0 +1,239 @@
+"""Parity of every stage of the HIP path with the oracle, at config-1 shape
+(nfine = 28: 58^3 fine and coarse grids) on a seeded Thomas-Fermi cloud.
+
+Bars: bit-exact for the sort (records, occupied cells, bounds, out-of-stencil
+list), the table columns, the Float32 Verlet and the Float32 contraction of
+the field; 1e-5 relative (max |diff| / max |ref|) for FP32 densities,
+potentials and forces against the FP64 oracle."""
+import numpy as np
+import pytest
+
+from conftest import load_package
+
+pytestmark = pytest.mark.gpu
+va = load_package()
+
+NPART = 100_000
+REL = 1e-5
+
+
+def relerr(got, ref):
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    return float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+
+
+@pytest.fixture(scope="module")
+def setup(config1, orc):
+    p, fine, coarse, sm = config1
+    w = p.nelectrons / NPART
+    pos, mom = va.sample_thomas_fermi(va.read_radial_profile(), NPART, w, seed=-1)
+    # a cloud in motion: previous positions half a step back along the momenta
+    prev = pos - (p.dt / w) * mom
+    x0 = float(fine.axis.knots[0])
+    h = float((fine.axis.knots[-1] - fine.axis.knots[0]) / (fine.axis.nknots - 1))
+    rec = va.physics.packed_records(pos, prev, x0, h)
+    # add a few particles outside the fine grid and outside both grids
+    rec[:50]["knode"][:, 0] = 1
+    rec[50:60]["knode"][:, 1] = fine.axis.nknots + 3
+    rec[60:62]["knode"][:, 2] = -200
+    ctx = va.Context(0)
+    dmf = va.DeviceMesh(ctx, fine)
+    dmc = va.DeviceMesh(ctx, coarse)
+    acc = va.DeviceAccelerator(ctx, fine.axis, sm, NPART)
+    jel = va.Jellium.of(p.nions)
+    ost = orc.OracleStep(fine, coarse, sm, jel, w, w, p.dt, p.rcmax)
+    yield dict(p=p, fine=fine, coarse=coarse, sm=sm, w=w, rec=rec, ctx=ctx, dmf=dmf, dmc=dmc,
+               acc=acc, jel=jel, ost=ost, x0=x0, h=h)
+    acc.close(); dmf.close(); dmc.close(); ctx.close()
+
+
+def test_sort_bit_exact_and_idempotent(setup, orc):
+    s = setup
+    acc, rec = s["acc"], s["rec"]
+    acc.upload(rec)
+    nocc, nout = acc.sort()
+    got = acc.download()
+    perm, occ, bnd = orc.cellsort(rec, s["fine"].axis.nknots)
+    want = rec[perm]
+    assert got.tobytes() == want.tobytes()
+    assert nocc == len(occ)
+    occ_d, bnd_d, out_d = acc.sort_lists(nocc, nout)
+    assert np.array_equal(occ_d, occ + 1)
+    assert np.array_equal(bnd_d, bnd)
+    outl = orc.outside(want, s["fine"].n)
+    assert nout == len(outl) and np.array_equal(out_d, outl + 1)
+    # sorting a sorted cloud moves nothing
+    acc.sort()
+    assert acc.download().tobytes() == want.tobytes()
+
+
+def test_columns_bit_exact(setup, orc):
+    s = setup
+    acc = s["acc"]
+    acc.upload(s["rec"])
+    acc.sort()
+    srec = acc.download()
+    cols, nout = acc.columns()
+    ost = s["ost"]
+    wc, wn = orc.columns(srec, np.float32(ost.x0), np.float32(ost.h), ost.lo, ost.hi, ost.spacing,
+                         s["sm"].nbdt, s["sm"].nbdt + 1)
+    assert nout == wn and np.array_equal(cols, wc)
+
+
+def test_fine_deposit(setup, orc):
+    s = setup
+    acc, dmf = s["acc"], s["dmf"]
+    acc.upload(s["rec"])
+    nout = acc.deposit_smoothed(dmf, s["w"])
+    rho = dmf.rho
+    srec = acc.download()
+    perm, occ, bnd = orc.cellsort(srec, s["fine"].axis.nknots)
+    assert np.array_equal(perm, np.arange(NPART))
+    want, wn = s["ost"].deposit_fine(srec, occ, bnd)
+    assert nout == wn
+    assert relerr(rho, want) < REL
+    # renormalised charge, and deterministic to the bit
+    q = dmf.total_charge()
+    assert abs(q - (NPART - nout) * s["w"]) < 1e-5 * (NPART - nout) * s["w"]
+    acc.deposit_smoothed(dmf, s["w"])
+    assert dmf.rho.tobytes() == rho.tobytes()
+
+
+def test_coarse_deposit(setup, orc):
+    s = setup
+    acc, dmc = s["acc"], s["dmc"]
+    acc.upload(s["rec"])
+    acc.sort()
+    acc.deposit_cic(dmc, s["w"])
+    rho = dmc.rho
+    want = s["ost"].deposit_coarse(acc.download())
+    assert relerr(rho, want) < REL
+    acc.deposit_cic(dmc, s["w"])
+    assert dmc.rho.tobytes() == rho.tobytes()
+
+
+def test_poisson_chain(setup, orc):
+    """nested poisson!, spline_coefficients! and effective_potential! on the
+    oracle's own densities (narrowed to Float32)."""
+    s = setup
+    fine, coarse, dmf, dmc = s["fine"], s["coarse"], s["dmf"], s["dmc"]
+    acc = s["acc"]
+    acc.upload(s["rec"])
+    acc.sort()
+    srec = acc.download()
+    perm, occ, bnd = orc.cellsort(srec, fine.axis.nknots)
+    rho_f, _ = s["ost"].deposit_fine(srec, occ, bnd)
+    rho_c = s["ost"].deposit_coarse(srec)
+    rho_f32, rho_c32 = rho_f.astype(np.float32), rho_c.astype(np.float32)
+    dmf.upload(va._lib.VH_GRID_RHO, rho_f32)
+    dmc.upload(va._lib.VH_GRID_RHO, rho_c32)
+    # multipole moments in FP64
+    mp = dmc.multipole()
+    want_mp = orc.multipole(orc.moments(rho_c32.astype(np.float64), coarse.dual_moments))
+    assert np.allclose(mp, want_mp, rtol=1e-9, atol=1e-9 * abs(want_mp[0]))
+    va.poisson_nested(dmf, dmc)
+    phi_f, phi_c, csol_c = dmf.phi, dmc.phi, dmc.csol
+    wf, wc, _ = orc.poisson_nested(rho_f32.astype(np.float64), rho_c32.astype(np.float64), fine, coarse)
+    assert relerr(phi_c, wc) < REL
+    assert relerr(phi_f, wf) < REL
+    assert relerr(csol_c, orc.spline_coefficients(wc, coarse)) < REL
+    dmf.spline_coefficients()
+    csol_f = dmf.csol
+    want_csol_f = orc.spline_coefficients(wf, fine)
+    assert relerr(csol_f, want_csol_f) < REL
+    dmf.effective_potential(s["jel"])
+    want_eff = orc.effective_potential(want_csol_f, rho_f32.astype(np.float64), fine, s["jel"])
+    assert relerr(dmf.csol, want_eff) < REL
+
+
+def test_spline_coefficients_mfma_layout(setup, orc):
+    """The rotating product on a random grid, FP32 MFMA vs FP64."""
+    s = setup
+    rng = np.random.default_rng(11)
+    g = rng.standard_normal(s["fine"].n ** 3).astype(np.float32)
+    s["dmf"].upload(va._lib.VH_GRID_PHI, g)
+    s["dmf"].spline_coefficients()
+    want = orc.spline_coefficients(g.astype(np.float64), s["fine"])
+    assert relerr(s["dmf"].csol, want) < REL
+
+
+def _forces_setup(s, orc, seed=5):
+    rng = np.random.default_rng(seed)
+    fine, coarse = s["fine"], s["coarse"]
+    acc = s["acc"]
+    acc.upload(s["rec"])
+    acc.sort()
+    srec = acc.download()
+    perm, occ, bnd = orc.cellsort(srec, fine.axis.nknots)
+    # smooth potentials: a cluster-like well plus noise, as coefficients
+    csol_f = (rng.standard_normal(fine.n ** 3) * 0.01 + 1.0).astype(np.float32)
+    csol_c = (rng.standard_normal(coarse.n ** 3) * 0.01 + 1.0).astype(np.float32)
+    s["dmf"].upload(va._lib.VH_GRID_CSOL, csol_f)
+    s["dmc"].upload(va._lib.VH_GRID_CSOL, csol_c)
+    return srec, occ, bnd, csol_f, csol_c
+
+
+def test_field_contraction_bit_exact(setup, orc):
+    s = setup
+    srec, occ, bnd, csol_f, csol_c = _forces_setup(s, orc)
+    sm, ost = s["sm"], s["ost"]
+    nout, red = s["acc"].compute_forces(s["dmf"], s["dmc"], s["w"], None)
+    f = s["acc"].forces()
+    want, _ = orc.field(srec, occ, bnd, ost.nk, s["fine"].n, csol_f.astype(np.float64),
+                        sm.overlap, sm.gradient, ost.x0, ost.h, ost.spacing, sm.nbdt, sm.nbdt + 1,
+                        s["w"], None, 0, 0.0, f32=True)
+    inside = ~np.isnan(want[:, 0])
+    assert inside.sum() > 0.9 * NPART
+    assert np.array_equal(f[inside].view(np.uint32), want[inside].astype(np.float32).view(np.uint32))
+    # the others took the coarse fallback
+    outl = orc.outside(srec, s["fine"].n)
+    assert nout == len(outl) and np.array_equal(np.nonzero(~inside)[0], outl)
+    wfb = orc.fallback(srec, outl, ost.x0, ost.h, s["coarse"].axis.knots, csol_c.astype(np.float64),
+                       s["w"], None)
+    assert relerr(f[outl], wfb[outl]) < REL
+
+
+@pytest.mark.parametrize("mode", ["device", "ball"])
+def test_field_with_projectile(setup, orc, mode):
+    s = setup
+    srec, occ, bnd, csol_f, csol_c = _forces_setup(s, orc, seed=6)
+    proj = va.xenon_projectile(dt=0.5, x0=-20.0, impact=10.0)
+    nout, red = s["acc"].compute_forces(s["dmf"], s["dmc"], s["w"], proj, mode)
+    f = s["acc"].forces()
+    want, wred, _ = s["ost"].forces(srec, occ, bnd, csol_f.astype(np.float64),
+                                    csol_c.astype(np.float64), proj, mode)
+    assert relerr(f, want) < REL
+    assert np.allclose(red, wred, rtol=1e-5, atol=1e-6 * np.max(np.abs(wred)))
+
+
+def test_verlet_bit_exact(setup, orc):
+    s = setup
+    acc = s["acc"]
+    acc.upload(s["rec"])
+    rng = np.random.default_rng(9)
+    f = (rng.standard_normal((NPART, 3)) * 1e-3).astype(np.float32)
+    acc.upload_forces(f)
+    d = acc.verlet(0.5, s["w"], 100.0)
+    got = acc.download()
+    want = s["rec"].copy()
+    wd = orc.verlet_f32(want, f, 0.5, s["w"], s["h"], s["x0"], 100.0)
+    assert got.tobytes() == want.tobytes()
+    assert np.allclose([d["kinetic"], d["escaped"], *d["angular"]], wd, rtol=1e-10, atol=1e-12)
+
+
+def test_enclosed_count(setup, orc):
+    s = setup
+    acc = s["acc"]
+    acc.upload(s["rec"])
+    c = np.array([3.0, -2.0, 1.0])
+    assert acc.enclosed_count(c, 8.0) == orc.enclosed(s["rec"], s["x0"], s["h"], c, 8.0)
+
+
+def test_bad_arguments_refused_before_launch(setup):
+    s = setup
+    with pytest.raises(va.VlasovHIPError):
+        va._lib.call("vh_forces", s["acc"].handle, s["dmf"].handle, s["dmc"].handle, 1.0, None, 7, 0.0,
+                     None, None)
+    with pytest.raises(ValueError):
+        s["acc"].upload(np.zeros(3, dtype=va.PACKED_DTYPE))

@@ -1,0 +1,794 @@
+// Context, meshes, accelerators, buffers and the public ABI wrappers.
+#include "vh_internal.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+static thread_local char g_err[1024] = "";
+
+void vh_set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+extern "C" const char *vh_last_error(void) { return g_err; }
+extern "C" const char *vh_version(void) { return "vlasov_hip 0.1 (gfx950)"; }
+
+extern "C" int vh_device_count(int *count) {
+  VH_REQUIRE(count != nullptr, "vh_device_count: null output");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  *count = c;
+  return VH_OK;
+}
+
+int vh_alloc(void **p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  VH_CHECK_HIP(hipMalloc(p, bytes));
+  return VH_OK;
+}
+
+int vh_free(void *p) {
+  if (p) VH_CHECK_HIP(hipFree(p));
+  return VH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fills (kernels rather than hipMemsetAsync so that a captured step stays valid)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_fill(T *p, int64_t n, T v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+template <typename T>
+static int fill_impl(vh_context *c, T *p, int64_t n, T v) {
+  if (n <= 0) return VH_OK;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)blocks), dim3(256), 0, c->stream, p, n, v);
+  VH_LAUNCH_CHECK();
+  return VH_OK;
+}
+int vh_fill_i32(vh_context *c, int32_t *p, int64_t n, int32_t v) { return fill_impl(c, p, n, v); }
+int vh_fill_u64(vh_context *c, uint64_t *p, int64_t n, uint64_t v) { return fill_impl(c, p, n, v); }
+int vh_fill_f32(vh_context *c, float *p, int64_t n, float v) { return fill_impl(c, p, n, v); }
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" int vh_context_create(int device, vh_context **out) {
+  VH_REQUIRE(out != nullptr, "vh_context_create: null output");
+  *out = nullptr;
+  int count = 0;
+  VH_CHECK_HIP(hipGetDeviceCount(&count));
+  VH_REQUIRE(device >= 0 && device < count, "vh_context_create: device %d of %d", device, count);
+  VH_CHECK_HIP(hipSetDevice(device));
+  vh_context *c = new vh_context();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    vh_set_error("hipStreamCreate: %s", hipGetErrorString(e));
+    return VH_EHIP;
+  }
+  hipDeviceProp_t prop;
+  VH_CHECK_HIP(hipGetDeviceProperties(&prop, device));
+  c->num_cus = prop.multiProcessorCount;
+  *out = c;
+  return VH_OK;
+}
+
+extern "C" int vh_context_destroy(vh_context *ctx) {
+  if (!ctx) return VH_OK;
+  VH_CHECK_HIP(hipSetDevice(ctx->device));
+  VH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  VH_CHECK_HIP(hipStreamDestroy(ctx->stream));
+  delete ctx;
+  return VH_OK;
+}
+
+extern "C" int vh_sync(vh_context *ctx) {
+  VH_REQUIRE(ctx != nullptr, "vh_sync: null context");
+  VH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  return VH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// upload helpers
+// ---------------------------------------------------------------------------
+template <typename T>
+static int upload_converted(vh_context *c, T **dst, const double *src, size_t n) {
+  std::vector<T> tmp(n);
+  for (size_t i = 0; i < n; ++i) tmp[i] = (T)src[i];
+  VH_TRY(vh_alloc((void **)dst, n * sizeof(T)));
+  VH_CHECK_HIP(hipMemcpy(*dst, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  return VH_OK;
+}
+
+// column-major (Julia) n x n double matrix -> row-major float
+static int upload_rowmajor(vh_context *c, float **dst, const double *colmajor, int n) {
+  std::vector<double> t((size_t)n * n);
+  for (int r = 0; r < n; ++r)
+    for (int cc = 0; cc < n; ++cc) t[(size_t)r * n + cc] = colmajor[r + (size_t)n * cc];
+  return upload_converted(c, dst, t.data(), t.size());
+}
+
+// ---------------------------------------------------------------------------
+// mesh
+// ---------------------------------------------------------------------------
+__global__ void k_invlsum(float *out, const double *lam, int ni) {
+  int64_t tot = (int64_t)ni * ni * ni;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int i = (int)(t % ni), j = (int)((t / ni) % ni), k = (int)(t / ((int64_t)ni * ni));
+    double s = (lam[i] + lam[j]) + lam[k];
+    out[t] = (float)(1.0 / s);
+  }
+}
+
+extern "C" int vh_mesh_destroy(vh_mesh *m) {
+  if (!m) return VH_OK;
+  hipSetDevice(m->ctx->device);
+  hipStreamSynchronize(m->ctx->stream);
+  void *ptrs[] = {m->colloc, m->knots, m->colloc64, m->knots64, m->lap, m->Minv_rm, m->M_rm,
+                  m->Sinv_rm, m->invlsum, m->dual, m->dual32, m->duallen, m->loc_cells,
+                  m->rho, m->phi, m->csol, m->extra, m->work, m->inner, m->work1, m->work2,
+                  m->partials, m->scal, m->acc64};
+  for (void *p : ptrs) vh_free(p);
+  delete m;
+  return VH_OK;
+}
+
+extern "C" int vh_mesh_create(vh_context *ctx, int n, int nk, const double *colloc,
+                              const double *knots, const double *lap, const double *M,
+                              const double *Minv, const double *lam, const double *Sinv,
+                              const double *dual, const double *duallen, double loc_x0,
+                              double loc_invwidth, const int32_t *loc_cells, int nbin,
+                              vh_mesh **out) {
+  VH_REQUIRE(ctx && out, "vh_mesh_create: null argument");
+  VH_REQUIRE(n == 2 * nk && nk >= 4, "vh_mesh_create: n (%d) must equal 2 nk (nk=%d >= 4)", n, nk);
+  VH_REQUIRE(n <= 1024, "vh_mesh_create: n = %d exceeds 1024", n);
+  VH_REQUIRE(colloc && knots && lap && M && Minv && lam && Sinv && dual && duallen && loc_cells,
+             "vh_mesh_create: null table");
+  VH_REQUIRE(nbin >= 1, "vh_mesh_create: empty locate table");
+  for (int b = 0; b < nbin; ++b)
+    VH_REQUIRE(loc_cells[b] >= 1 && loc_cells[b] <= n - 1, "vh_mesh_create: locate cell out of range");
+  VH_CHECK_HIP(hipSetDevice(ctx->device));
+  vh_mesh *m = new vh_mesh();
+  memset(m, 0, sizeof *m);
+  m->ctx = ctx;
+  m->n = n;
+  m->nk = nk;
+  m->ni = n - 2;
+  int ni = m->ni;
+  int r = VH_OK;
+#define MK(x)                 \
+  if ((r = (x)) != VH_OK) {   \
+    vh_mesh_destroy(m);       \
+    return r;                 \
+  }
+  MK(upload_converted(ctx, &m->colloc, colloc, n));
+  MK(upload_converted(ctx, &m->knots, knots, nk));
+  MK(upload_converted(ctx, &m->colloc64, colloc, n));
+  MK(upload_converted(ctx, &m->knots64, knots, nk));
+  MK(upload_rowmajor(ctx, &m->lap, lap, n));
+  MK(upload_rowmajor(ctx, &m->Minv_rm, Minv, ni));
+  MK(upload_rowmajor(ctx, &m->M_rm, M, ni));
+  MK(upload_rowmajor(ctx, &m->Sinv_rm, Sinv, n));
+  MK(upload_converted(ctx, &m->dual, dual, 3 * (size_t)n));
+  MK(upload_converted(ctx, &m->dual32, dual, 3 * (size_t)n));
+  MK(upload_converted(ctx, &m->duallen, duallen, n));
+  m->loc_x0 = (float)loc_x0;
+  m->loc_iw = (float)loc_invwidth;
+  m->nbin = nbin;
+  MK(vh_alloc((void **)&m->loc_cells, sizeof(int32_t) * nbin));
+  if (hipMemcpy(m->loc_cells, loc_cells, sizeof(int32_t) * nbin, hipMemcpyHostToDevice) != hipSuccess) {
+    vh_mesh_destroy(m);
+    vh_set_error("vh_mesh_create: copy of the locate table failed");
+    return VH_EHIP;
+  }
+  size_t g = (size_t)n * n * n, gi = (size_t)ni * ni * ni;
+  MK(vh_alloc((void **)&m->rho, g * 4));
+  MK(vh_alloc((void **)&m->phi, g * 4));
+  MK(vh_alloc((void **)&m->csol, g * 4));
+  MK(vh_alloc((void **)&m->extra, g * 4));
+  MK(vh_alloc((void **)&m->work, g * 4));
+  MK(vh_alloc((void **)&m->inner, gi * 4));
+  MK(vh_alloc((void **)&m->work1, gi * 4));
+  MK(vh_alloc((void **)&m->work2, gi * 4));
+  MK(vh_alloc((void **)&m->invlsum, gi * 4));
+  MK(vh_alloc((void **)&m->partials, sizeof(double) * 10 * (size_t)n * n));
+  MK(vh_alloc((void **)&m->scal, sizeof(double) * 32));
+  MK(vh_alloc((void **)&m->acc64, g * 8));
+  {
+    double *dlam = nullptr;
+    MK(upload_converted(ctx, &dlam, lam, ni));
+    hipLaunchKernelGGL(k_invlsum, dim3(1024), dim3(256), 0, ctx->stream, m->invlsum, dlam, ni);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    hipFree(dlam);
+    if (e != hipSuccess) {
+      vh_mesh_destroy(m);
+      vh_set_error("vh_mesh_create: %s", hipGetErrorString(e));
+      return VH_EHIP;
+    }
+  }
+  MK(vh_fill_f32(ctx, m->rho, (int64_t)g, 0.f));
+  MK(vh_fill_f32(ctx, m->phi, (int64_t)g, 0.f));
+  MK(vh_fill_f32(ctx, m->csol, (int64_t)g, 0.f));
+#undef MK
+  VH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  *out = m;
+  return VH_OK;
+}
+
+static float *mesh_grid(vh_mesh *m, int which) {
+  switch (which) {
+    case VH_GRID_RHO: return m->rho;
+    case VH_GRID_PHI: return m->phi;
+    case VH_GRID_CSOL: return m->csol;
+    default: return nullptr;
+  }
+}
+
+extern "C" int vh_mesh_grid_download(vh_mesh *m, int which, float *host) {
+  VH_REQUIRE(m && host, "vh_mesh_grid_download: null argument");
+  float *g = mesh_grid(m, which);
+  VH_REQUIRE(g != nullptr, "vh_mesh_grid_download: unknown grid %d", which);
+  VH_CHECK_HIP(hipSetDevice(m->ctx->device));
+  size_t bytes = (size_t)m->n * m->n * m->n * 4;
+  VH_CHECK_HIP(hipMemcpyAsync(host, g, bytes, hipMemcpyDeviceToHost, m->ctx->stream));
+  VH_CHECK_HIP(hipStreamSynchronize(m->ctx->stream));
+  return VH_OK;
+}
+
+extern "C" int vh_mesh_grid_upload(vh_mesh *m, int which, const float *host) {
+  VH_REQUIRE(m && host, "vh_mesh_grid_upload: null argument");
+  float *g = mesh_grid(m, which);
+  VH_REQUIRE(g != nullptr, "vh_mesh_grid_upload: unknown grid %d", which);
+  VH_CHECK_HIP(hipSetDevice(m->ctx->device));
+  size_t bytes = (size_t)m->n * m->n * m->n * 4;
+  VH_CHECK_HIP(hipMemcpyAsync(g, host, bytes, hipMemcpyHostToDevice, m->ctx->stream));
+  VH_CHECK_HIP(hipStreamSynchronize(m->ctx->stream));
+  return VH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// accelerator
+// ---------------------------------------------------------------------------
+extern "C" int vh_accel_destroy(vh_accel *a) {
+  if (!a) return VH_OK;
+  hipSetDevice(a->ctx->device);
+  hipStreamSynchronize(a->ctx->stream);
+  void *ptrs[] = {a->ovl, a->grad, a->nodes, a->parts[0], a->parts[1], a->force,
+                  a->keys[0], a->keys[1], a->vals[0], a->vals[1], a->counts, a->scan_tmp,
+                  a->occupied, a->bounds, a->cellmap, a->fitlist, a->nonfit, a->nonfit_prefix,
+                  a->outlist, a->counters, a->flags, a->tiles, a->dpart, a->dscal, a->cellscratch};
+  for (void *p : ptrs) vh_free(p);
+  delete a;
+  return VH_OK;
+}
+
+extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const double *knots,
+                               double spacing, int nbdt, const double *overlap,
+                               const double *gradient, const double *nodes, vh_accel **out) {
+  VH_REQUIRE(ctx && out && knots && overlap && gradient && nodes, "vh_accel_create: null argument");
+  VH_REQUIRE(npart >= 1 && npart <= ((int64_t)1 << 31) - 4096,
+             "vh_accel_create: npart = %lld out of range", (long long)npart);
+  VH_REQUIRE(nk >= 6 && nk <= 512, "vh_accel_create: nk = %d out of range", nk);
+  VH_REQUIRE(nbdt >= 1, "vh_accel_create: nbdt must be positive");
+  const double x0 = knots[0];
+  const double h = (knots[nk - 1] - knots[0]) / (nk - 1);
+  VH_REQUIRE(h > 0 && std::isfinite(h) && std::isfinite(x0), "vh_accel_create: bad axis");
+  for (int j = 0; j + 1 < nk; ++j)
+    VH_REQUIRE(std::fabs((knots[j + 1] - knots[j]) - h) <= 1e-9 * std::fabs(h),
+               "vh_accel_create: the accelerated path assumes a uniform fine grid");
+  VH_REQUIRE(spacing > 0 && std::isfinite(spacing), "vh_accel_create: bad smoothing spacing");
+  *out = nullptr;
+  VH_CHECK_HIP(hipSetDevice(ctx->device));
+  vh_accel *a = new vh_accel();
+  memset(a, 0, sizeof *a);
+  a->ctx = ctx;
+  a->npart = npart;
+  a->npad = (npart + 255) / 256 * 256;
+  a->nk = nk;
+  a->n = 2 * nk;
+  a->ncell = nk * nk * nk;
+  a->x0 = x0;
+  a->h = h;
+  a->spacing = (float)spacing;
+  a->lo = (float)(knots[1] + spacing / 2);
+  a->hi = (float)(knots[nk - 2] - spacing / 2);
+  a->nbdt = nbdt;
+  a->ncol = nbdt + 1;
+  // radix digits: ceil(log2(ncell)) bits in passes of at most 11
+  int bits = 1;
+  while (((int64_t)1 << bits) < a->ncell) ++bits;
+  a->bits_total = bits;
+  a->passes = (bits + 10) / 11;
+  a->digit_bits = (bits + a->passes - 1) / a->passes;
+  a->nbins = 1 << a->digit_bits;
+  a->tile = 4096;
+  a->nblocks = (npart + a->tile - 1) / a->tile;
+  // coarse CIC fixed point: a point's sum is at most npart
+  int lg = 0;
+  while (((int64_t)1 << lg) <= npart) ++lg;
+  a->cic_shift = 61 - lg;
+  if (a->cic_shift > 40) a->cic_shift = 40;
+  int r = VH_OK;
+#define MK(x)                 \
+  if ((r = (x)) != VH_OK) {   \
+    vh_accel_destroy(a);      \
+    return r;                 \
+  }
+  // tables: the reference's column-major (a + 10 col) layout is kept
+  MK(upload_converted(ctx, &a->ovl, overlap, (size_t)10 * a->ncol));
+  MK(upload_converted(ctx, &a->grad, gradient, (size_t)10 * a->ncol));
+  MK(upload_converted(ctx, &a->nodes, nodes, (size_t)8 * a->ncol));
+  size_t np = (size_t)a->npad;
+  MK(vh_alloc((void **)&a->parts[0], np * 4 * PS_COUNT));
+  MK(vh_alloc((void **)&a->parts[1], np * 4 * PS_COUNT));
+  MK(vh_alloc((void **)&a->force, np * 4 * 3));
+  MK(vh_alloc((void **)&a->keys[0], np * 4));
+  MK(vh_alloc((void **)&a->keys[1], np * 4));
+  MK(vh_alloc((void **)&a->vals[0], np * 4));
+  MK(vh_alloc((void **)&a->vals[1], np * 4));
+  MK(vh_alloc((void **)&a->counts, (size_t)a->nbins * a->nblocks * 4 + 64));
+  a->scan_tmp_len = ((int64_t)a->nbins * a->nblocks + np) / 1024 + 4096;
+  MK(vh_alloc((void **)&a->scan_tmp, (size_t)a->scan_tmp_len * 4));
+  int64_t maxocc = a->ncell < npart ? a->ncell : npart;
+  a->maxocc = maxocc;
+  MK(vh_alloc((void **)&a->cellscratch, sizeof(uint32_t) * 3 * (maxocc + 2)));
+  MK(vh_alloc((void **)&a->occupied, sizeof(int32_t) * (maxocc + 1)));
+  MK(vh_alloc((void **)&a->bounds, sizeof(int32_t) * (maxocc + 2)));
+  MK(vh_alloc((void **)&a->cellmap, sizeof(int32_t) * a->ncell));
+  MK(vh_alloc((void **)&a->fitlist, sizeof(int32_t) * (maxocc + 1)));
+  MK(vh_alloc((void **)&a->nonfit, sizeof(int32_t) * (maxocc + 1)));
+  MK(vh_alloc((void **)&a->nonfit_prefix, sizeof(int32_t) * (maxocc + 2)));
+  MK(vh_alloc((void **)&a->outlist, np * 4));
+  MK(vh_alloc((void **)&a->counters, sizeof(int32_t) * 64));
+  MK(vh_alloc((void **)&a->flags, (np + 1) * 4));
+  MK(vh_alloc((void **)&a->tiles, sizeof(float) * 512 * (size_t)maxocc));
+  a->dpart_len = 1 << 16;
+  MK(vh_alloc((void **)&a->dpart, sizeof(double) * 8 * a->dpart_len));
+  MK(vh_alloc((void **)&a->dscal, sizeof(double) * 64));
+  MK(vh_fill_i32(ctx, a->counters, 64, 0));
+  MK(vh_fill_f32(ctx, a->force, (int64_t)np * 3, 0.f));
+  MK(vh_fill_i32(ctx, a->parts[0], (int64_t)np * PS_COUNT, 0));
+#undef MK
+  VH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  a->cur = 0;
+  *out = a;
+  return VH_OK;
+}
+
+// AoS record <-> SoA streams
+__global__ void k_aos_to_soa(const int32_t *__restrict__ rec, int32_t *__restrict__ soa, int64_t n,
+                             int64_t npad) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t *r = rec + 12 * i;
+#pragma unroll
+    for (int s = 0; s < 12; ++s) soa[s * npad + i] = r[s];
+  }
+}
+
+__global__ void k_soa_to_aos(const int32_t *__restrict__ soa, int32_t *__restrict__ rec, int64_t n,
+                             int64_t npad) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t *r = rec + 12 * i;
+#pragma unroll
+    for (int s = 0; s < 12; ++s) r[s] = soa[s * npad + i];
+  }
+}
+
+__global__ void k_interleave3(const float *__restrict__ f, float *__restrict__ out, int64_t n,
+                              int64_t npad, int dir) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (dir == 0) {
+      out[3 * i] = f[i];
+      out[3 * i + 1] = f[npad + i];
+      out[3 * i + 2] = f[2 * npad + i];
+    } else {
+      out[i] = f[3 * i];
+      out[npad + i] = f[3 * i + 1];
+      out[2 * npad + i] = f[3 * i + 2];
+    }
+  }
+}
+
+extern "C" int vh_particles_upload(vh_accel *a, const void *records) {
+  VH_REQUIRE(a && records, "vh_particles_upload: null argument");
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  int32_t *stage = a->parts[1 - a->cur];
+  VH_CHECK_HIP(hipMemcpyAsync(stage, records, (size_t)a->npart * 48, hipMemcpyHostToDevice,
+                              a->ctx->stream));
+  hipLaunchKernelGGL(k_aos_to_soa, dim3(2048), dim3(256), 0, a->ctx->stream, stage,
+                     a->parts[a->cur], a->npart, a->npad);
+  VH_LAUNCH_CHECK();
+  VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
+  return VH_OK;
+}
+
+extern "C" int vh_particles_download(vh_accel *a, void *records) {
+  VH_REQUIRE(a && records, "vh_particles_download: null argument");
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  int32_t *stage = a->parts[1 - a->cur];
+  hipLaunchKernelGGL(k_soa_to_aos, dim3(2048), dim3(256), 0, a->ctx->stream, a->parts[a->cur],
+                     stage, a->npart, a->npad);
+  VH_LAUNCH_CHECK();
+  VH_CHECK_HIP(hipMemcpyAsync(records, stage, (size_t)a->npart * 48, hipMemcpyDeviceToHost,
+                              a->ctx->stream));
+  VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
+  return VH_OK;
+}
+
+extern "C" int vh_forces_download(vh_accel *a, float *host) {
+  VH_REQUIRE(a && host, "vh_forces_download: null argument");
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  float *stage = (float *)a->parts[1 - a->cur];
+  hipLaunchKernelGGL(k_interleave3, dim3(2048), dim3(256), 0, a->ctx->stream, a->force, stage,
+                     a->npart, a->npad, 0);
+  VH_LAUNCH_CHECK();
+  VH_CHECK_HIP(hipMemcpyAsync(host, stage, (size_t)a->npart * 12, hipMemcpyDeviceToHost,
+                              a->ctx->stream));
+  VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
+  return VH_OK;
+}
+
+extern "C" int vh_forces_upload(vh_accel *a, const float *host) {
+  VH_REQUIRE(a && host, "vh_forces_upload: null argument");
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  float *stage = (float *)a->parts[1 - a->cur];
+  VH_CHECK_HIP(hipMemcpyAsync(stage, host, (size_t)a->npart * 12, hipMemcpyHostToDevice,
+                              a->ctx->stream));
+  hipLaunchKernelGGL(k_interleave3, dim3(2048), dim3(256), 0, a->ctx->stream, stage, a->force,
+                     a->npart, a->npad, 1);
+  VH_LAUNCH_CHECK();
+  VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
+  return VH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// region timing
+// ---------------------------------------------------------------------------
+static hipEvent_t take_event(vh_context *c) {
+  if (!c->prof_pool.empty()) {
+    hipEvent_t e = c->prof_pool.back();
+    c->prof_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int vh_prof_begin(vh_context *c, int stage) {
+  if (!c->prof_on) return -1;
+  vh_prof_rec r;
+  r.stage = stage;
+  r.start = take_event(c);
+  r.stop = take_event(c);
+  if (!r.start || !r.stop) return -1;
+  hipEventRecord(r.start, c->stream);
+  c->prof_open.push_back(r);
+  return (int)c->prof_open.size() - 1;
+}
+
+void vh_prof_end(vh_context *c, int token) {
+  if (token < 0 || token >= (int)c->prof_open.size()) return;
+  hipEventRecord(c->prof_open[token].stop, c->stream);
+}
+
+extern "C" int vh_profile_enable(vh_context *ctx, int on) {
+  VH_REQUIRE(ctx != nullptr, "vh_profile_enable: null context");
+  ctx->prof_on = on ? 1 : 0;
+  return VH_OK;
+}
+
+extern "C" int vh_profile_read(vh_context *ctx, double *ms, int64_t *count) {
+  VH_REQUIRE(ctx && ms && count, "vh_profile_read: null argument");
+  VH_CHECK_HIP(hipSetDevice(ctx->device));
+  VH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto &r : ctx->prof_open) {
+    float t = 0.f;
+    VH_CHECK_HIP(hipEventElapsedTime(&t, r.start, r.stop));
+    if (r.stage >= 0 && r.stage < VH_ST_COUNT) {
+      ms[r.stage] += t;
+      count[r.stage] += 1;
+    }
+    ctx->prof_pool.push_back(r.start);
+    ctx->prof_pool.push_back(r.stop);
+  }
+  ctx->prof_open.clear();
+  return VH_OK;
+}
+
+struct vh_region {
+  vh_context *c;
+  int tok;
+  vh_region(vh_context *ctx, int stage) : c(ctx), tok(vh_prof_begin(ctx, stage)) {}
+  ~vh_region() { vh_prof_end(c, tok); }
+};
+
+// ---------------------------------------------------------------------------
+// public wrappers of the step's stages
+// ---------------------------------------------------------------------------
+static int read_counters(vh_accel *a, int32_t *h) {
+  VH_CHECK_HIP(hipMemcpyAsync(h, a->counters, sizeof(int32_t) * CNT_COUNT, hipMemcpyDeviceToHost,
+                              a->ctx->stream));
+  VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
+  return VH_OK;
+}
+
+static int sort_timed(vh_accel *a) {
+  vh_region r(a->ctx, VH_ST_SORT);
+  VH_TRY(vh_sort_impl(a));
+  a->sorted_once = 1;
+  return VH_OK;
+}
+
+extern "C" int vh_sort(vh_accel *a, int32_t *n_occupied, int32_t *outcount) {
+  VH_REQUIRE(a != nullptr, "vh_sort: null accelerator");
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  VH_TRY(sort_timed(a));
+  if (n_occupied || outcount) {
+    int32_t h[CNT_COUNT];
+    VH_TRY(read_counters(a, h));
+    if (n_occupied) *n_occupied = h[CNT_NOCC];
+    if (outcount) *outcount = h[CNT_OUTCOUNT];
+  }
+  return VH_OK;
+}
+
+__global__ void k_lists_export(const int32_t *__restrict__ occ, const int32_t *__restrict__ bounds,
+                               const int32_t *__restrict__ outl, const int32_t *__restrict__ cnt,
+                               int32_t *occ1, int32_t *bnd, int32_t *out1) {
+  int nocc = cnt[CNT_NOCC], nout = cnt[CNT_OUTCOUNT];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= nocc || i < nout;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (i < nocc) occ1[i] = occ[i] + 1;
+    if (i <= nocc) bnd[i] = bounds[i];
+    if (i < nout) out1[i] = outl[i] + 1;
+  }
+}
+
+extern "C" int vh_sort_lists(vh_accel *a, int32_t *occupied, int32_t *bounds, int32_t *outlist) {
+  VH_REQUIRE(a && occupied && bounds && outlist, "vh_sort_lists: null argument");
+  VH_REQUIRE(a->sorted_once, "vh_sort_lists: no sort has run");
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  int32_t h[CNT_COUNT];
+  VH_TRY(read_counters(a, h));
+  int nocc = h[CNT_NOCC], nout = h[CNT_OUTCOUNT];
+  int32_t *d = (int32_t *)a->parts[1 - a->cur];
+  int32_t *d_occ = d, *d_bnd = d + (nocc + 1), *d_out = d + 2 * (nocc + 1) + 1;
+  hipLaunchKernelGGL(k_lists_export, dim3(1024), dim3(256), 0, a->ctx->stream, a->occupied,
+                     a->bounds, a->outlist, a->counters, d_occ, d_bnd, d_out);
+  VH_LAUNCH_CHECK();
+  if (nocc) VH_CHECK_HIP(hipMemcpyAsync(occupied, d_occ, 4 * (size_t)nocc, hipMemcpyDeviceToHost, a->ctx->stream));
+  VH_CHECK_HIP(hipMemcpyAsync(bounds, d_bnd, 4 * (size_t)(nocc + 1), hipMemcpyDeviceToHost, a->ctx->stream));
+  if (nout) VH_CHECK_HIP(hipMemcpyAsync(outlist, d_out, 4 * (size_t)nout, hipMemcpyDeviceToHost, a->ctx->stream));
+  VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
+  return VH_OK;
+}
+
+// columns of src/kernels.jl:754-779, in the device's current order
+__global__ void k_columns(const int32_t *__restrict__ P, int64_t npad, int64_t n, float x0,
+                          float h, float lo, float hi, float spacing, float nbdt, int ncol,
+                          int32_t *__restrict__ cols, int32_t *cnt) {
+  int local = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float *D = (const float *)P;
+    float half = spacing * 0.5f;
+    bool inside = true;
+    for (int d = 0; d < 3; ++d) {
+      float p = x0 + (float)(P[d * npad + i] - 1) * h + D[(PS_DX + d) * npad + i];
+      inside = inside && (lo <= p) && (p <= hi);
+    }
+    if (inside) {
+      for (int d = 0; d < 3; ++d) {
+        float t = (D[(PS_DX + d) * npad + i] + half) / spacing * nbdt + 0.5f;
+        int c = (int)floorf(t) + 1;
+        c = c < 1 ? 1 : (c > ncol ? ncol : c);
+        cols[3 * i + d] = c;
+      }
+    } else {
+      cols[3 * i] = cols[3 * i + 1] = cols[3 * i + 2] = 1;
+      ++local;
+    }
+  }
+  local = vh_wave_sum(local);
+  if (vh_lane() == 0 && local) atomicAdd(cnt, local);
+}
+
+extern "C" int vh_columns(vh_accel *a, int32_t *cols, int32_t *nout) {
+  VH_REQUIRE(a && cols, "vh_columns: null argument");
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  int32_t *dcols = (int32_t *)a->parts[1 - a->cur];
+  VH_TRY(vh_fill_i32(a->ctx, a->counters + 32, 1, 0));
+  hipLaunchKernelGGL(k_columns, dim3(2048), dim3(256), 0, a->ctx->stream, a->parts[a->cur],
+                     a->npad, a->npart, (float)a->x0, (float)a->h, a->lo, a->hi,
+                     a->spacing, (float)a->nbdt, a->ncol, dcols, a->counters + 32);
+  VH_LAUNCH_CHECK();
+  VH_CHECK_HIP(hipMemcpyAsync(cols, dcols, (size_t)a->npart * 12, hipMemcpyDeviceToHost, a->ctx->stream));
+  int32_t hn = 0;
+  VH_CHECK_HIP(hipMemcpyAsync(&hn, a->counters + 32, 4, hipMemcpyDeviceToHost, a->ctx->stream));
+  VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
+  if (nout) *nout = hn;
+  return VH_OK;
+}
+
+extern "C" int vh_deposit_smoothed(vh_accel *a, vh_mesh *fine, double charge, int32_t *nout) {
+  VH_REQUIRE(a && fine, "vh_deposit_smoothed: null argument");
+  VH_REQUIRE(fine->n == a->n, "vh_deposit_smoothed: rho must cover the whole collocation grid "
+             "(mesh n=%d, accelerator n=%d)", fine->n, a->n);
+  VH_REQUIRE(fine->ctx == a->ctx, "vh_deposit_smoothed: mesh and accelerator on different contexts");
+  VH_REQUIRE(std::isfinite(charge), "vh_deposit_smoothed: charge must be finite");
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  VH_TRY(sort_timed(a));
+  {
+    vh_region r(a->ctx, VH_ST_DEPOSIT_FINE);
+    VH_TRY(vh_deposit_fine_impl(a, fine, charge));
+  }
+  if (nout) {
+    int32_t h[CNT_COUNT];
+    VH_TRY(read_counters(a, h));
+    *nout = h[CNT_NOUT];
+  }
+  return VH_OK;
+}
+
+extern "C" int vh_deposit_cic(vh_accel *a, vh_mesh *coarse, double charge) {
+  VH_REQUIRE(a && coarse, "vh_deposit_cic: null argument");
+  VH_REQUIRE(coarse->ctx == a->ctx, "vh_deposit_cic: mesh and accelerator on different contexts");
+  VH_REQUIRE(std::isfinite(charge), "vh_deposit_cic: charge must be finite");
+  if (!a->sorted_once) {
+    vh_set_error("vh_deposit_cic: the cloud has not been sorted");
+    return VH_ESTATE;
+  }
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  vh_region r(a->ctx, VH_ST_DEPOSIT_CIC);
+  VH_TRY(vh_deposit_cic_impl(a, coarse, charge));
+  return VH_OK;
+}
+
+extern "C" int vh_poisson(vh_mesh *m) {
+  VH_REQUIRE(m != nullptr, "vh_poisson: null mesh");
+  VH_CHECK_HIP(hipSetDevice(m->ctx->device));
+  vh_region r(m->ctx, VH_ST_POISSON);
+  VH_TRY(vh_poisson_single_impl(m));
+  return VH_OK;
+}
+
+extern "C" int vh_poisson_nested(vh_mesh *fine, vh_mesh *coarse) {
+  VH_REQUIRE(fine && coarse, "vh_poisson_nested: null mesh");
+  VH_REQUIRE(fine->ctx == coarse->ctx, "vh_poisson_nested: meshes on different contexts");
+  VH_CHECK_HIP(hipSetDevice(fine->ctx->device));
+  vh_region r(fine->ctx, VH_ST_POISSON);
+  VH_TRY(vh_poisson_nested_impl(fine, coarse));
+  return VH_OK;
+}
+
+extern "C" int vh_spline_coefficients(vh_mesh *m) {
+  VH_REQUIRE(m != nullptr, "vh_spline_coefficients: null mesh");
+  VH_CHECK_HIP(hipSetDevice(m->ctx->device));
+  vh_region r(m->ctx, VH_ST_SPLINE);
+  VH_TRY(vh_spline_coefficients_impl(m, m->phi, m->csol));
+  return VH_OK;
+}
+
+extern "C" int vh_effective_potential(vh_mesh *m, double nions, double radius) {
+  VH_REQUIRE(m != nullptr, "vh_effective_potential: null mesh");
+  VH_REQUIRE(radius > 0, "vh_effective_potential: jellium radius must be positive");
+  VH_CHECK_HIP(hipSetDevice(m->ctx->device));
+  vh_region r(m->ctx, VH_ST_MEANFIELD);
+  VH_TRY(vh_effective_potential_impl(m, nions, radius));
+  return VH_OK;
+}
+
+extern "C" int vh_total_charge(vh_mesh *m, double *q) {
+  VH_REQUIRE(m && q, "vh_total_charge: null argument");
+  VH_CHECK_HIP(hipSetDevice(m->ctx->device));
+  VH_TRY(vh_total_charge_impl(m, m->rho, m->scal + 20));
+  VH_CHECK_HIP(hipMemcpyAsync(q, m->scal + 20, 8, hipMemcpyDeviceToHost, m->ctx->stream));
+  VH_CHECK_HIP(hipStreamSynchronize(m->ctx->stream));
+  return VH_OK;
+}
+
+int vh_multipole_impl(vh_mesh *m);
+extern "C" int vh_multipole(vh_mesh *m, double out10[10]) {
+  VH_REQUIRE(m && out10, "vh_multipole: null argument");
+  VH_CHECK_HIP(hipSetDevice(m->ctx->device));
+  VH_TRY(vh_multipole_impl(m));
+  VH_CHECK_HIP(hipMemcpyAsync(out10, m->scal + 10, 80, hipMemcpyDeviceToHost, m->ctx->stream));
+  VH_CHECK_HIP(hipStreamSynchronize(m->ctx->stream));
+  return VH_OK;
+}
+
+extern "C" int vh_forces(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight,
+                         const double *proj, int soft_mode, double ball_radius, double red[4],
+                         int32_t *nout) {
+  VH_REQUIRE(a && fine && coarse, "vh_forces: null argument");
+  VH_REQUIRE(fine->n == a->n, "vh_forces: fine csol has n=%d, accelerator n=%d", fine->n, a->n);
+  VH_REQUIRE(fine->ctx == a->ctx && coarse->ctx == a->ctx, "vh_forces: objects on different contexts");
+  VH_REQUIRE(soft_mode == VH_SOFT_DEVICE || soft_mode == VH_SOFT_BALL,
+             "vh_forces: unknown softening mode %d", soft_mode);
+  VH_REQUIRE(std::isfinite(weight), "vh_forces: weight must be finite");
+  if (proj) {
+    VH_REQUIRE(proj[4] > 0, "vh_forces: projectile scale must be positive");
+    VH_REQUIRE(std::isfinite(proj[0]) && std::isfinite(proj[1]) && std::isfinite(proj[2]) &&
+               std::isfinite(proj[3]), "vh_forces: projectile state must be finite");
+  }
+  if (!a->sorted_once) {
+    vh_set_error("vh_forces: the cloud has not been sorted");
+    return VH_ESTATE;
+  }
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  {
+    vh_region r(a->ctx, VH_ST_FIELD);
+    VH_TRY(vh_forces_impl(a, fine, coarse, weight, proj, soft_mode, ball_radius));
+  }
+  if (red) VH_CHECK_HIP(hipMemcpyAsync(red, a->dscal + 8, 32, hipMemcpyDeviceToHost, a->ctx->stream));
+  if (nout) VH_CHECK_HIP(hipMemcpyAsync(nout, a->counters + CNT_OUTCOUNT, 4, hipMemcpyDeviceToHost, a->ctx->stream));
+  if (red || nout) VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
+  return VH_OK;
+}
+
+extern "C" int vh_verlet(vh_accel *a, double dt, double mass, double rcmax, double diag[5]) {
+  VH_REQUIRE(a != nullptr, "vh_verlet: null accelerator");
+  VH_REQUIRE(dt > 0 && mass > 0, "vh_verlet: dt and mass must be positive");
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  {
+    vh_region r(a->ctx, VH_ST_VERLET);
+    VH_TRY(vh_verlet_impl(a, dt, mass, rcmax));
+  }
+  if (diag) {
+    VH_CHECK_HIP(hipMemcpyAsync(diag, a->dscal, 40, hipMemcpyDeviceToHost, a->ctx->stream));
+    VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
+  }
+  return VH_OK;
+}
+
+__global__ void k_enclosed(const int32_t *__restrict__ P, int64_t npad, int64_t n, double x0,
+                           double h, double cx, double cy, double cz, double r2,
+                           unsigned long long *out) {
+  unsigned long long local = 0;
+  const float *D = (const float *)P;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double px = x0 + (double)(P[PS_KX * npad + i] - 1) * h + (double)D[PS_DX * npad + i];
+    double py = x0 + (double)(P[PS_KY * npad + i] - 1) * h + (double)D[PS_DY * npad + i];
+    double pz = x0 + (double)(P[PS_KZ * npad + i] - 1) * h + (double)D[PS_DZ * npad + i];
+    double dx = px - cx, dy = py - cy, dz = pz - cz;
+    if ((dx * dx + dy * dy) + dz * dz < r2) ++local;
+  }
+  local = vh_wave_sum(local);
+  if (vh_lane() == 0 && local) atomicAdd(out, local);
+}
+
+extern "C" int vh_enclosed_count(vh_accel *a, const double center[3], double radius, int64_t *count) {
+  VH_REQUIRE(a && center && count, "vh_enclosed_count: null argument");
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  unsigned long long *d = (unsigned long long *)(a->dscal + 40);
+  VH_TRY(vh_fill_u64(a->ctx, (uint64_t *)d, 1, 0));
+  hipLaunchKernelGGL(k_enclosed, dim3(2048), dim3(256), 0, a->ctx->stream, a->parts[a->cur],
+                     a->npad, a->npart, a->x0, a->h, center[0], center[1], center[2],
+                     radius * radius, d);
+  VH_LAUNCH_CHECK();
+  unsigned long long h = 0;
+  VH_CHECK_HIP(hipMemcpyAsync(&h, d, 8, hipMemcpyDeviceToHost, a->ctx->stream));
+  VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
+  *count = (int64_t)h;
+  return VH_OK;
+}

@@ -1,0 +1,454 @@
+// Smoothed field at the pseudo-particles with the projectile fused in, the
+// out-of-stencil fallback, and the packed position Verlet.
+//
+// Field (src/kernels.jl:339-470).  One wave per occupied cell: the cell's
+// 10^3 block of csol is staged once in LDS (read by every lane at the same
+// address, i.e. broadcast), each lane takes one particle at a time, keeps
+// the 60 tabulated overlap/gradient values of its particle in registers and
+// runs contract_spline_10 (src/fields.jl:305-326) in the reference's fma
+// order.  The projectile's per-particle reaction and pair potential are
+// summed per lane in FP64 and reduced in a fixed order.
+//
+// Fallback (src/accelerator.jl:679-731).  The reference downloads the whole
+// force buffer and loops on the host over the particles whose stencil leaves
+// the fine grid; here one kernel walks the out-of-stencil list on the device
+// in FP64: the unsmoothed spline field of the coarse csol
+// (src/fields.jl:358-393) and the projectile's own law.
+//
+// Verlet (src/kernels.jl:985-1025, src/particles.jl:337-349): integer cell
+// part exact, FP32 offsets, diagnostics summed in FP64.
+#include "vh_internal.h"
+
+#include <cmath>
+
+// ---------------------------------------------------------------------------
+// projectile laws (src/projectile.jl:79-108), FP32 as in the device kernel
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float gaussian_force_kernel_f(float r2, float s) {
+  const float u2 = r2 / (s * s);
+  if (u2 <= 0.25f) {
+    const float c[7] = {(float)(1.0 / 3.0), (float)(-1.0 / 10.0), (float)(1.0 / 56.0),
+                        (float)(-1.0 / 432.0), (float)(1.0 / 4224.0), (float)(-1.0 / 49920.0),
+                        (float)(1.0 / 685440.0)};
+    float p = 0.f;
+    for (int k = 6; k >= 0; --k) p = c[k] + u2 * p;
+    return (float)0.79788456080286535588 * p / (s * s * s);
+  }
+  const float r = sqrtf(r2);
+  const float x = r / (sqrtf(2.0f) * s);
+  return (erff(x) - (float)0.79788456080286535588 * (r / s) * expf(-u2 / 2.0f)) / (r2 * r);
+}
+
+__device__ __forceinline__ float ball_force_kernel_f(float r2, float R) {
+  return r2 > R * R ? 1.0f / (r2 * sqrtf(r2)) : 1.0f / (R * R * R);
+}
+
+// FP64 versions for the fallback (the reference runs it on the host in T)
+__device__ __forceinline__ double gaussian_force_kernel_d(double r2, double s) {
+  const double u2 = r2 / (s * s);
+  if (u2 <= 0.25) {
+    const double c[7] = {1.0 / 3.0, -1.0 / 10.0, 1.0 / 56.0, -1.0 / 432.0,
+                         1.0 / 4224.0, -1.0 / 49920.0, 1.0 / 685440.0};
+    double p = 0.0;
+    for (int k = 6; k >= 0; --k) p = c[k] + u2 * p;
+    return 0.79788456080286535588 * p / (s * s * s);
+  }
+  const double r = sqrt(r2);
+  const double x = r / (sqrt(2.0) * s);
+  return (erf(x) - 0.79788456080286535588 * (r / s) * exp(-u2 / 2.0)) / (r2 * r);
+}
+
+__device__ __forceinline__ double ball_force_kernel_d(double r2, double R) {
+  return r2 > R * R ? 1.0 / (r2 * sqrt(r2)) : 1.0 / (R * R * R);
+}
+
+// ---------------------------------------------------------------------------
+// field kernel
+// ---------------------------------------------------------------------------
+struct vh_proj_args {
+  float px, py, pz;   // E.(projectile.position)
+  float coef;         // E(-w q), 0 without projectile
+  float sigma;        // E(scale(softening))
+  int ball;           // VH_SOFT_BALL
+  float radius;       // ball radius (ball mode)
+};
+
+__global__ void __launch_bounds__(64)
+    k_field(const int32_t *__restrict__ P, int64_t npad, const float *__restrict__ csol, int n,
+            const float *__restrict__ ovl, const float *__restrict__ grad,
+            const int32_t *__restrict__ occupied, const int32_t *__restrict__ bounds,
+            const int32_t *__restrict__ cnt, int nk, float x0, float h, float spacing, float nbdt,
+            float w, int ncol, vh_proj_args pa, float *__restrict__ force,
+            double *__restrict__ part) {
+  __shared__ __attribute__((aligned(16))) float tile[100 * 12];
+  const float *D = (const float *)P;
+  const int t = threadIdx.x;
+  const float half = spacing * 0.5f;
+  const float nan = __int_as_float(0x7fc00000);
+  const int nocc = cnt[CNT_NOCC];
+  double sax = 0.0, say = 0.0, saz = 0.0, sae = 0.0;
+  for (int g = blockIdx.x; g < nocc; g += gridDim.x) {
+    const int key = occupied[g];
+    const int kx = key % nk + 1, ky = (key / nk) % nk + 1, kz = key / (nk * nk) + 1;
+    const int bx = 2 * kx - 5, by = 2 * ky - 5, bz = 2 * kz - 5;  // 1-based stencil corner
+    const bool ok = bx >= 1 && by >= 1 && bz >= 1 && bx + 9 <= n && by + 9 <= n && bz + 9 <= n;
+    __syncthreads();
+    if (ok) {
+      for (int e = t; e < 1000; e += 64) {
+        const int ii = e % 10, jj = (e / 10) % 10, kk = e / 100;
+        tile[(kk * 10 + jj) * 12 + ii] =
+            csol[(bx - 1 + ii) + (int64_t)n * ((by - 1 + jj) + (int64_t)n * (bz - 1 + kk))];
+      }
+    }
+    __syncthreads();
+    const int lo = bounds[g], hi = bounds[g + 1];
+    for (int s = lo + t; s < hi; s += 64) {
+      const float dx0 = D[PS_DX * npad + s], dy0 = D[PS_DY * npad + s], dz0 = D[PS_DZ * npad + s];
+      float fxp = nan, fyp = nan, fzp = nan;
+      if (ok) {
+        int cx = (int)floorf((dx0 + half) / spacing * nbdt + 0.5f) + 1;
+        int cy = (int)floorf((dy0 + half) / spacing * nbdt + 0.5f) + 1;
+        int cz = (int)floorf((dz0 + half) / spacing * nbdt + 0.5f) + 1;
+        cx = min(max(cx, 1), ncol) - 1;
+        cy = min(max(cy, 1), ncol) - 1;
+        cz = min(max(cz, 1), ncol) - 1;
+        float ox[10], gx[10], oy[10], gy[10], oz[10], gz[10];
+#pragma unroll
+        for (int a = 0; a < 10; ++a) {
+          ox[a] = ovl[cx * 10 + a];
+          gx[a] = grad[cx * 10 + a];
+          oy[a] = ovl[cy * 10 + a];
+          gy[a] = grad[cy * 10 + a];
+          oz[a] = ovl[cz * 10 + a];
+          gz[a] = grad[cz * 10 + a];
+        }
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        for (int kk = 0; kk < 10; ++kk) {
+          const float ozk = oz[kk], gzk = gz[kk];
+#pragma unroll
+          for (int jj = 0; jj < 10; ++jj) {
+            const float *row = tile + (kk * 10 + jj) * 12;
+            const float4 c0 = *(const float4 *)row;
+            const float4 c1 = *(const float4 *)(row + 4);
+            const float2 c2 = *(const float2 *)(row + 8);
+            const float cv[10] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w, c2.x, c2.y};
+            float dxp = 0.f, val = 0.f;
+#pragma unroll
+            for (int ii = 0; ii < 10; ++ii) {
+              dxp = __builtin_fmaf(cv[ii], gx[ii], dxp);
+              val = __builtin_fmaf(cv[ii], ox[ii], val);
+            }
+            fx = __builtin_fmaf(oy[jj] * ozk, dxp, fx);
+            fy = __builtin_fmaf(gy[jj] * ozk, val, fy);
+            fz = __builtin_fmaf(oy[jj] * gzk, val, fz);
+          }
+        }
+        fxp = -w * fx;
+        fyp = -w * fy;
+        fzp = -w * fz;
+      }
+      if (pa.coef != 0.f) {
+        float px, py, pz;
+        if (pa.ball) {  // the particle's own position
+          px = x0 + (float)(P[PS_KX * npad + s] - 1) * h + dx0;
+          py = x0 + (float)(P[PS_KY * npad + s] - 1) * h + dy0;
+          pz = x0 + (float)(P[PS_KZ * npad + s] - 1) * h + dz0;
+        } else {        // rebuilt from the cell, as the reference's kernel does
+          px = x0 + (float)(kx - 1) * h + dx0;
+          py = x0 + (float)(ky - 1) * h + dy0;
+          pz = x0 + (float)(kz - 1) * h + dz0;
+        }
+        const float dx = pa.px - px, dy = pa.py - py, dz = pa.pz - pz;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        const float mm = pa.coef * (pa.ball ? ball_force_kernel_f(d2, pa.radius)
+                                            : gaussian_force_kernel_f(d2, pa.sigma));
+        const float fx2 = mm * dx, fy2 = mm * dy, fz2 = mm * dz;
+        if (ok) {
+          fxp -= fx2;
+          fyp -= fy2;
+          fzp -= fz2;
+        }
+        const float r = sqrtf(d2);
+        sax += fx2;
+        say += fy2;
+        saz += fz2;
+        if (pa.ball) {
+          const float R = pa.radius;
+          sae += r < R ? -(3.0f - (r / R) * (r / R)) / (2.0f * R) : -1.0f / r;
+        } else {
+          sae += r < 1.0e-4f * pa.sigma ? 0.7978845608f / pa.sigma
+                                        : erff(r / 1.4142135624f / pa.sigma) / r;
+        }
+      }
+      force[s] = fxp;
+      force[npad + s] = fyp;
+      force[2 * npad + s] = fzp;
+    }
+  }
+  // fixed-order reduction: wave, then one partial per block
+  sax = vh_wave_sum(sax);
+  say = vh_wave_sum(say);
+  saz = vh_wave_sum(saz);
+  sae = vh_wave_sum(sae);
+  if (t == 0) {
+    part[blockIdx.x] = sax;
+    part[gridDim.x + blockIdx.x] = say;
+    part[2 * gridDim.x + blockIdx.x] = saz;
+    part[3 * gridDim.x + blockIdx.x] = sae;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// out-of-stencil fallback, FP64
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void hermite_vd(const double *__restrict__ g, int nk, int b, double x,
+                                           double &v, double &dv) {
+  const int kn = b >> 1, kind = b & 1;
+  double a, h, s;
+  v = 0.0;
+  dv = 0.0;
+  if (kn > 0 && x <= g[kn]) {
+    if (x < g[kn - 1]) return;
+    h = g[kn] - g[kn - 1];
+    a = (x - g[kn - 1]) / h;
+    s = -1.0;
+  } else if (kn < nk - 1 && x >= g[kn]) {
+    if (x > g[kn + 1]) return;
+    h = g[kn + 1] - g[kn];
+    a = (g[kn + 1] - x) / h;
+    s = 1.0;
+  } else {
+    return;
+  }
+  if (kind == 0) {
+    v = 3.0 * (a * a) - 2.0 * (a * a * a);
+    dv = -6.0 * (a - a * a) * s / h;
+  } else {
+    v = h * ((a * a) - (a * a * a)) * s;
+    dv = -2.0 * a + 3.0 * (a * a);
+  }
+}
+
+__device__ __forceinline__ int cell_index_d(const double *__restrict__ g, int nk, double x) {
+  if (x < g[0] || x > g[nk - 1]) return -1;
+  int lo = 0, hi = nk;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (g[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  const int c = lo - 1;
+  return c < 0 ? 0 : c;
+}
+
+struct vh_fallback_args {
+  double px, py, pz;  // projectile position, T
+  double q;           // projectile charge
+  double radius;      // ball law radius (> 0) or Gaussian width (ball == 0)
+  int ball;
+  int has_proj;
+  double w;           // weight
+  double escaped;     // enclosed-charge count for particles outside both grids
+};
+
+__global__ void k_fallback(const int32_t *__restrict__ P, int64_t npad, const int32_t *__restrict__ outlist,
+                           const int32_t *__restrict__ cnt, double x0, double h,
+                           const float *__restrict__ csolc, const double *__restrict__ kc, int nkc,
+                           int nc, vh_fallback_args fa, float *__restrict__ force) {
+  const int nout = cnt[CNT_OUTCOUNT];
+  const float *D = (const float *)P;
+  for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nout;
+       u += (int64_t)gridDim.x * blockDim.x) {
+    const int s = outlist[u];
+    double p[3];
+    for (int d = 0; d < 3; ++d)
+      p[d] = x0 + (double)(P[(PS_KX + d) * npad + s] - 1) * h + (double)D[(PS_DX + d) * npad + s];
+    int cell[3];
+    bool out = false;
+    for (int d = 0; d < 3; ++d) {
+      cell[d] = cell_index_d(kc, nkc, p[d]);
+      out = out || cell[d] < 0;
+    }
+    double f0, f1, f2;
+    if (out) {
+      const double r2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+      const double r3 = pow(r2, 1.5);
+      const double cf = -(fa.w * fa.w) * fa.escaped / r3;
+      f0 = cf * p[0];
+      f1 = cf * p[1];
+      f2 = cf * p[2];
+    } else {
+      double v[3][4], dv[3][4];
+      for (int d = 0; d < 3; ++d)
+        for (int a = 0; a < 4; ++a) hermite_vd(kc, nkc, 2 * cell[d] + a, p[d], v[d][a], dv[d][a]);
+      double ex = 0.0, ey = 0.0, ez = 0.0;
+      for (int ak = 0; ak < 4; ++ak) {
+        const int k = 2 * cell[2] + ak;
+        const double vk = v[2][ak], dk = dv[2][ak];
+        for (int aj = 0; aj < 4; ++aj) {
+          const int j = 2 * cell[1] + aj;
+          const double vj = v[1][aj], dj = dv[1][aj];
+          for (int ai = 0; ai < 4; ++ai) {
+            const int i = 2 * cell[0] + ai;
+            const double vi = v[0][ai], di = dv[0][ai];
+            const double c = (double)csolc[i + (int64_t)nc * (j + (int64_t)nc * k)];
+            ex -= c * di * vj * vk;
+            ey -= c * vi * dj * vk;
+            ez -= c * vi * vj * dk;
+          }
+        }
+      }
+      f0 = fa.w * ex;
+      f1 = fa.w * ey;
+      f2 = fa.w * ez;
+    }
+    if (fa.has_proj) {
+      const double d0 = fa.px - p[0], d1 = fa.py - p[1], d2 = fa.pz - p[2];
+      const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      const double fk = fa.ball ? ball_force_kernel_d(r2, fa.radius) : gaussian_force_kernel_d(r2, fa.radius);
+      const double m = -fa.w * fa.q * fk;
+      f0 = f0 - m * d0;
+      f1 = f1 - m * d1;
+      f2 = f2 - m * d2;
+    }
+    force[s] = (float)f0;
+    force[npad + s] = (float)f1;
+    force[2 * npad + s] = (float)f2;
+  }
+}
+
+__global__ void k_reduce_cols(const double *__restrict__ part, int nparts, int nvals,
+                              double *__restrict__ out) {
+  __shared__ double lds[16];
+  for (int v = 0; v < nvals; ++v) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[(int64_t)v * nparts + i];
+    const double r = vh_block_sum_d(s, lds);
+    if (threadIdx.x == 0) out[v] = r;
+  }
+}
+
+int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, const double *proj,
+                   int soft_mode, double ball_radius) {
+  vh_context *c = a->ctx;
+  hipStream_t st = c->stream;
+  vh_proj_args pa;
+  pa.px = pa.py = pa.pz = 0.f;
+  pa.coef = 0.f;
+  pa.sigma = 1.f;
+  pa.ball = soft_mode == VH_SOFT_BALL;
+  pa.radius = 1.f;
+  if (proj) {
+    pa.px = (float)proj[0];
+    pa.py = (float)proj[1];
+    pa.pz = (float)proj[2];
+    pa.coef = (float)(-weight * proj[3]);
+    pa.sigma = (float)proj[4];
+    pa.radius = (float)(ball_radius > 0 ? ball_radius : proj[4]);
+  }
+  int grid = 8 * c->num_cus;
+  if (grid > a->dpart_len / 4) grid = (int)(a->dpart_len / 4);
+  int tok = vh_prof_begin(c, VH_ST_FIELD_KERNEL);
+  hipLaunchKernelGGL(k_field, dim3(grid), dim3(64), 0, st, a->parts[a->cur], a->npad, fine->csol,
+                     a->n, a->ovl, a->grad, a->occupied, a->bounds, a->counters, a->nk,
+                     (float)a->x0, (float)a->h, a->spacing, (float)a->nbdt, (float)weight, a->ncol,
+                     pa, a->force, a->dpart);
+  vh_prof_end(c, tok);
+  VH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, a->dpart, grid, 4, a->dscal + 8);
+  VH_LAUNCH_CHECK();
+  vh_fallback_args fa;
+  fa.px = proj ? proj[0] : 0.0;
+  fa.py = proj ? proj[1] : 0.0;
+  fa.pz = proj ? proj[2] : 0.0;
+  fa.q = proj ? proj[3] : 0.0;
+  fa.ball = ball_radius > 0;
+  fa.radius = ball_radius > 0 ? ball_radius : (proj ? proj[4] : 1.0);
+  fa.has_proj = proj != nullptr;
+  fa.w = weight;
+  fa.escaped = 0.0;
+  hipLaunchKernelGGL(k_fallback, dim3(1024), dim3(256), 0, st, a->parts[a->cur], a->npad,
+                     a->outlist, a->counters, a->x0, a->h, coarse->csol, coarse->knots64, coarse->nk,
+                     coarse->n, fa, a->force);
+  VH_LAUNCH_CHECK();
+  return VH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// packed Verlet (src/kernels.jl:985-1025)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void verlet1(int kq, float dq, int ok, float od, float a, float f, float h,
+                                        int &kn, float &dn, float &gap) {
+  kn = 2 * kq - ok;
+  dn = 2.0f * dq - od + a * f;
+  const int m = (int)rintf(dn / h);
+  kn += m;
+  dn -= (float)m * h;
+  gap = (float)(kn - ok) * h + (dn - od);
+}
+
+__global__ void __launch_bounds__(256)
+    k_verlet(int32_t *__restrict__ P, int64_t npad, int64_t n, const float *__restrict__ force,
+             float a, float h, float x0, float pfac, float r2max, float inv2M,
+             double *__restrict__ part) {
+  __shared__ double lds[8];
+  float *D = (float *)P;
+  double ek = 0.0, eo = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int k1 = P[PS_KX * npad + i], k2 = P[PS_KY * npad + i], k3 = P[PS_KZ * npad + i];
+    const float d1 = D[PS_DX * npad + i], d2 = D[PS_DY * npad + i], d3 = D[PS_DZ * npad + i];
+    const int o1 = P[PS_PKX * npad + i], o2 = P[PS_PKY * npad + i], o3 = P[PS_PKZ * npad + i];
+    const float e1 = D[PS_PDX * npad + i], e2 = D[PS_PDY * npad + i], e3 = D[PS_PDZ * npad + i];
+    int n1, n2, n3;
+    float m1, m2, m3, g1, g2, g3;
+    verlet1(k1, d1, o1, e1, a, force[i], h, n1, m1, g1);
+    verlet1(k2, d2, o2, e2, a, force[npad + i], h, n2, m2, g2);
+    verlet1(k3, d3, o3, e3, a, force[2 * npad + i], h, n3, m3, g3);
+    const float px = g1 * pfac, py = g2 * pfac, pz = g3 * pfac;
+    const float e = (px * px + py * py + pz * pz) * inv2M;
+    ek += e;
+    const float qx = x0 + (float)(k1 - 1) * h + d1;
+    const float qy = x0 + (float)(k2 - 1) * h + d2;
+    const float qz = x0 + (float)(k3 - 1) * h + d3;
+    if (qx * qx + qy * qy + qz * qz > r2max) eo += e;
+    lx += qy * pz - qz * py;
+    ly += qz * px - qx * pz;
+    lz += qx * py - qy * px;
+    P[PS_KX * npad + i] = n1;
+    P[PS_KY * npad + i] = n2;
+    P[PS_KZ * npad + i] = n3;
+    D[PS_DX * npad + i] = m1;
+    D[PS_DY * npad + i] = m2;
+    D[PS_DZ * npad + i] = m3;
+    P[PS_PKX * npad + i] = k1;
+    P[PS_PKY * npad + i] = k2;
+    P[PS_PKZ * npad + i] = k3;
+    D[PS_PDX * npad + i] = d1;
+    D[PS_PDY * npad + i] = d2;
+    D[PS_PDZ * npad + i] = d3;
+  }
+  double vals[5] = {ek, eo, lx, ly, lz};
+  for (int v = 0; v < 5; ++v) {
+    const double s = vh_block_sum_d(vals[v], lds);
+    if (threadIdx.x == 0) part[(int64_t)v * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+#define VERLET_BLOCKS 2048
+
+int vh_verlet_impl(vh_accel *a, double dt, double mass, double rcmax) {
+  vh_context *c = a->ctx;
+  const float fa = (float)(dt * dt / mass);
+  const float fh = (float)a->h, fx0 = (float)a->x0;
+  const float pfac = (float)(mass / (2.0 * dt));
+  const float r2max = (float)(rcmax * rcmax);
+  const float inv2M = (float)(1.0 / (2.0 * mass));
+  hipLaunchKernelGGL(k_verlet, dim3(VERLET_BLOCKS), dim3(256), 0, c->stream, a->parts[a->cur],
+                     a->npad, a->npart, a->force, fa, fh, fx0, pfac, r2max, inv2M, a->dpart);
+  VH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, c->stream, a->dpart, VERLET_BLOCKS, 5,
+                     a->dscal);
+  VH_LAUNCH_CHECK();
+  return VH_OK;
+}

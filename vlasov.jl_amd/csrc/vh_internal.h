@@ -1,0 +1,206 @@
+// Internal definitions shared by the translation units of libvlasov_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/vlasov_hip.h"
+
+// ---------------------------------------------------------------------------
+// error plumbing: a thread-local message, never an exception across the ABI
+// ---------------------------------------------------------------------------
+void vh_set_error(const char *fmt, ...);
+
+#define VH_CHECK_HIP(expr)                                                       \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess) {                                                      \
+      vh_set_error("%s:%d: %s: %s", __FILE__, __LINE__, #expr, hipGetErrorString(e_)); \
+      return VH_EHIP;                                                            \
+    }                                                                            \
+  } while (0)
+
+#define VH_REQUIRE(cond, ...)                                                    \
+  do {                                                                           \
+    if (!(cond)) {                                                               \
+      vh_set_error(__VA_ARGS__);                                                 \
+      return VH_EINVAL;                                                          \
+    }                                                                            \
+  } while (0)
+
+#define VH_LAUNCH_CHECK()                                                        \
+  do {                                                                           \
+    hipError_t e_ = hipGetLastError();                                           \
+    if (e_ != hipSuccess) {                                                      \
+      vh_set_error("%s:%d: launch: %s", __FILE__, __LINE__, hipGetErrorString(e_)); \
+      return VH_EHIP;                                                            \
+    }                                                                            \
+  } while (0)
+
+#define VH_TRY(expr)                                                             \
+  do {                                                                           \
+    int r_ = (expr);                                                             \
+    if (r_ != VH_OK) return r_;                                                  \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// objects behind the opaque handles
+// ---------------------------------------------------------------------------
+struct vh_prof_rec {
+  int stage;
+  hipEvent_t start, stop;
+};
+
+struct vh_context {
+  int device;
+  hipStream_t stream;
+  int num_cus;
+  // timing of regions with events on `stream`
+  int prof_on;
+  std::vector<vh_prof_rec> prof_open;   // recorded, not yet read
+  std::vector<hipEvent_t> prof_pool;    // free events
+};
+
+// region timing: begin returns a token (-1 when profiling is off)
+int vh_prof_begin(vh_context *c, int stage);
+void vh_prof_end(vh_context *c, int token);
+
+struct vh_mesh {
+  vh_context *ctx;
+  int n, nk, ni;
+  // tables, float unless stated (the reference narrows them to E = Float32)
+  float *colloc, *knots;
+  double *colloc64, *knots64;
+  float *lap;        // full D, row-major [r*n + c]
+  float *Minv_rm;    // ni x ni row-major, A operand of the rotating product
+  float *M_rm;
+  float *Sinv_rm;    // n x n row-major
+  float *invlsum;    // ni^3, i fastest
+  double *dual;      // 3 x n (order k at k*n), FP64
+  float *dual32;     // same, narrowed
+  float *duallen;    // n
+  float loc_x0, loc_iw;
+  int32_t *loc_cells;  // 1-based
+  int nbin;
+  // grids, n^3 floats, [i + n j + n^2 k]
+  float *rho, *phi, *csol, *extra, *work;
+  // interior work, ni^3
+  float *inner, *work1, *work2;
+  // reductions
+  double *partials;  // 10 x (n*n) moments / charge partials
+  double *scal;      // [0..9] moments, [10..19] multipole, [20] total charge
+  int64_t *acc64;    // n^3 fixed-point accumulator of the CIC deposit
+};
+
+// Particle streams, structure of arrays: 12 int32/float streams of npad words.
+enum { PS_KX = 0, PS_KY, PS_KZ, PS_DX, PS_DY, PS_DZ, PS_PKX, PS_PKY, PS_PKZ, PS_PDX, PS_PDY, PS_PDZ, PS_COUNT };
+
+// device-side counters of the accelerator
+enum {
+  CNT_NOCC = 0,   // occupied cells
+  CNT_NOUT,       // columns-kernel rejections (deposit nout)
+  CNT_OUTCOUNT,   // particles whose 10^3 stencil leaves the fine grid
+  CNT_NFIT,       // occupied cells whose stencil fits
+  CNT_NNONFIT,    // occupied cells that do not fit
+  CNT_SORTED,     // 1 once a sort has run
+  CNT_COUNT
+};
+
+struct vh_accel {
+  vh_context *ctx;
+  int64_t npart, npad;
+  int nk, n, ncell;
+  double x0, h;
+  float spacing;
+  int nbdt, ncol;
+  float *ovl, *grad, *nodes;   // [col*10 + a], [col*8 + a]
+  int32_t *parts[2];           // 12 streams each
+  int cur;
+  float *force;                // 3 streams (fx, fy, fz), slot order
+  // sort
+  int bits_total, passes, digit_bits, nbins;
+  int64_t tile, nblocks;
+  uint32_t *keys[2], *vals[2];
+  uint32_t *counts;            // nbins * nblocks (digit-major), scanned in place
+  uint32_t *scan_tmp;          // block sums of the scans
+  int64_t scan_tmp_len;
+  int32_t *occupied;           // 0-based keys, up to ncell
+  int32_t *bounds;             // n_occ + 1
+  int32_t *cellmap;            // ncell -> fitting occupied index or -1
+  int32_t *fitlist;            // occupied index of fitting cells
+  int32_t *nonfit;             // occupied index of non-fitting cells
+  int32_t *nonfit_prefix;      // exclusive prefix of their particle counts (+ total)
+  int32_t *outlist;            // 0-based slots of out-of-stencil particles
+  int32_t *counters;           // CNT_*
+  uint32_t *flags;             // npad scratch (run starts)
+  float *tiles;                // 512 per fitting cell (fine deposit)
+  double *dpart;               // per-block FP64 partials of the reductions
+  int64_t dpart_len;
+  double *dscal;               // [0..4] verlet diag, [8..11] field reduction, [16] q, [17] enclosed
+  int cic_shift;               // fixed-point scale 2^shift of the coarse deposit
+  float lo, hi;                // E(knots[2] + h/2), E(knots[end-1] - h/2): deposit bounds
+  int64_t maxocc;              // min(ncell, npart)
+  uint32_t *cellscratch;       // 3 x (maxocc + 1) scan scratch of the cell classification
+  int sorted_once;             // host flag: the lists of a sort exist
+};
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+#define VH_WAVE 64
+
+__device__ __forceinline__ int vh_lane() { return __lane_id(); }
+
+template <typename T>
+__device__ __forceinline__ T vh_wave_sum(T v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// fixed-order block reduction of a double (blockDim.x multiple of 64, <= 1024)
+__device__ __forceinline__ double vh_block_sum_d(double v, double *lds) {
+  v = vh_wave_sum(v);
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) lds[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) r += lds[i];
+  }
+  __syncthreads();
+  return r;  // valid in thread 0
+}
+
+// ---------------------------------------------------------------------------
+// internal entry points shared between translation units
+// ---------------------------------------------------------------------------
+int vh_alloc(void **p, size_t bytes);
+int vh_free(void *p);
+int vh_fill_i32(vh_context *c, int32_t *p, int64_t n, int32_t v);
+int vh_fill_u64(vh_context *c, uint64_t *p, int64_t n, uint64_t v);
+int vh_fill_f32(vh_context *c, float *p, int64_t n, float v);
+
+// exclusive scan of n uint32 in place; returns nothing on host (total at out[n] if with_total)
+int vh_scan_exclusive(vh_context *c, uint32_t *data, int64_t n, uint32_t *tmp, int64_t tmp_len,
+                      uint32_t *total_out);
+
+// rotating product: out[i' m + j] = sum_i A[i' n + i] src[j n + i], m = len/n
+enum { EPI_STORE = 0, EPI_HADAMARD = 1, EPI_FILL_INTERIOR = 2, EPI_ACCUMULATE = 3 };
+int vh_rotate_gemm(vh_context *c, const float *A, const float *src, float *out, int n, int64_t m,
+                   int epi, const float *aux, int nfull);
+
+int vh_sort_impl(vh_accel *a);
+int vh_deposit_fine_impl(vh_accel *a, vh_mesh *fine, double charge);
+int vh_deposit_cic_impl(vh_accel *a, vh_mesh *coarse, double charge);
+int vh_poisson_single_impl(vh_mesh *m);
+int vh_poisson_nested_impl(vh_mesh *fine, vh_mesh *coarse);
+int vh_spline_coefficients_impl(vh_mesh *m, const float *src, float *dst);
+int vh_effective_potential_impl(vh_mesh *m, double nions, double radius);
+int vh_total_charge_impl(vh_mesh *m, const float *rho, double *dq);
+int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, const double *proj,
+                   int soft_mode, double ball_radius);
+int vh_verlet_impl(vh_accel *a, double dt, double mass, double rcmax);

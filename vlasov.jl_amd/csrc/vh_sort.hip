@@ -1,0 +1,393 @@
+// Stable counting sort of the particles by fine cell, on the device.
+//
+// The reference sorts with an atomic-cursor placement whose order inside a
+// cell is not deterministic (src/kernels.jl:945-955); its host sort is a
+// stable counting sort (src/sorting.jl:163-204).  Here the permutation is the
+// stable one: an LSD radix sort of (cell key, index) pairs in passes of at
+// most 11 bits, each pass = per-tile digit histogram (LDS) -> exclusive scan
+// over (digit, tile) -> stable scatter with in-wave ranks from ballots.  The
+// 48-byte records (12 SoA streams) are then gathered once through the
+// permutation.  Keys, occupied cells and bounds follow src/kernels.jl:902-908
+// and src/accelerator.jl:397-411.
+#include "vh_internal.h"
+
+#define SORT_THREADS 256
+#define SORT_ITEMS 16  // tile = 4096 elements
+
+// ---------------------------------------------------------------------------
+// generic exclusive scan (uint32, in place)
+// ---------------------------------------------------------------------------
+#define SCAN_THREADS 256
+#define SCAN_ITEMS 4  // 1024 per block
+
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
+  int l = vh_lane();
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t t = __shfl_up(v, o, 64);
+    if (l >= o) v += t;
+  }
+  return v;
+}
+
+// exclusive scan of one value per thread across the block; returns the block total
+__device__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t &excl, uint32_t *lds) {
+  uint32_t inc = wave_incl_scan_u32(v);
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  int nw = blockDim.x >> 6;
+  if (l == 63) lds[w] = inc;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  for (int i = 0; i < nw; ++i) {
+    uint32_t s = lds[i];
+    if (i < w) off += s;
+    tot += s;
+  }
+  __syncthreads();
+  excl = off + inc - v;
+  return tot;
+}
+
+__global__ void k_scan_reduce(const uint32_t *__restrict__ data, int64_t n, uint32_t *sums) {
+  __shared__ uint32_t lds[16];
+  int64_t base = (int64_t)blockIdx.x * SCAN_THREADS * SCAN_ITEMS;
+  uint32_t s = 0;
+  for (int it = 0; it < SCAN_ITEMS; ++it) {
+    int64_t i = base + it * SCAN_THREADS + threadIdx.x;
+    if (i < n) s += data[i];
+  }
+  uint32_t ex;
+  uint32_t tot = block_excl_scan_u32(s, ex, lds);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// single block: exclusive scan of nb sums in place, total written to sums[nb]
+__global__ void k_scan_sums(uint32_t *sums, int64_t nb) {
+  __shared__ uint32_t lds[16];
+  uint32_t carry = 0;
+  for (int64_t base = 0; base < nb; base += blockDim.x) {
+    int64_t i = base + threadIdx.x;
+    uint32_t v = i < nb ? sums[i] : 0;
+    uint32_t ex;
+    uint32_t tot = block_excl_scan_u32(v, ex, lds);
+    if (i < nb) sums[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) sums[nb] = carry;
+}
+
+__global__ void k_scan_down(uint32_t *data, int64_t n, const uint32_t *__restrict__ sums) {
+  __shared__ uint32_t lds[16];
+  int64_t base = (int64_t)blockIdx.x * SCAN_THREADS * SCAN_ITEMS;
+  // each thread owns SCAN_ITEMS consecutive elements
+  int64_t i0 = base + (int64_t)threadIdx.x * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS];
+  uint32_t s = 0;
+  for (int it = 0; it < SCAN_ITEMS; ++it) {
+    v[it] = (i0 + it < n) ? data[i0 + it] : 0;
+    s += v[it];
+  }
+  uint32_t ex;
+  block_excl_scan_u32(s, ex, lds);
+  uint32_t run = sums[blockIdx.x] + ex;
+  for (int it = 0; it < SCAN_ITEMS; ++it) {
+    if (i0 + it < n) data[i0 + it] = run;
+    run += v[it];
+  }
+}
+
+int vh_scan_exclusive(vh_context *c, uint32_t *data, int64_t n, uint32_t *tmp, int64_t tmp_len,
+                      uint32_t *total_out) {
+  int64_t per = SCAN_THREADS * SCAN_ITEMS;
+  int64_t nb = (n + per - 1) / per;
+  VH_REQUIRE(nb + 1 <= tmp_len, "scan: scratch too small (%lld blocks)", (long long)nb);
+  if (nb == 0) {
+    if (total_out) {
+      uint32_t z = 0;
+      VH_CHECK_HIP(hipMemcpyAsync(total_out, &z, 4, hipMemcpyHostToDevice, c->stream));
+    }
+    return VH_OK;
+  }
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, c->stream, data, n, tmp);
+  VH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, c->stream, tmp, nb);
+  VH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, c->stream, data, n, tmp);
+  VH_LAUNCH_CHECK();
+  if (total_out) {
+    VH_CHECK_HIP(hipMemcpyAsync(total_out, tmp + nb, 4, hipMemcpyDeviceToDevice, c->stream));
+  }
+  return VH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// keys
+// ---------------------------------------------------------------------------
+// clamp(k, 1, nk) per axis, 0-based linear key kx + nk (ky + nk kz)
+__device__ __forceinline__ uint32_t cell_key0(int kx, int ky, int kz, int nk) {
+  kx = kx < 1 ? 1 : (kx > nk ? nk : kx);
+  ky = ky < 1 ? 1 : (ky > nk ? nk : ky);
+  kz = kz < 1 ? 1 : (kz > nk ? nk : kz);
+  return (uint32_t)((kx - 1) + nk * ((ky - 1) + nk * (kz - 1)));
+}
+
+__device__ __forceinline__ uint32_t load_key(bool first, const uint32_t *__restrict__ keys,
+                                             const int32_t *__restrict__ P, int64_t npad, int64_t i,
+                                             int nk) {
+  if (first) return cell_key0(P[PS_KX * npad + i], P[PS_KY * npad + i], P[PS_KZ * npad + i], nk);
+  return keys[i];
+}
+
+// per-tile digit histogram, written digit-major: counts[d * nblocks + b]
+__global__ void __launch_bounds__(SORT_THREADS)
+    k_radix_hist(const uint32_t *__restrict__ keys, const int32_t *__restrict__ P, int64_t npad,
+                 int first, int nk, int64_t n, int shift, int nbins, int64_t nblocks,
+                 uint32_t *__restrict__ counts) {
+  extern __shared__ uint32_t hist[];
+  for (int d = threadIdx.x; d < nbins; d += SORT_THREADS) hist[d] = 0;
+  __syncthreads();
+  int64_t base = (int64_t)blockIdx.x * SORT_THREADS * SORT_ITEMS;
+  uint32_t mask = (uint32_t)nbins - 1;
+  for (int it = 0; it < SORT_ITEMS; ++it) {
+    int64_t i = base + it * SORT_THREADS + threadIdx.x;
+    if (i < n) {
+      uint32_t k = load_key(first, keys, P, npad, i, nk);
+      atomicAdd(&hist[(k >> shift) & mask], 1u);
+    }
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < nbins; d += SORT_THREADS)
+    counts[(int64_t)d * nblocks + blockIdx.x] = hist[d];
+}
+
+// stable scatter of one tile.  Element e of the tile is (round r, thread t),
+// e = r * 256 + t; within a round ranks follow (wave, lane) order.
+__global__ void __launch_bounds__(SORT_THREADS)
+    k_radix_scatter(const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
+                    const int32_t *__restrict__ P, int64_t npad, int first, int nk, int64_t n,
+                    int shift, int bits, int64_t nblocks, const uint32_t *__restrict__ offsets,
+                    uint32_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out) {
+  extern __shared__ uint32_t sm[];
+  int nbins = 1 << bits;
+  uint32_t *goff = sm;                 // nbins
+  uint32_t *run = sm + nbins;          // nbins
+  uint32_t *cnt = sm + 2 * nbins;      // 4 * nbins
+  for (int d = threadIdx.x; d < nbins; d += SORT_THREADS) {
+    goff[d] = offsets[(int64_t)d * nblocks + blockIdx.x];
+    run[d] = 0;
+    cnt[d] = cnt[nbins + d] = cnt[2 * nbins + d] = cnt[3 * nbins + d] = 0;
+  }
+  __syncthreads();
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t mask = (uint32_t)nbins - 1;
+  int64_t base = (int64_t)blockIdx.x * SORT_THREADS * SORT_ITEMS;
+  for (int r = 0; r < SORT_ITEMS; ++r) {
+    int64_t i = base + r * SORT_THREADS + threadIdx.x;
+    bool valid = i < n;
+    uint32_t k = 0, v = 0, d = 0;
+    if (valid) {
+      k = load_key(first, keys_in, P, npad, i, nk);
+      v = first ? (uint32_t)i : vals_in[i];
+      d = (k >> shift) & mask;
+    }
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < bits; ++b) {
+      uint64_t bb = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    uint32_t rank = (uint32_t)__popcll(peers & lt);
+    uint32_t tot = (uint32_t)__popcll(peers);
+    bool leader = valid && rank == 0;
+    if (leader) cnt[w * nbins + d] = tot;
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = run[d] + rank;
+      for (int ww = 0; ww < w; ++ww) pos += cnt[ww * nbins + d];
+      uint32_t dst = goff[d] + pos;
+      keys_out[dst] = k;
+      vals_out[dst] = v;
+    }
+    __syncthreads();
+    if (leader) {
+      atomicAdd(&run[d], tot);
+      cnt[w * nbins + d] = 0;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// record gather + deposit "nout" (columns test of src/kernels.jl:754-779)
+// ---------------------------------------------------------------------------
+__global__ void k_permute(const int32_t *__restrict__ src, int32_t *__restrict__ dst,
+                          const uint32_t *__restrict__ perm, int64_t n, int64_t npad, float x0,
+                          float h, float lo, float hi, int32_t *cnt) {
+  int local = 0;
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = perm[s];
+    int32_t v[PS_COUNT];
+#pragma unroll
+    for (int q = 0; q < PS_COUNT; ++q) v[q] = src[q * npad + i];
+#pragma unroll
+    for (int q = 0; q < PS_COUNT; ++q) dst[q * npad + s] = v[q];
+    bool inside = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      float p = x0 + (float)(v[PS_KX + d] - 1) * h + __int_as_float(v[PS_DX + d]);
+      inside = inside && (lo <= p) && (p <= hi);
+    }
+    local += inside ? 0 : 1;
+  }
+  local = vh_wave_sum(local);
+  if (vh_lane() == 0 && local) atomicAdd(&cnt[CNT_NOUT], local);
+}
+
+// run starts of the sorted keys
+__global__ void k_run_flags(const uint32_t *__restrict__ keys, int64_t n, uint32_t *flags) {
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
+       s += (int64_t)gridDim.x * blockDim.x)
+    flags[s] = (s == 0 || keys[s] != keys[s - 1]) ? 1u : 0u;
+}
+
+// flags hold the exclusive scan: occupied[g] = key, bounds[g] = first slot
+__global__ void k_run_write(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ idx,
+                            int64_t n, int32_t *occupied, int32_t *bounds, int32_t *cnt) {
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    bool start = (s == 0 || keys[s] != keys[s - 1]);
+    if (start) {
+      occupied[idx[s]] = (int32_t)keys[s];
+      bounds[idx[s]] = (int32_t)s;
+    }
+    if (s == n - 1) {
+      uint32_t nocc = idx[s] + (start ? 1u : 0u);
+      bounds[nocc] = (int32_t)n;
+      cnt[CNT_NOCC] = (int32_t)nocc;
+    }
+  }
+}
+
+// per occupied cell: does the 10^3 stencil fit (src/kernels.jl:1047-1052)?
+// Fitting cells enter the deposit map; the particle counts of the others are
+// compacted into the out-of-stencil list.  Entries past n_occ are zero so
+// that the exclusive scans over cap + 1 entries end with the totals.
+__global__ void k_cell_classify(const int32_t *__restrict__ occupied, const int32_t *__restrict__ bounds,
+                                const int32_t *__restrict__ cnt, int nk, int n, int32_t *cellmap,
+                                uint32_t *fitidx, uint32_t *nfidx, uint32_t *nfpre, int64_t len) {
+  int nocc = cnt[CNT_NOCC];
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < len;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    if (g >= nocc) {
+      fitidx[g] = 0;
+      nfidx[g] = 0;
+      nfpre[g] = 0;
+      continue;
+    }
+    int c0 = occupied[g];
+    int kx = c0 % nk + 1, ky = (c0 / nk) % nk + 1, kz = c0 / (nk * nk) + 1;
+    int bx = 2 * kx - 5, by = 2 * ky - 5, bz = 2 * kz - 5;
+    bool ok = bx >= 1 && by >= 1 && bz >= 1 && bx + 9 <= n && by + 9 <= n && bz + 9 <= n;
+    fitidx[g] = ok ? 1u : 0u;
+    nfidx[g] = ok ? 0u : 1u;
+    nfpre[g] = ok ? 0u : (uint32_t)(bounds[g + 1] - bounds[g]);
+    cellmap[c0] = ok ? (int32_t)g : -1;
+  }
+}
+
+// after the scans: scatter the fitting / non-fitting lists, write the totals
+__global__ void k_cell_lists(const uint32_t *__restrict__ fitidx, const uint32_t *__restrict__ nfidx,
+                             const uint32_t *__restrict__ nfpre, int32_t *fitlist, int32_t *nonfit,
+                             int32_t *nonfit_prefix, int32_t *cnt, int64_t len) {
+  int nocc = cnt[CNT_NOCC];
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < len;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    if (g < nocc) {
+      if (fitidx[g + 1] != fitidx[g]) fitlist[fitidx[g]] = (int32_t)g;
+      if (nfidx[g + 1] != nfidx[g]) {
+        nonfit[nfidx[g]] = (int32_t)g;
+        nonfit_prefix[nfidx[g]] = (int32_t)nfpre[g];
+      }
+    } else if (g == nocc) {
+      cnt[CNT_NFIT] = (int32_t)fitidx[g];
+      cnt[CNT_NNONFIT] = (int32_t)nfidx[g];
+      cnt[CNT_OUTCOUNT] = (int32_t)nfpre[g];
+      nonfit_prefix[nfidx[g]] = (int32_t)nfpre[g];
+      cnt[CNT_SORTED] = 1;
+    }
+  }
+}
+
+// the out-of-stencil list: the slots of every non-fitting cell, in order
+__global__ void k_outlist(const int32_t *__restrict__ nonfit, const int32_t *__restrict__ nonfit_prefix,
+                          const int32_t *__restrict__ bounds, const int32_t *__restrict__ cnt,
+                          int32_t *outlist) {
+  int nn = cnt[CNT_NNONFIT];
+  for (int64_t c = blockIdx.x; c < nn; c += gridDim.x) {
+    int g = nonfit[c];
+    int lo = bounds[g], hi = bounds[g + 1];
+    int off = nonfit_prefix[c];
+    for (int s = lo + threadIdx.x; s < hi; s += blockDim.x) outlist[off + (s - lo)] = s;
+  }
+}
+
+int vh_sort_impl(vh_accel *a) {
+  vh_context *c = a->ctx;
+  hipStream_t st = c->stream;
+  const int64_t n = a->npart;
+  // counters of this step
+  VH_TRY(vh_fill_i32(c, a->counters, CNT_COUNT, 0));
+  // --- radix passes over (key, index) ---
+  int src = 0;
+  for (int p = 0; p < a->passes; ++p) {
+    int shift = p * a->digit_bits;
+    int bits = a->digit_bits;
+    if (shift + bits > a->bits_total) bits = a->bits_total - shift;
+    if (bits < 1) bits = 1;
+    int nbins = 1 << bits;
+    int first = (p == 0);
+    hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)a->nblocks), dim3(SORT_THREADS),
+                       nbins * sizeof(uint32_t), st, a->keys[src], a->parts[a->cur], a->npad, first,
+                       a->nk, n, shift, nbins, a->nblocks, a->counts);
+    VH_LAUNCH_CHECK();
+    VH_TRY(vh_scan_exclusive(c, a->counts, (int64_t)nbins * a->nblocks, a->scan_tmp,
+                             a->scan_tmp_len, nullptr));
+    hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)a->nblocks), dim3(SORT_THREADS),
+                       6 * nbins * sizeof(uint32_t), st, a->keys[src], a->vals[src],
+                       a->parts[a->cur], a->npad, first, a->nk, n, shift, bits, a->nblocks,
+                       a->counts, a->keys[1 - src], a->vals[1 - src]);
+    VH_LAUNCH_CHECK();
+    src = 1 - src;
+  }
+  const uint32_t *skeys = a->keys[src];
+  const uint32_t *perm = a->vals[src];
+  // --- move the records; count the deposit's rejections on the way ---
+  hipLaunchKernelGGL(k_permute, dim3(4096), dim3(256), 0, st, a->parts[a->cur], a->parts[1 - a->cur],
+                     perm, n, a->npad, (float)a->x0, (float)a->h, a->lo, a->hi, a->counters);
+  VH_LAUNCH_CHECK();
+  a->cur = 1 - a->cur;
+  // --- occupied cells and bounds from the runs of sorted keys ---
+  hipLaunchKernelGGL(k_run_flags, dim3(4096), dim3(256), 0, st, skeys, n, a->flags);
+  VH_LAUNCH_CHECK();
+  VH_TRY(vh_scan_exclusive(c, a->flags, n, a->scan_tmp, a->scan_tmp_len, nullptr));
+  hipLaunchKernelGGL(k_run_write, dim3(4096), dim3(256), 0, st, skeys, a->flags, n, a->occupied,
+                     a->bounds, a->counters);
+  VH_LAUNCH_CHECK();
+  // --- classification of the occupied cells ---
+  int64_t len = a->maxocc + 1;
+  uint32_t *fitidx = a->cellscratch, *nfidx = fitidx + len, *nfpre = nfidx + len;
+  VH_TRY(vh_fill_i32(c, a->cellmap, a->ncell, -1));
+  hipLaunchKernelGGL(k_cell_classify, dim3(1024), dim3(256), 0, st, a->occupied, a->bounds,
+                     a->counters, a->nk, a->n, a->cellmap, fitidx, nfidx, nfpre, len);
+  VH_LAUNCH_CHECK();
+  VH_TRY(vh_scan_exclusive(c, fitidx, len, a->scan_tmp, a->scan_tmp_len, nullptr));
+  VH_TRY(vh_scan_exclusive(c, nfidx, len, a->scan_tmp, a->scan_tmp_len, nullptr));
+  VH_TRY(vh_scan_exclusive(c, nfpre, len, a->scan_tmp, a->scan_tmp_len, nullptr));
+  hipLaunchKernelGGL(k_cell_lists, dim3(1024), dim3(256), 0, st, fitidx, nfidx, nfpre, a->fitlist,
+                     a->nonfit, a->nonfit_prefix, a->counters, len);
+  VH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_outlist, dim3(1024), dim3(256), 0, st, a->nonfit, a->nonfit_prefix,
+                     a->bounds, a->counters, a->outlist);
+  VH_LAUNCH_CHECK();
+  return VH_OK;
+}

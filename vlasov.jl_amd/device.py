@@ -1,0 +1,259 @@
+"""The reference's device layer, backed by libvlasov_hip.so.
+
+`DeviceMesh` and `DeviceAccelerator` mirror src/devicemesh.jl and
+src/accelerator.jl: same constructors' inputs, same method names (with `!`
+dropped), same argument meaning and the same refusals (ValueError where the
+reference throws ArgumentError / DimensionMismatch, before any launch).
+Device memory is owned by the library behind opaque handles; numpy arrays
+cross the boundary only through explicit upload/download calls, as
+`DualBuffer` does (src/accelerator.jl:28-68).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import VH_GRID_CSOL, VH_GRID_PHI, VH_GRID_RHO, VH_SOFT_BALL, VH_SOFT_DEVICE
+from .grids import GaussianSmoothing, SplineAxis, SplineMesh
+from .physics import PACKED_DTYPE, BallSoftening, Jellium, Projectile
+
+SOFTENING_MODES = {"device": VH_SOFT_DEVICE, "ball": VH_SOFT_BALL}
+
+
+class Context:
+    """One device and one stream."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _lib.call("vh_context_create", int(device), C.byref(h))
+        self.handle = h
+        self.device = device
+
+    def sync(self):
+        _lib.call("vh_sync", self.handle)
+
+    def close(self):
+        if self.handle:
+            _lib.call("vh_context_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _colmajor(m: np.ndarray) -> np.ndarray:
+    """Julia column-major flattening of a math-indexed matrix."""
+    return np.ascontiguousarray(np.asarray(m, dtype=np.float64).T).ravel()
+
+
+class DeviceMesh:
+    """Device mirror of a SplineMesh (src/devicemesh.jl:21-81)."""
+
+    def __init__(self, ctx: Context, mesh: SplineMesh):
+        ax = mesh.axis
+        n, nk = ax.nbasis, ax.nknots
+        op = mesh.op
+        dual = np.ascontiguousarray(np.concatenate(mesh.dual_moments))
+        h = C.c_void_p()
+        _lib.call("vh_mesh_create", ctx.handle, n, nk,
+                  np.ascontiguousarray(ax.colloc), np.ascontiguousarray(ax.knots),
+                  _colmajor(mesh.laplacian), _colmajor(op.M), _colmajor(op.Minv),
+                  np.ascontiguousarray(op.lam), _colmajor(mesh.coll.Sinv), dual,
+                  np.ascontiguousarray(mesh.dual_len), float(mesh.locator.x0),
+                  float(mesh.locator.invwidth), np.ascontiguousarray(mesh.locator.cell, dtype=np.int32),
+                  len(mesh.locator.cell), C.byref(h))
+        self.handle = h
+        self.ctx = ctx
+        self.mesh = mesh
+        self.n = n
+
+    # --- grids ---------------------------------------------------------------
+    def download(self, which: int) -> np.ndarray:
+        out = np.empty(self.n ** 3, dtype=np.float32)
+        _lib.call("vh_mesh_grid_download", self.handle, int(which), out)
+        return out
+
+    def upload(self, which: int, data: np.ndarray):
+        a = np.ascontiguousarray(np.asarray(data, dtype=np.float32).ravel())
+        if a.size != self.n ** 3:
+            raise ValueError(f"grid must hold {self.n}^3 values, got {a.size}")
+        _lib.call("vh_mesh_grid_upload", self.handle, int(which), a)
+
+    @property
+    def rho(self):
+        return self.download(VH_GRID_RHO)
+
+    @property
+    def phi(self):
+        return self.download(VH_GRID_PHI)
+
+    @property
+    def csol(self):
+        return self.download(VH_GRID_CSOL)
+
+    # --- the chain -----------------------------------------------------------
+    def poisson(self):
+        """poisson!(phi, rho, dm), multipole faces (src/devicemesh.jl:148-151)."""
+        _lib.call("vh_poisson", self.handle)
+
+    def spline_coefficients(self):
+        """csol = S^-1 phi along every axis (src/devicemesh.jl:126-127)."""
+        _lib.call("vh_spline_coefficients", self.handle)
+
+    def effective_potential(self, jel: Jellium):
+        """csol += S^-1 (V_xc(rho) + V_jel) (src/devicemesh.jl:171-183)."""
+        _lib.call("vh_effective_potential", self.handle, float(jel.nions), float(jel.radius))
+
+    def total_charge(self) -> float:
+        q = C.c_double()
+        _lib.call("vh_total_charge", self.handle, C.byref(q))
+        return float(q.value)
+
+    def multipole(self) -> np.ndarray:
+        out = np.zeros(10)
+        _lib.call("vh_multipole", self.handle, out)
+        return out
+
+    def close(self):
+        if self.handle:
+            _lib.call("vh_mesh_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def poisson_nested(fine: DeviceMesh, coarse: DeviceMesh):
+    """poisson!(phis, rhos, dms) (src/devicemesh.jl:160-169)."""
+    _lib.call("vh_poisson_nested", fine.handle, coarse.handle)
+
+
+class DeviceAccelerator:
+    """src/accelerator.jl:126-331, for a cloud resident on the device
+    (`packed = true`)."""
+
+    def __init__(self, ctx: Context, fine: SplineAxis, sm: GaussianSmoothing, npart: int):
+        knots = np.ascontiguousarray(fine.knots, dtype=np.float64)
+        h = (knots[-1] - knots[0]) / (len(knots) - 1)
+        if np.max(np.abs(np.diff(knots) - h)) > 1e-9 * abs(h):
+            raise ValueError("the accelerated path assumes a uniform fine grid")
+        hdl = C.c_void_p()
+        _lib.call("vh_accel_create", ctx.handle, int(npart), len(knots), knots, float(sm.spacing),
+                  int(sm.nbdt), np.ascontiguousarray(sm.overlap), np.ascontiguousarray(sm.gradient),
+                  np.ascontiguousarray(sm.nodes), C.byref(hdl))
+        self.handle = hdl
+        self.ctx = ctx
+        self.npart = int(npart)
+        self.x0 = float(knots[0])
+        self.h = float(h)
+        self.nk = len(knots)
+        self.n = 2 * len(knots)
+
+    # --- buffers -------------------------------------------------------------
+    def upload(self, records: np.ndarray):
+        r = np.ascontiguousarray(records)
+        if r.dtype != PACKED_DTYPE or r.shape != (self.npart,):
+            raise ValueError(f"expected {self.npart} PackedParticle{{Float32}} records")
+        _lib.call("vh_particles_upload", self.handle, r.ctypes.data)
+
+    def download(self) -> np.ndarray:
+        r = np.zeros(self.npart, dtype=PACKED_DTYPE)
+        _lib.call("vh_particles_download", self.handle, r.ctypes.data)
+        return r
+
+    def forces(self) -> np.ndarray:
+        """(N, 3) forces in slot order."""
+        f = np.zeros(3 * self.npart, dtype=np.float32)
+        _lib.call("vh_forces_download", self.handle, f)
+        return f.reshape(self.npart, 3)
+
+    def upload_forces(self, f: np.ndarray):
+        a = np.ascontiguousarray(np.asarray(f, dtype=np.float32).reshape(-1))
+        if a.size != 3 * self.npart:
+            raise ValueError("force buffer must be (3, npart)")
+        _lib.call("vh_forces_upload", self.handle, a)
+
+    # --- methods -------------------------------------------------------------
+    def sort(self):
+        """_sort_and_list!: returns (n_occupied, outcount)."""
+        nocc, nout = C.c_int32(), C.c_int32()
+        _lib.call("vh_sort", self.handle, C.byref(nocc), C.byref(nout))
+        return int(nocc.value), int(nout.value)
+
+    def sort_lists(self, nocc: int, outcount: int):
+        """(occupied keys 1-based, bounds, outlist 1-based slots) of the last sort."""
+        occ = np.zeros(max(nocc, 1), dtype=np.int32)
+        bnd = np.zeros(nocc + 1, dtype=np.int32)
+        out = np.zeros(max(outcount, 1), dtype=np.int32)
+        _lib.call("vh_sort_lists", self.handle, occ, bnd, out)
+        return occ[:nocc], bnd, out[:outcount]
+
+    def columns(self):
+        """_fill_columns!: ((N, 3) 1-based columns, nout)."""
+        cols = np.zeros(3 * self.npart, dtype=np.int32)
+        nout = C.c_int32()
+        _lib.call("vh_columns", self.handle, cols, C.byref(nout))
+        return cols.reshape(self.npart, 3), int(nout.value)
+
+    def deposit_smoothed(self, mesh: DeviceMesh, charge: float) -> int:
+        """deposit_smoothed!(rho, acc, mesh, sm, positions; charge) -> nout."""
+        if mesh.n != self.n:
+            raise ValueError("rho must cover the whole collocation grid")
+        nout = C.c_int32()
+        _lib.call("vh_deposit_smoothed", self.handle, mesh.handle, float(charge), C.byref(nout))
+        return int(nout.value)
+
+    def deposit_cic(self, mesh: DeviceMesh, charge: float):
+        """deposit_cic!(rho, dm, acc, npart, charge, x0f, hf)."""
+        _lib.call("vh_deposit_cic", self.handle, mesh.handle, float(charge))
+
+    def compute_forces(self, fine: DeviceMesh, coarse: DeviceMesh, weight: float,
+                       projectile: Projectile | None = None, mode: str = "device"):
+        """forces!(cloud, acc, ...; projectile, packed=true): returns
+        (outcount, red[4]) with red the projectile reduction."""
+        if mode not in SOFTENING_MODES:
+            raise ValueError(f"unknown softening mode {mode!r}")
+        if fine.n != self.n:
+            raise ValueError(f"accelerator sized for n={self.n}")
+        red = np.zeros(4)
+        nout = C.c_int32()
+        if projectile is None:
+            pp, radius = None, 0.0
+        else:
+            pp = np.array([*projectile.position, projectile.charge, projectile.scale], dtype=np.float64)
+            radius = projectile.softening.radius if isinstance(projectile.softening, BallSoftening) else 0.0
+        _lib.call("vh_forces", self.handle, fine.handle, coarse.handle, float(weight),
+                  None if pp is None else pp.ctypes.data, SOFTENING_MODES[mode], float(radius),
+                  red.ctypes.data, C.byref(nout))
+        return int(nout.value), red
+
+    def verlet(self, dt: float, mass: float, rcmax: float = float("inf")):
+        """step!(cloud, dt, acc; rcmax) -> (kinetic, escaped, angular)."""
+        d = np.zeros(5)
+        _lib.call("vh_verlet", self.handle, float(dt), float(mass), float(rcmax), d.ctypes.data)
+        return {"kinetic": d[0], "escaped": d[1], "angular": (d[2], d[3], d[4])}
+
+    def enclosed_count(self, center, radius: float) -> int:
+        c = np.ascontiguousarray(np.asarray(center, dtype=np.float64))
+        cnt = C.c_int64()
+        _lib.call("vh_enclosed_count", self.handle, c, float(radius), C.byref(cnt))
+        return int(cnt.value)
+
+    def close(self):
+        if self.handle:
+            _lib.call("vh_accel_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

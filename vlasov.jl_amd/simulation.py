@@ -1,0 +1,181 @@
+"""The simulation driver: the reference's `Simulation` on its resident device
+path (src/simulation.jl:171-277, 382-416, 472-531), with every stage of the
+step running in libvlasov_hip.so.
+
+    sim = Simulation(params, read_radial_profile(), projectile=proj)
+    for s in range(nsteps):
+        sim.step()
+    q = sim.enclosed_charge(8.0)
+
+The constructor does what the reference's does once on the host in FP64:
+axes, meshes, eigen-factorisations, smoothing tables, the Thomas-Fermi
+sample, and the leapfrog priming (which needs one force evaluation).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import grids
+from .device import Context, DeviceAccelerator, DeviceMesh, poisson_nested
+from .physics import (ELECTRON_MASS, Jellium, Projectile, jellium_force, packed_records,
+                      record_positions, sample_thomas_fermi, split)
+
+
+@dataclass
+class SimulationParameters:
+    """src/simulation.jl:26-38."""
+    nfine: int = 28
+    ninner: int = 14
+    nouter: int = 14
+    rcluster: float = 50.0
+    rbox: float = 150.0
+    nions: float = 196.0
+    nelectrons: float = 196.0
+    nparticles: int = 20_000
+    nsteps: int = 10
+    dt: float = 1.0
+    rcmax: float = 100.0
+
+
+def xenon_parameters(nparticles: int, nfine: int, dt: float = 0.5) -> SimulationParameters:
+    """The Na196 + Xe25+ crossing of scripts/xenon.jl:303-333."""
+    ninner, nouter = grids.coarse_grid_sizes(nfine)
+    return SimulationParameters(nfine=nfine, ninner=ninner, nouter=nouter, rcluster=78.0,
+                                rbox=235.0, nions=196.0, nelectrons=196.0,
+                                nparticles=nparticles, nsteps=0, dt=dt)
+
+
+def xenon_projectile(dt: float = 0.5, v: float = 0.40, x0: float = -70.0, impact: float = 45.0,
+                     softening=None) -> Projectile:
+    """Xe25+ at v = 0.40 a.u., b = 45 a0, from x = -70 a0, ball softening of
+    radius 5 (scripts/xenon.jl:306-340)."""
+    from .physics import BallSoftening
+    mass = 131.3 * 1836.154
+    return Projectile.of(mass=mass, charge=25.0, energy=0.5 * mass * v ** 2, impact=impact, x0=x0,
+                         dt=dt, softening=softening or BallSoftening(5.0))
+
+
+def build_meshes(p: SimulationParameters):
+    fine_ax = grids.uniform_axis(-p.rcluster, p.rcluster, p.nfine)
+    coarse_ax = grids.stretched_axis(p.rcluster, p.rbox, p.ninner // 2, (p.nouter + 2) // 2)
+    fine, coarse = grids.SplineMesh.of(fine_ax), grids.SplineMesh.of(coarse_ax)
+    grids.nested_meshes(fine, coarse)
+    if fine.n != coarse.n:
+        raise ValueError("rho must cover the whole collocation grid: the two levels carry "
+                         f"{fine.n} and {coarse.n} basis functions (nfine must be even)")
+    return fine, coarse
+
+
+class Simulation:
+    """State of one run on one device."""
+
+    def __init__(self, params: SimulationParameters, profile, *, projectile: Projectile | None = None,
+                 seed: int = -1, softening_mode: str = "device", consistent_startup: bool = False,
+                 device: int = 0, tables=None, positions=None, momenta=None):
+        self.params = params
+        self.projectile = projectile
+        self.softening_mode = softening_mode
+        if tables is None:
+            fine, coarse = build_meshes(params)
+            sm = grids.GaussianSmoothing.of(fine.axis)
+        else:
+            fine, coarse, sm = tables
+        self.fine, self.coarse, self.smoothing = fine, coarse, sm
+        self.jellium = Jellium.of(params.nions)
+        self.weight = params.nelectrons / params.nparticles
+        self.mass = ELECTRON_MASS * self.weight
+        if positions is None:
+            positions, momenta = sample_thomas_fermi(profile, params.nparticles, self.weight, seed)
+        self.ctx = Context(device)
+        self.dmf = DeviceMesh(self.ctx, fine)
+        self.dmc = DeviceMesh(self.ctx, coarse)
+        self.acc = DeviceAccelerator(self.ctx, fine.axis, sm, params.nparticles)
+        self.x0, self.h = self.acc.x0, self.acc.h
+        self.nout = 0
+        self.outcount = 0
+        self.red = np.zeros(4)
+        self.step_count = 0
+        self._prime_leapfrog(positions, momenta, consistent_startup)
+
+    # --- construction ------------------------------------------------------------
+    def _prime_leapfrog(self, positions, momenta, consistent):
+        """src/simulation.jl:233-277: q(-dt/2) from the momenta, forces there,
+        then q(-dt) by the thesis's (dimensionally dubious) coefficient dt/M."""
+        M, dt = self.mass, self.params.dt
+        half = positions - (dt / (2 * M)) * momenta
+        rec = packed_records(half, positions, self.x0, self.h)
+        self.acc.upload(rec)
+        self.update_forces(advance=False)
+        rec = self.acc.download()
+        f = self.acc.forces().astype(np.float64)
+        q0 = record_positions(rec, self.x0, self.h, previous=True)
+        hp = record_positions(rec, self.x0, self.h)
+        coef = dt ** 2 / (4 * M) if consistent else dt / M
+        back = (-q0 + 2 * hp) + coef * f
+        rec["knode"], rec["delta"] = split(q0, self.x0, self.h)
+        rec["pknode"], rec["pdelta"] = split(back, self.x0, self.h)
+        self.acc.upload(rec)
+
+    # --- the step ------------------------------------------------------------------
+    def update_forces(self, advance: bool = True):
+        """_update_forces_resident! (src/simulation.jl:472-531), energy = false."""
+        w = self.weight
+        self.nout = self.acc.deposit_smoothed(self.dmf, w)
+        self.acc.deposit_cic(self.dmc, w)
+        poisson_nested(self.dmf, self.dmc)        # leaves csol_c = S^-1 phi_c
+        self.dmf.spline_coefficients()
+        self.dmf.effective_potential(self.jellium)
+        self.dmc.effective_potential(self.jellium)
+        self.outcount, self.red = self.acc.compute_forces(self.dmf, self.dmc, w, self.projectile,
+                                                          self.softening_mode)
+        if advance and self.projectile is not None:
+            self.advance_projectile()
+
+    def projectile_forces(self):
+        """projectile_forces!(cloud, acc, proj, jel) (src/accelerator.jl:746-756)."""
+        p = self.projectile
+        fjel, e_jel = jellium_force(p, self.jellium)
+        force = tuple(a + b for a, b in zip(fjel, self.red[:3]))
+        return force, self.weight * p.charge * self.red[3], e_jel
+
+    def advance_projectile(self):
+        force, _, _ = self.projectile_forces()
+        self.projectile.step(force, self.params.dt)
+
+    def step(self):
+        """step!(sim; energy = false) (src/simulation.jl:382-396)."""
+        self.update_forces(advance=True)
+        self.diagnostics = self.acc.verlet(self.params.dt, self.mass, self.params.rcmax)
+        self.step_count += 1
+        return self.diagnostics
+
+    def run(self, nsteps: int, callback=None):
+        for i in range(nsteps):
+            d = self.step()
+            if callback is not None:
+                callback(i + 1, d)
+
+    # --- observables -------------------------------------------------------------
+    def enclosed_charge(self, radius: float = 8.0) -> float:
+        """q(< radius) around the projectile (src/projectile.jl:243-247)."""
+        if self.projectile is None:
+            raise ValueError("no projectile")
+        return self.weight * self.acc.enclosed_count(self.projectile.position, radius)
+
+    def positions(self) -> np.ndarray:
+        return record_positions(self.acc.download(), self.x0, self.h)
+
+    def sync_host(self):
+        """sync_host!: the grids of both levels (src/simulation.jl:544-552)."""
+        return {"rho": (self.dmf.rho, self.dmc.rho), "phi": (self.dmf.phi, self.dmc.phi),
+                "csol": (self.dmf.csol, self.dmc.csol)}
+
+    def close(self):
+        for o in (getattr(self, "acc", None), getattr(self, "dmf", None), getattr(self, "dmc", None)):
+            if o is not None:
+                o.close()
+        if getattr(self, "ctx", None) is not None:
+            self.ctx.close()
