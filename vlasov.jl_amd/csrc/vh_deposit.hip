@@ -28,6 +28,13 @@ __device__ __forceinline__ int table_col(float d, float half, float spacing, flo
   return c < 1 ? 1 : (c > ncol ? ncol : c);
 }
 
+typedef float vh_f4d __attribute__((ext_vector_type(4)));
+
+// The cell's 8^3 sums as a rank-P update on the matrix cores:
+//   C[kk][ii + 8 jj] = sum_p gz[p][kk] * (gx[p][ii] gy[p][jj])
+// four particles per v_mfma_f32_16x16x4_f32 (rows kk, 8 of 16 used), four
+// 16-column tiles for the 64 (ii, jj) pairs.  Node values of 32 particles at
+// a time are staged in LDS; missing particles of a group contribute zeros.
 __global__ void __launch_bounds__(64)
     k_dep_tiles(const int32_t *__restrict__ P, int64_t npad, const float *__restrict__ nodes,
                 const int32_t *__restrict__ fitlist, const int32_t *__restrict__ bounds,
@@ -35,25 +42,26 @@ __global__ void __launch_bounds__(64)
                 float *__restrict__ tiles) {
   __shared__ float vals[24 * DEP_STAGE];
   const float *D = (const float *)P;
-  const int t = threadIdx.x;
-  const int jj = t & 7, kk = t >> 3;
+  const int l = threadIdx.x;
+  const int row = l & 15, kq = l >> 4;       // A: row kk, particle kq of the group
+  const int ii = l & 7, jh = (l >> 3) & 1;   // B: column l & 15 -> (ii, jj = 2 t + jh)
   const float half = spacing * 0.5f;
   const int nfit = cnt[CNT_NFIT];
   for (int f = blockIdx.x; f < nfit; f += gridDim.x) {
     const int g = fitlist[f];
     const int lo = bounds[g], hi = bounds[g + 1];
-    float acc[8];
+    vh_f4d acc[4];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) acc[i] = 0.f;
+    for (int tt = 0; tt < 4; ++tt) acc[tt] = (vh_f4d){0.f, 0.f, 0.f, 0.f};
     for (int cur = lo; cur < hi; cur += DEP_STAGE) {
       const int chunk = min(DEP_STAGE, hi - cur);
       __syncthreads();
-      if (t < chunk) {
-        const int64_t s = cur + t;
+      if (l < chunk) {
+        const int64_t s = cur + l;
         const int c1 = table_col(D[PS_DX * npad + s], half, spacing, nbdt, ncol) - 1;
         const int c2 = table_col(D[PS_DY * npad + s], half, spacing, nbdt, ncol) - 1;
         const int c3 = table_col(D[PS_DZ * npad + s], half, spacing, nbdt, ncol) - 1;
-        float *v = vals + 24 * t;
+        float *v = vals + 24 * l;
 #pragma unroll
         for (int a = 0; a < 8; ++a) {
           v[a] = nodes[c1 * 8 + a];
@@ -62,21 +70,31 @@ __global__ void __launch_bounds__(64)
         }
       }
       __syncthreads();
-      float a8[8];
+      for (int q0 = 0; q0 < chunk; q0 += 4) {
+        const int p = q0 + kq;
+        const bool live = p < chunk;
+        const float *v = vals + 24 * (live ? p : 0);
+        const float a = (live && row < 8) ? v[16 + row] : 0.f;
+        const float gxv = live ? v[ii] : 0.f;
 #pragma unroll
-      for (int i = 0; i < 8; ++i) a8[i] = 0.f;
-      for (int q = 0; q < chunk; ++q) {
-        const float *v = vals + 24 * q;
-        const float w = v[8 + jj] * v[16 + kk];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) a8[i] = __builtin_fmaf(v[i], w, a8[i]);
+        for (int tt = 0; tt < 4; ++tt) {
+          const float b = gxv * v[8 + 2 * tt + jh];
+          acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[tt], 0, 0, 0);
+        }
       }
-#pragma unroll
-      for (int i = 0; i < 8; ++i) acc[i] += a8[i];
     }
-    float4 *out = (float4 *)(tiles + (int64_t)g * 512 + 8 * t);
-    out[0] = make_float4(acc[0], acc[1], acc[2], acc[3]);
-    out[1] = make_float4(acc[4], acc[5], acc[6], acc[7]);
+    // lane l holds rows 4 (l >> 4) + r, column l & 15 of each tile; rows >= 8 are padding
+    if ((l >> 4) < 2) {
+      float *out = tiles + (int64_t)g * 512;
+#pragma unroll
+      for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int kk = 4 * (l >> 4) + r;
+          const int jj = 2 * tt + jh;
+          out[ii + 8 * (jj + 8 * kk)] = acc[tt][r];
+        }
+    }
   }
 }
 
@@ -163,7 +181,7 @@ __global__ void k_dep_scale(float *__restrict__ rho, int64_t len, const double *
 int vh_deposit_fine_impl(vh_accel *a, vh_mesh *fine, double charge) {
   vh_context *c = a->ctx;
   hipStream_t st = c->stream;
-  int grid = 8 * c->num_cus;
+  const int grid = (int)(a->maxocc < 16384 ? a->maxocc : 16384);
   int tok = vh_prof_begin(c, VH_ST_DEPOSIT_KERNEL);
   hipLaunchKernelGGL(k_dep_tiles, dim3(grid), dim3(64), 0, st, a->parts[a->cur], a->npad, a->nodes,
                      a->fitlist, a->bounds, a->counters, a->spacing, (float)a->nbdt, a->ncol,
@@ -203,98 +221,96 @@ __device__ __forceinline__ int locate_cell(const float *__restrict__ gt, int n, 
   return cc;
 }
 
-__device__ __forceinline__ void fx_add(unsigned long long *acc, int64_t idx, float v, double scale) {
-  if (v != 0.f) atomicAdd(acc + idx, (unsigned long long)(long long)rint((double)v * scale));
+__device__ __forceinline__ unsigned long long fx_of(float v, double scale) {
+  return (unsigned long long)(long long)rint((double)v * scale);
 }
 
-__global__ void __launch_bounds__(64)
+__device__ __forceinline__ void fx_add(unsigned long long *acc, int64_t idx, float v, double scale) {
+  if (v != 0.f) atomicAdd(acc + idx, fx_of(v, scale));
+}
+
+#define CIC_THREADS 256
+
+// One workgroup per occupied fine cell.  Each thread takes particles of the
+// cell in turn and adds its eight corner weights, as 64-bit fixed point, into
+// a 4^3 LDS window of coarse points anchored at the cell's lower corner;
+// particles whose corners leave the window go straight to the global
+// accumulator.  Integer sums are exact, so the result does not depend on the
+// order the atomics land in.
+__global__ void __launch_bounds__(CIC_THREADS)
     k_cic_tiles(const int32_t *__restrict__ P, int64_t npad, const int32_t *__restrict__ occupied,
                 const int32_t *__restrict__ bounds, const int32_t *__restrict__ cnt, int nkf,
                 float x0f, float hf, const float *__restrict__ gt, int n, float x0t, float iwt,
                 const int32_t *__restrict__ cells, int nbin, double scale,
                 unsigned long long *__restrict__ acc) {
-  __shared__ int rel[3 * 64];
-  __shared__ float wts[3 * 64];
+  __shared__ unsigned long long win[64];
   const float *D = (const float *)P;
   const int t = threadIdx.x;
-  const int tx = t & 3, ty = (t >> 2) & 3, tz = t >> 4;
   const int nocc = cnt[CNT_NOCC];
   for (int g = blockIdx.x; g < nocc; g += gridDim.x) {
     const int key = occupied[g];
-    const int kx = key % nkf + 1, ky = (key / nkf) % nkf + 1, kz = key / (nkf * nkf) + 1;
+    const int kc[3] = {key % nkf + 1, (key / nkf) % nkf + 1, key / (nkf * nkf) + 1};
     // window anchor: coarse interval of the cell's lower corner, per axis
     int anc[3];
-    {
-      const int kk[3] = {kx, ky, kz};
-      for (int d = 0; d < 3; ++d) {
-        float w;
-        float xl = x0f + ((float)(kk[d] - 1) - 0.5f) * hf;
-        int cc = locate_cell(gt, n, x0t, iwt, cells, nbin, xl, w);
-        anc[d] = cc;  // 0 when the corner lies outside: every particle escapes
-      }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      float w;
+      const float xl = x0f + ((float)(kc[d] - 1) - 0.5f) * hf;
+      anc[d] = locate_cell(gt, n, x0t, iwt, cells, nbin, xl, w);  // 0: every particle escapes
     }
+    const bool anchored = anc[0] > 0 && anc[1] > 0 && anc[2] > 0;
+    if (t < 64) win[t] = 0ull;
+    __syncthreads();
     const int lo = bounds[g], hi = bounds[g + 1];
-    float tile = 0.f;
-    for (int cur = lo; cur < hi; cur += 64) {
-      const int chunk = min(64, hi - cur);
-      __syncthreads();
-      if (t < chunk) {
-        const int64_t s = cur + t;
-        int ix[3];
-        float ax[3];
-        bool inside = true;
-        for (int d = 0; d < 3; ++d) {
-          float p = x0f + (float)(P[(PS_KX + d) * npad + s] - 1) * hf + D[(PS_DX + d) * npad + s];
-          ix[d] = locate_cell(gt, n, x0t, iwt, cells, nbin, p, ax[d]);
-          inside = inside && ix[d] > 0;
-        }
-        int r0 = -100, r1 = -100, r2 = -100;
-        if (inside) {
-          int q0 = ix[0] - anc[0], q1 = ix[1] - anc[1], q2 = ix[2] - anc[2];
-          bool fits = anc[0] > 0 && anc[1] > 0 && anc[2] > 0 && q0 >= 0 && q0 <= 2 && q1 >= 0 &&
-                      q1 <= 2 && q2 >= 0 && q2 <= 2;
-          if (fits) {
-            r0 = q0;
-            r1 = q1;
-            r2 = q2;
-          } else {
-            // the particle leaves the window: its eight corners directly
-            const float bx = 1.f - ax[0], by = 1.f - ax[1], bz = 1.f - ax[2];
-            const int64_t i0 = ix[0] - 1, j0 = ix[1] - 1, k0 = ix[2] - 1;
-            const int64_t nn = n;
-            fx_add(acc, i0 + nn * (j0 + nn * k0), ax[0] * ax[1] * ax[2], scale);
-            fx_add(acc, i0 + nn * (j0 + nn * (k0 + 1)), ax[0] * ax[1] * bz, scale);
-            fx_add(acc, i0 + nn * ((j0 + 1) + nn * k0), ax[0] * by * ax[2], scale);
-            fx_add(acc, i0 + nn * ((j0 + 1) + nn * (k0 + 1)), ax[0] * by * bz, scale);
-            fx_add(acc, (i0 + 1) + nn * (j0 + nn * k0), bx * ax[1] * ax[2], scale);
-            fx_add(acc, (i0 + 1) + nn * (j0 + nn * (k0 + 1)), bx * ax[1] * bz, scale);
-            fx_add(acc, (i0 + 1) + nn * ((j0 + 1) + nn * k0), bx * by * ax[2], scale);
-            fx_add(acc, (i0 + 1) + nn * ((j0 + 1) + nn * (k0 + 1)), bx * by * bz, scale);
-          }
-        }
-        rel[3 * t] = r0;
-        rel[3 * t + 1] = r1;
-        rel[3 * t + 2] = r2;
-        wts[3 * t] = ax[0];
-        wts[3 * t + 1] = ax[1];
-        wts[3 * t + 2] = ax[2];
+    for (int s = lo + t; s < hi; s += CIC_THREADS) {
+      int ix[3];
+      float ax[3];
+      bool inside = true;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const float p = x0f + (float)(P[(PS_KX + d) * npad + s] - 1) * hf + D[(PS_DX + d) * npad + s];
+        ix[d] = locate_cell(gt, n, x0t, iwt, cells, nbin, p, ax[d]);
+        inside = inside && ix[d] > 0;
       }
-      __syncthreads();
-      for (int q = 0; q < chunk; ++q) {
-        const int rx = rel[3 * q];
-        if (rx < 0) continue;
-        const int ry = rel[3 * q + 1], rz = rel[3 * q + 2];
-        const float ax = wts[3 * q], ay = wts[3 * q + 1], az = wts[3 * q + 2];
-        const float wx = (tx == rx) ? ax : ((tx == rx + 1) ? 1.f - ax : 0.f);
-        const float wy = (ty == ry) ? ay : ((ty == ry + 1) ? 1.f - ay : 0.f);
-        const float wz = (tz == rz) ? az : ((tz == rz + 1) ? 1.f - az : 0.f);
-        tile += wx * wy * wz;
+      if (!inside) continue;
+      const float bx = 1.f - ax[0], by = 1.f - ax[1], bz = 1.f - ax[2];
+      const float c[8] = {ax[0] * ax[1] * ax[2], ax[0] * ax[1] * bz, ax[0] * by * ax[2],
+                          ax[0] * by * bz,       bx * ax[1] * ax[2], bx * ax[1] * bz,
+                          bx * by * ax[2],       bx * by * bz};
+      const int q0 = ix[0] - anc[0], q1 = ix[1] - anc[1], q2 = ix[2] - anc[2];
+      const bool fits = anchored && q0 >= 0 && q0 <= 2 && q1 >= 0 && q1 <= 2 && q2 >= 0 && q2 <= 2;
+      if (fits) {
+        const int base = q0 + 4 * (q1 + 4 * q2);
+        // corner (di, dj, dk) -> window point base + di + 4 dj + 16 dk
+        atomicAdd(&win[base], fx_of(c[0], scale));
+        atomicAdd(&win[base + 16], fx_of(c[1], scale));
+        atomicAdd(&win[base + 4], fx_of(c[2], scale));
+        atomicAdd(&win[base + 20], fx_of(c[3], scale));
+        atomicAdd(&win[base + 1], fx_of(c[4], scale));
+        atomicAdd(&win[base + 17], fx_of(c[5], scale));
+        atomicAdd(&win[base + 5], fx_of(c[6], scale));
+        atomicAdd(&win[base + 21], fx_of(c[7], scale));
+      } else {
+        const int64_t i0 = ix[0] - 1, j0 = ix[1] - 1, k0 = ix[2] - 1;
+        const int64_t nn = n;
+        fx_add(acc, i0 + nn * (j0 + nn * k0), c[0], scale);
+        fx_add(acc, i0 + nn * (j0 + nn * (k0 + 1)), c[1], scale);
+        fx_add(acc, i0 + nn * ((j0 + 1) + nn * k0), c[2], scale);
+        fx_add(acc, i0 + nn * ((j0 + 1) + nn * (k0 + 1)), c[3], scale);
+        fx_add(acc, (i0 + 1) + nn * (j0 + nn * k0), c[4], scale);
+        fx_add(acc, (i0 + 1) + nn * (j0 + nn * (k0 + 1)), c[5], scale);
+        fx_add(acc, (i0 + 1) + nn * ((j0 + 1) + nn * k0), c[6], scale);
+        fx_add(acc, (i0 + 1) + nn * ((j0 + 1) + nn * (k0 + 1)), c[7], scale);
       }
     }
-    if (anc[0] > 0 && anc[1] > 0 && anc[2] > 0) {
+    __syncthreads();
+    if (t < 64 && anchored) {
+      const int tx = t & 3, ty = (t >> 2) & 3, tz = t >> 4;
       const int64_t i = anc[0] - 1 + tx, j = anc[1] - 1 + ty, k = anc[2] - 1 + tz;
-      if (i < n && j < n && k < n) fx_add(acc, i + (int64_t)n * (j + (int64_t)n * k), tile, scale);
+      const unsigned long long v = win[t];
+      if (v != 0ull && i < n && j < n && k < n) atomicAdd(acc + i + (int64_t)n * (j + (int64_t)n * k), v);
     }
+    __syncthreads();
   }
 }
 
@@ -316,8 +332,8 @@ int vh_deposit_cic_impl(vh_accel *a, vh_mesh *m, double charge) {
   vh_context *c = a->ctx;
   hipStream_t st = c->stream;
   const double scale = ldexp(1.0, a->cic_shift);
-  int grid = 8 * c->num_cus;
-  hipLaunchKernelGGL(k_cic_tiles, dim3(grid), dim3(64), 0, st, a->parts[a->cur], a->npad,
+  const int grid = (int)(a->maxocc < 65535 ? a->maxocc : 65535);
+  hipLaunchKernelGGL(k_cic_tiles, dim3(grid), dim3(CIC_THREADS), 0, st, a->parts[a->cur], a->npad,
                      a->occupied, a->bounds, a->counters, a->nk, (float)a->x0, (float)a->h,
                      m->colloc, m->n, m->loc_x0, m->loc_iw, m->loc_cells, m->nbin, scale,
                      (unsigned long long *)m->acc64);

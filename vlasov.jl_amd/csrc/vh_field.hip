@@ -80,27 +80,20 @@ __global__ void __launch_bounds__(64)
             const int32_t *__restrict__ cnt, int nk, float x0, float h, float spacing, float nbdt,
             float w, int ncol, vh_proj_args pa, float *__restrict__ force,
             double *__restrict__ part) {
-  __shared__ __attribute__((aligned(16))) float tile[100 * 12];
   const float *D = (const float *)P;
   const int t = threadIdx.x;
   const float half = spacing * 0.5f;
   const float nan = __int_as_float(0x7fc00000);
   const int nocc = cnt[CNT_NOCC];
+  const int64_t nn = n;
   double sax = 0.0, say = 0.0, saz = 0.0, sae = 0.0;
   for (int g = blockIdx.x; g < nocc; g += gridDim.x) {
     const int key = occupied[g];
     const int kx = key % nk + 1, ky = (key / nk) % nk + 1, kz = key / (nk * nk) + 1;
     const int bx = 2 * kx - 5, by = 2 * ky - 5, bz = 2 * kz - 5;  // 1-based stencil corner
     const bool ok = bx >= 1 && by >= 1 && bz >= 1 && bx + 9 <= n && by + 9 <= n && bz + 9 <= n;
-    __syncthreads();
-    if (ok) {
-      for (int e = t; e < 1000; e += 64) {
-        const int ii = e % 10, jj = (e / 10) % 10, kk = e / 100;
-        tile[(kk * 10 + jj) * 12 + ii] =
-            csol[(bx - 1 + ii) + (int64_t)n * ((by - 1 + jj) + (int64_t)n * (bz - 1 + kk))];
-      }
-    }
-    __syncthreads();
+    // the cell's 10^3 block of csol, read at wave-uniform addresses
+    const float *tb = csol + (ok ? (bx - 1) + nn * ((by - 1) + nn * (bz - 1)) : 0);
     const int lo = bounds[g], hi = bounds[g + 1];
     for (int s = lo + t; s < hi; s += 64) {
       const float dx0 = D[PS_DX * npad + s], dy0 = D[PS_DY * npad + s], dz0 = D[PS_DZ * npad + s];
@@ -112,31 +105,27 @@ __global__ void __launch_bounds__(64)
         cx = min(max(cx, 1), ncol) - 1;
         cy = min(max(cy, 1), ncol) - 1;
         cz = min(max(cz, 1), ncol) - 1;
-        float ox[10], gx[10], oy[10], gy[10], oz[10], gz[10];
+        float ox[10], gx[10], oy[10], gy[10];
 #pragma unroll
         for (int a = 0; a < 10; ++a) {
           ox[a] = ovl[cx * 10 + a];
           gx[a] = grad[cx * 10 + a];
           oy[a] = ovl[cy * 10 + a];
           gy[a] = grad[cy * 10 + a];
-          oz[a] = ovl[cz * 10 + a];
-          gz[a] = grad[cz * 10 + a];
         }
         float fx = 0.f, fy = 0.f, fz = 0.f;
         for (int kk = 0; kk < 10; ++kk) {
-          const float ozk = oz[kk], gzk = gz[kk];
+          const float ozk = ovl[cz * 10 + kk], gzk = grad[cz * 10 + kk];
+          const float *plane = tb + nn * nn * kk;
 #pragma unroll
           for (int jj = 0; jj < 10; ++jj) {
-            const float *row = tile + (kk * 10 + jj) * 12;
-            const float4 c0 = *(const float4 *)row;
-            const float4 c1 = *(const float4 *)(row + 4);
-            const float2 c2 = *(const float2 *)(row + 8);
-            const float cv[10] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w, c2.x, c2.y};
+            const float *row = plane + nn * jj;
             float dxp = 0.f, val = 0.f;
 #pragma unroll
             for (int ii = 0; ii < 10; ++ii) {
-              dxp = __builtin_fmaf(cv[ii], gx[ii], dxp);
-              val = __builtin_fmaf(cv[ii], ox[ii], val);
+              const float c = row[ii];
+              dxp = __builtin_fmaf(c, gx[ii], dxp);
+              val = __builtin_fmaf(c, ox[ii], val);
             }
             fx = __builtin_fmaf(oy[jj] * ozk, dxp, fx);
             fy = __builtin_fmaf(gy[jj] * ozk, val, fy);
@@ -346,7 +335,7 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
     pa.sigma = (float)proj[4];
     pa.radius = (float)(ball_radius > 0 ? ball_radius : proj[4]);
   }
-  int grid = 8 * c->num_cus;
+  int grid = (int)(a->maxocc < 16384 ? a->maxocc : 16384);
   if (grid > a->dpart_len / 4) grid = (int)(a->dpart_len / 4);
   int tok = vh_prof_begin(c, VH_ST_FIELD_KERNEL);
   hipLaunchKernelGGL(k_field, dim3(grid), dim3(64), 0, st, a->parts[a->cur], a->npad, fine->csol,

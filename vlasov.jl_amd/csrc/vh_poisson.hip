@@ -16,59 +16,70 @@
 
 typedef float vh_f4 __attribute__((ext_vector_type(4)));
 
+// One block owns 32 output columns j (2 MFMA column tiles) and all n output
+// rows: the 32 x n source panel is staged in LDS once (k contiguous, padded
+// stride), and the four waves split the MT row tiles; A (n x n, at most
+// 266 KB) is read through the L1 at wave-uniform rows.
 template <int MT>
 __global__ void __launch_bounds__(256)
     k_rot_gemm(const float *__restrict__ A, const float *__restrict__ src, float *__restrict__ out,
                int n, int64_t m, int epi, const float *__restrict__ aux, int nfull) {
-  __shared__ float As[MT * 16][17];
-  __shared__ float Bs[64][17];
+  extern __shared__ float Bs[];
+  constexpr int RT = (MT + 3) / 4;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int64_t j0 = (int64_t)blockIdx.x * 64;
-  vh_f4 acc[MT];
-#pragma unroll
-  for (int r = 0; r < MT; ++r) acc[r] = (vh_f4){0.f, 0.f, 0.f, 0.f};
-  for (int kc = 0; kc < n; kc += 16) {
-    for (int e = threadIdx.x; e < MT * 16 * 16; e += 256) {
-      const int r = e >> 4, kk = e & 15, k = kc + kk;
-      As[r][kk] = (r < n && k < n) ? A[r * n + k] : 0.f;
-    }
-    for (int e = threadIdx.x; e < 64 * 16; e += 256) {
-      const int c = e >> 4, kk = e & 15, k = kc + kk;
-      const int64_t j = j0 + c;
-      Bs[c][kk] = (j < m && k < n) ? src[j * n + k] : 0.f;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int ks = 0; ks < 16; ks += 4) {
-      const float b = Bs[w * 16 + (l & 15)][ks + (l >> 4)];
-#pragma unroll
-      for (int r = 0; r < MT; ++r) {
-        const float av = As[r * 16 + (l & 15)][ks + (l >> 4)];
-        acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b, acc[r], 0, 0, 0);
-      }
-    }
-    __syncthreads();
+  const int ldb = n + 1;
+  const int64_t j0 = (int64_t)blockIdx.x * 32;
+  const int64_t ncols = (m - j0) < 32 ? (m - j0) : 32;
+  const float *panel = src + j0 * n;
+  for (int e = threadIdx.x; e < 32 * n; e += 256) {
+    const int c = e / n, k = e - c * n;
+    Bs[c * ldb + k] = (c < ncols) ? panel[e] : 0.f;
   }
-  const int64_t j = j0 + w * 16 + (l & 15);
-  if (j >= m) return;
+  __syncthreads();
+  vh_f4 acc[RT][2];
 #pragma unroll
-  for (int r = 0; r < MT; ++r) {
+  for (int rr = 0; rr < RT; ++rr) acc[rr][0] = acc[rr][1] = (vh_f4){0.f, 0.f, 0.f, 0.f};
+  const int lc = l & 15, lk = l >> 4;
+  for (int ks = 0; ks < n; ks += 4) {
+    const int k = ks + lk;
+    const bool kin = k < n;
+    const float b0 = kin ? Bs[lc * ldb + k] : 0.f;
+    const float b1 = kin ? Bs[(16 + lc) * ldb + k] : 0.f;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int row = r * 16 + 4 * (l >> 4) + v;
-      if (row >= n) continue;
-      const int64_t idx = (int64_t)row * m + j;
-      const float val = acc[r][v];
-      if (epi == EPI_STORE) {
-        out[idx] = val;
-      } else if (epi == EPI_HADAMARD) {
-        out[idx] = val * aux[idx];
-      } else if (epi == EPI_ACCUMULATE) {
-        out[idx] = out[idx] + val;
-      } else {  // EPI_FILL_INTERIOR: idx is (i, j, k) of the interior, i fastest
-        const int64_t nn = n;
-        const int64_t i = idx % nn, jj = (idx / nn) % nn, k = idx / (nn * nn);
-        out[(i + 1) + nfull * ((jj + 1) + (int64_t)nfull * (k + 1))] = val;
+    for (int rr = 0; rr < RT; ++rr) {
+      const int r = w + 4 * rr;
+      if (r >= MT) break;
+      const int row = r * 16 + lc;
+      const float av = (row < n && kin) ? A[row * n + k] : 0.f;
+      acc[rr][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b0, acc[rr][0], 0, 0, 0);
+      acc[rr][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b1, acc[rr][1], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int rr = 0; rr < RT; ++rr) {
+    const int r = w + 4 * rr;
+    if (r >= MT) break;
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+      const int64_t j = j0 + ct * 16 + lc;
+      if (j >= m) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = r * 16 + 4 * lk + v;
+        if (row >= n) continue;
+        const int64_t idx = (int64_t)row * m + j;
+        const float val = acc[rr][ct][v];
+        if (epi == EPI_STORE) {
+          out[idx] = val;
+        } else if (epi == EPI_HADAMARD) {
+          out[idx] = val * aux[idx];
+        } else if (epi == EPI_ACCUMULATE) {
+          out[idx] = out[idx] + val;
+        } else {  // EPI_FILL_INTERIOR: idx is (i, j, k) of the interior, i fastest
+          const int64_t nn = n;
+          const int64_t i = idx % nn, jj = (idx / nn) % nn, kz = idx / (nn * nn);
+          out[(i + 1) + nfull * ((jj + 1) + (int64_t)nfull * (kz + 1))] = val;
+        }
       }
     }
   }
@@ -78,12 +89,13 @@ int vh_rotate_gemm(vh_context *c, const float *A, const float *src, float *out, 
                    int epi, const float *aux, int nfull) {
   const int mt = (n + 15) / 16;
   VH_REQUIRE(mt >= 1 && mt <= 17, "rotating product: n = %d beyond 272", n);
-  dim3 grid((unsigned)((m + 63) / 64)), block(256);
+  dim3 grid((unsigned)((m + 31) / 32)), block(256);
+  const size_t lds = sizeof(float) * 32 * (size_t)(n + 1);
   switch (mt) {
-#define CASE(T)                                                                               \
-  case T:                                                                                     \
-    hipLaunchKernelGGL(k_rot_gemm<T>, grid, block, 0, c->stream, A, src, out, n, m, epi, aux, \
-                       nfull);                                                                \
+#define CASE(T)                                                                                 \
+  case T:                                                                                       \
+    hipLaunchKernelGGL(k_rot_gemm<T>, grid, block, lds, c->stream, A, src, out, n, m, epi, aux, \
+                       nfull);                                                                  \
     break;
     CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9)
     CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17)
