@@ -6,9 +6,12 @@ shape: 8e6 Float32 pseudo-particles, nfine = 64 (130^3 fine and coarse
 grids), dt = 0.5 a.u.; a "step" is one full step of the resident path (sort,
 smoothed and CIC deposits, nested Poisson, mean field, field gather with the
 projectile, Verlet).  By default the 700 steps of the crossing are run: 40
-untimed warm-up steps, then 660 timed.  Prints one JSON line.
+untimed warm-up steps, then 660 timed.  The crossing is then run again, untimed
+in the metric, at 8e7 particles, the converged run the capture charge
+q(< 8 a0) is compared with; then the CPU baseline.  Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--particles N] [--nfine n]
+                    [--reference-particles M] [--no-cpu-baseline]
 """
 from __future__ import annotations
 
@@ -70,6 +73,79 @@ def cpu_baseline(va, nfine: int, npart: int, steps: int):
                       f"nfine={nfine}, same Xe25+ crossing; {dt:.1f} s"}
 
 
+def copy_calibration(mib: int):
+    """A plain copy of `mib` MiB between two device buffers, five times: its
+    byte counters per dispatch calibrate those of the step's kernels."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copied {mib} MiB five times")
+
+
+def run_crossing(va, npart: int, nfine: int, mode: str, warmup: int, steps: int, profile: bool):
+    """The crossing at `npart` particles: `warmup` untimed steps, then `steps`
+    timed ones.  The energy budget is taken on the first warm-up step and on
+    the last timed step (two interaction-energy sweeps, inside the timed
+    region for the second)."""
+    t0 = time.perf_counter()
+    p = va.xenon_parameters(npart, nfine)
+    proj = va.xenon_projectile()
+    sim = va.Simulation(p, va.read_radial_profile(), projectile=proj, softening_mode=mode)
+    log(f"built in {time.perf_counter() - t0:.1f} s: {npart} particles, {sim.fine.n}^3 grids")
+    e_first = None
+    for i in range(warmup):
+        b = sim.step(energy=(i == 0))
+        if b is not None:
+            e_first = b
+    sim.ctx.sync()
+    log(f"warm-up {warmup} steps done")
+    nst = len(va._lib.STAGES)
+    ms = np.zeros(nst)
+    cnt = np.zeros(nst, dtype=np.int64)
+    if profile:
+        va._lib.call("vh_profile_enable", sim.ctx.handle, 1)
+    sim.ctx.sync()
+    t1 = time.perf_counter()
+    e_last = None
+    for i in range(steps):
+        b = sim.step(energy=(i == steps - 1))
+        if b is not None:
+            e_last = b
+        if (i + 1) % 100 == 0:
+            log(f"step {warmup + i + 1}: x_ion = {sim.projectile.position[0]:.1f}")
+    sim.ctx.sync()
+    elapsed = time.perf_counter() - t1
+    if profile:
+        va._lib.call("vh_profile_read", sim.ctx.handle, ms, cnt)
+        va._lib.call("vh_profile_enable", sim.ctx.handle, 0)
+    stages = {s: (ms[i] / cnt[i] if cnt[i] else 0.0) for i, s in enumerate(va._lib.STAGES)}
+    q = sim.enclosed_charge(8.0)
+    nocc, outcount = sim.acc.sort()
+    out = dict(elapsed=elapsed, stages=stages, q=q, nocc=nocc, outcount=outcount,
+               step=sim.step_count, e_first=e_first, e_last=e_last,
+               eloss_ev=sim.projectile.energy_loss() * va.physics.HARTREE_TO_EV)
+    sim.close()
+    return out
+
+
+def isolated_drift(va, npart: int, nfine: int, steps: int, every: int):
+    """Energy conservation of the isolated Na196 cluster, the reference's
+    validation observable (src/energy.jl:1-7): the budget every `every` steps
+    over `steps` steps, no projectile."""
+    p = va.xenon_parameters(npart, nfine)
+    sim = va.Simulation(p, va.read_radial_profile())
+    hist = sim.run(steps, energy_every=every)
+    sim.close()
+    tot = np.array([b.total for b in hist])
+    return {"particles": npart, "steps": steps, "every": every, "first": float(tot[0]),
+            "last": float(tot[-1]), "max_abs_drift": float(np.max(np.abs(tot - tot[0]))),
+            "drift_rel": float((tot[-1] - tot[0]) / abs(tot[0]))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,47 +154,35 @@ def main():
     ap.add_argument("--particles", type=int, default=8_000_000)
     ap.add_argument("--nfine", type=int, default=64)
     ap.add_argument("--mode", default="device", choices=["device", "ball"])
-    ap.add_argument("--cpu-particles", type=int, default=400_000)
+    ap.add_argument("--reference-particles", type=int, default=80_000_000,
+                    help="particle count of the converged run the capture charge is compared "
+                         "with (0: skip)")
+    ap.add_argument("--drift-steps", type=int, default=700,
+                    help="steps of the isolated-cluster energy-conservation run (0: skip)")
+    ap.add_argument("--cpu-particles", type=int, default=4_000_000)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many MiB, five times "
+                         "(the reference kernel for scaling HBM byte counters), and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the path has no "
               "multi-GPU decomposition in this build)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration_mib > 0:
+        copy_calibration(args.copy_calibration_mib)
+        return
 
     va = load_package()
-    t0 = time.perf_counter()
-    p = va.xenon_parameters(args.particles, args.nfine)
-    proj = va.xenon_projectile()
-    sim = va.Simulation(p, va.read_radial_profile(), projectile=proj, softening_mode=args.mode)
-    log(f"built in {time.perf_counter() - t0:.1f} s: {args.particles} particles, "
-        f"{sim.fine.n}^3 grids")
-    for i in range(args.warmup):
-        sim.step()
-    sim.ctx.sync()
-    log(f"warm-up {args.warmup} steps done")
-    va._lib.call("vh_profile_enable", sim.ctx.handle, 1)
-    ms = np.zeros(len(va._lib.STAGES))
-    cnt = np.zeros(len(va._lib.STAGES), dtype=np.int64)
-    sim.ctx.sync()
-    t1 = time.perf_counter()
-    for i in range(args.steps):
-        sim.step()
-        if (i + 1) % 100 == 0:
-            log(f"step {args.warmup + i + 1}: x_ion = {sim.projectile.position[0]:.1f}")
-    sim.ctx.sync()
-    elapsed = time.perf_counter() - t1
-    va._lib.call("vh_profile_read", sim.ctx.handle, ms, cnt)
-    va._lib.call("vh_profile_enable", sim.ctx.handle, 0)
+    r = run_crossing(va, args.particles, args.nfine, args.mode, args.warmup, args.steps, True)
+    elapsed, stages, outcount = r["elapsed"], r["stages"], r["outcount"]
     ms_step = 1000.0 * elapsed / args.steps
-    stages = {s: (ms[i] / cnt[i] if cnt[i] else 0.0) for i, s in enumerate(va._lib.STAGES)}
-    q = sim.enclosed_charge(8.0)
-    nocc, outcount = sim.acc.sort()
-    # roofline of the dominant kernel: the smoothed-field contraction (FP32 VALU)
+    # roofline of the dominant kernel: the smoothed-field contraction (FP32)
     field_ms = stages["field_kernel"]
     inside = args.particles - outcount
     achieved = inside * FIELD_FLOP_PER_PARTICLE / (field_ms * 1e-3) / 1e12 if field_ms else 0.0
+    e0, e1 = r["e_first"], r["e_last"]
     line = {
         "metric": METRIC,
         "value": args.particles * args.steps / elapsed,
@@ -134,9 +198,9 @@ def main():
         "data": "synthetic: Thomas-Fermi Na196 cloud sampled from the reference's hm1.dat/rhoinit.dat "
                 "with its ran2 seed -1",
         "config": {"workload": f"Na196 + Xe25+ 524.8 keV b=45 a0 crossing, {args.particles} "
-                               f"pseudo-particles, nfine={args.nfine} ({sim.fine.n}^3 fine + coarse), "
+                               f"pseudo-particles, nfine={args.nfine} (130^3 fine + coarse), "
                                f"dt=0.5, steps {args.warmup + 1}-{args.warmup + args.steps} of 700",
-                   "particles": args.particles, "grid": sim.fine.n, "softening": args.mode},
+                   "particles": args.particles, "grid": 2 * args.nfine + 2, "softening": args.mode},
         "roofline": {"bound": "mfma", "engine": "VALU FP32 fma (packed FP32 disabled)",
                      "kernel": "k_field",
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -144,12 +208,28 @@ def main():
                      "per_launch_flop": inside * FIELD_FLOP_PER_PARTICLE,
                      "avg_launch_ms": field_ms},
         "stages_ms": stages,
-        "capture_q8": q,
-        "step": sim.step_count,
+        "capture_q8": r["q"],
+        "step": r["step"],
         "outcount": outcount,
-        "occupied_cells": nocc,
+        "occupied_cells": r["nocc"],
+        "projectile_energy_loss_eV": r["eloss_ev"],
+        # electrons only: the projectile's interaction is not in the budget
+        "electron_energy": {"step1": e0.total if e0 else None, "last": e1.total if e1 else None},
     }
-    sim.close()
+    if args.drift_steps > 0:
+        log("isolated cluster energy drift...")
+        line["isolated_energy_drift"] = isolated_drift(va, args.particles, args.nfine,
+                                                       args.drift_steps, 10)
+    if args.reference_particles > 0:
+        log(f"converged run: {args.reference_particles} particles...")
+        rr = run_crossing(va, args.reference_particles, args.nfine, args.mode, args.warmup,
+                          args.steps, False)
+        line["capture_reference"] = {
+            "particles": args.reference_particles, "capture_q8": rr["q"],
+            "capture_q_err": abs(r["q"] - rr["q"]),
+            "ms_per_step": 1000.0 * rr["elapsed"] / args.steps,
+            "projectile_energy_loss_eV": rr["eloss_ev"],
+            "published_q8": {"8e6 (RTX 4070 / M1 Max)": [0.842, 0.840], "8e7 (M1 Max)": 0.81}}
     if not args.no_cpu_baseline:
         log("cpu baseline...")
         line["cpu_baseline"] = cpu_baseline(va, args.nfine, args.cpu_particles, args.cpu_steps)

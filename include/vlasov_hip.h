@@ -181,6 +181,18 @@ int vh_verlet(vh_accel *a, double dt, double mass, double rcmax, double diag[5])
  * particles closer than `radius` to `center` (multiply by the weight). */
 int vh_enclosed_count(vh_accel *a, const double center[3], double radius, int64_t *count);
 
+/* interaction_energy(cloud, fine, csol_f, coarse, csol_c, sm; enclosed)
+ * (src/energy.jl:19-43): sum over the particles of weight * phi(r), in FP64,
+ * with phi the smoothed potential of the fine csol inside
+ * (knots[3], knots[end-2])^3 (src/fields.jl:405-428), the spline potential of
+ * the coarse csol elsewhere (src/fields.jl:440-466), and enclosed / |r|
+ * beyond both grids.  Reads the csol grids of `fine` and `coarse` as they
+ * stand: called between vh_spline_coefficients and vh_effective_potential it
+ * gives twice the Hartree energy (src/simulation.jl:501-510); after the mean
+ * field, the total interaction of the budget (src/simulation.jl:382-396). */
+int vh_interaction_energy(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight,
+                          double enclosed, double *energy);
+
 /* --- timing ---------------------------------------------------------------
  * HIP events recorded on the context's stream around each stage of the step
  * (and around the field kernel alone), for per-stage ms/step as

@@ -64,6 +64,9 @@ def lib():
         L.or_spline_potential.argtypes = [vp, i32, vp, i32, vp, i64, vp]
         L.or_fallback.argtypes = [vp, vp, i64, f64, f64, vp, i32, vp, i32, f64, vp, i32, f64, f64, vp]
         L.or_verlet_f32.argtypes = [vp, vp, i64, f32, f32, f32, f32, f32, f32, vp]
+        L.or_interaction_energy.argtypes = [vp, i64, f64, f64, vp, i32, vp, i32, vp, f64, i32, i32,
+                                            vp, i32, vp, i32, f64, f64, vp]
+        L.or_interaction_energy.restype = f64
         L.or_enclosed.argtypes = [vp, i64, f64, f64, vp, f64]
         L.or_enclosed.restype = i64
         L.or_num_threads.restype = i32
@@ -154,6 +157,24 @@ def spline_potential(knots, csol, pts):
     out = np.zeros(len(p))
     lib().or_spline_potential(_p(kc), len(kc), _p(cs), nc, _p(p), len(p), _p(out))
     return out
+
+
+def interaction_energy(rec, x0, h, knots_f, csolf, ovl64, spacing, nbdt, knots_c, csolc, w,
+                       enclosed=0.0, per_particle=False):
+    """interaction_energy (src/energy.jl:19-43) in FP64: the total, and each
+    particle's phi if `per_particle`."""
+    kf = np.ascontiguousarray(knots_f, dtype=np.float64)
+    kc = np.ascontiguousarray(knots_c, dtype=np.float64)
+    cf = np.ascontiguousarray(csolf, dtype=np.float64)
+    cc = np.ascontiguousarray(csolc, dtype=np.float64)
+    ov = np.ascontiguousarray(ovl64, dtype=np.float64)
+    nf, nc = round(len(cf) ** (1 / 3)), round(len(cc) ** (1 / 3))
+    r = np.ascontiguousarray(rec)
+    phi = np.zeros(len(r)) if per_particle else None
+    tot = lib().or_interaction_energy(_p(r), len(r), x0, h, _p(kf), len(kf), _p(cf), nf, _p(ov),
+                                      spacing, int(nbdt), int(nbdt) + 1, _p(kc), len(kc), _p(cc), nc,
+                                      w, enclosed, None if phi is None else _p(phi))
+    return (tot, phi) if per_particle else tot
 
 
 def verlet_f32(rec, force, dt, mass, h, x0, rcmax):
@@ -368,19 +389,27 @@ class OracleStep:
                      ball, radius, 0.0, force=f)
         return f, red, len(outl)
 
-    def update_forces(self, rec, projectile=None, mode="device"):
+    def energy(self, rec, csol_f, csol_c, enclosed=0.0):
+        """interaction_energy of the cloud on the given coefficients."""
+        sm = self.sm
+        return interaction_energy(rec, self.x0, self.h, self.fine.axis.knots, csol_f, sm.overlap,
+                                  float(sm.spacing), sm.nbdt, self.coarse.axis.knots, csol_c, self.w,
+                                  enclosed)
+
+    def update_forces(self, rec, projectile=None, mode="device", energy=False):
         srec, occ, bnd = self.sort(rec)
         rho_f, nout = self.deposit_fine(srec, occ, bnd)
         rho_c = self.deposit_coarse(srec)
         phi_f, phi_c, mp = poisson_nested(rho_f, rho_c, self.fine, self.coarse)
         csol_f = spline_coefficients(phi_f, self.fine)
         csol_c = spline_coefficients(phi_c, self.coarse)
+        hartree = self.energy(srec, csol_f, csol_c) / 2 if energy else None
         csol_f = effective_potential(csol_f, rho_f, self.fine, self.jel)
         csol_c = effective_potential(csol_c, rho_c, self.coarse, self.jel)
         f, red, outcount = self.forces(srec, occ, bnd, csol_f, csol_c, projectile, mode)
         return dict(srec=srec, occ=occ, bnd=bnd, rho_f=rho_f, rho_c=rho_c, phi_f=phi_f,
                     phi_c=phi_c, csol_f=csol_f, csol_c=csol_c, force=f, red=red, nout=nout,
-                    outcount=outcount, mp=mp)
+                    outcount=outcount, mp=mp, hartree=hartree)
 
     def verlet(self, srec, force):
         rec = srec.copy()

@@ -383,6 +383,76 @@ void or_spline_potential(const double *knots, int nk, const double *csol, int nc
   }
 }
 
+/* nearest_knot (src/fields.jl:224-227), 0-based.  searchsortedlast by a
+ * linear scan (the number of knots <= x), clamped to [1, nk-1], then the
+ * closer end of that interval, the left one on a tie. */
+static int nearest_knot(const double *g, int nk, double x) {
+  int i = 0;
+  while (i < nk && g[i] <= x) ++i;
+  i = clampi(i, 1, nk - 1);
+  return (g[i] - x) < (x - g[i - 1]) ? i : i - 1;
+}
+
+/* smoothed_potential (src/fields.jl:405-428): the potential convolved with
+ * the particle's Gaussian, FP64, the reference's loop order and roundings
+ * (s += c * ox, then phi += s * (oy * oz)). */
+static double smoothed_potential(const double *kf, int nkf, const double *csol, int nf,
+                                 const double *ovl, double spacing, int nbdt, int ncol,
+                                 const double *p) {
+  int64_t b[3];
+  const double *o[3];
+  for (int d = 0; d < 3; ++d) {
+    int ci = nearest_knot(kf, nkf, p[d]);
+    b[d] = 2 * (int64_t)(ci + 1) - 5 - 1;
+    int col = (int)floor((p[d] - kf[ci] + spacing / 2) / spacing * (double)nbdt + 0.5) + 1;
+    o[d] = ovl + 10 * (clampi(col, 1, ncol) - 1);
+  }
+  int64_t n = nf;
+  double phi = 0.0;
+  for (int kk = 0; kk < 10; ++kk) {
+    for (int jj = 0; jj < 10; ++jj) {
+      double w = o[1][jj] * o[2][kk];
+      double s = 0.0;
+      for (int ii = 0; ii < 10; ++ii) s += csol[b[0] + ii + n * ((b[1] + jj) + n * (b[2] + kk))] * o[0][ii];
+      phi += s * w;
+    }
+  }
+  return phi;
+}
+
+/* interaction_energy (src/energy.jl:19-43): sum_i w phi(r_i) with r from the
+ * current (k, delta) as T = Float64 (src/particles.jl:128-132).  Inside
+ * (knots[3], knots[end-2])^3 of the fine axis: smoothed_potential of the fine
+ * csol; otherwise spline_potential of the coarse csol, or enclosed / |r|
+ * beyond both grids.  phi_out (may be NULL) receives each particle's phi. */
+double or_interaction_energy(const rec_t *parts, int64_t n, double x0, double h, const double *kf,
+                             int nkf, const double *csolf, int nf, const double *ovl,
+                             double spacing, int nbdt, int ncol, const double *kc, int nkc,
+                             const double *csolc, int nc, double w, double enclosed,
+                             double *phi_out) {
+  const double lo = kf[2], hi = kf[nkf - 3];
+  double total = 0.0;
+#pragma omp parallel for reduction(+ : total)
+  for (int64_t i = 0; i < n; ++i) {
+    double p[3];
+    int inside = 1;
+    for (int d = 0; d < 3; ++d) {
+      p[d] = x0 + (double)(parts[i].k[d] - 1) * h + (double)parts[i].d[d];
+      inside = inside && lo < p[d] && p[d] < hi;
+    }
+    double phi;
+    if (inside) {
+      phi = smoothed_potential(kf, nkf, csolf, nf, ovl, spacing, nbdt, ncol, p);
+    } else {
+      or_spline_potential(kc, nkc, csolc, nc, p, 1, &phi);
+      if (isnan(phi)) phi = enclosed / sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    }
+    if (phi_out) phi_out[i] = phi;
+    total += w * phi;
+  }
+  return total;
+}
+
 /* spline_field (src/fields.jl:358-393); returns 0 outside (with out = 0) */
 static int spline_field(const double *knots, int nk, const double *csol, int nc, const double *p,
                         double *e) {

@@ -241,3 +241,54 @@ def test_bad_arguments_refused_before_launch(setup):
                      None, None)
     with pytest.raises(ValueError):
         s["acc"].upload(np.zeros(3, dtype=va.PACKED_DTYPE))
+
+
+def test_interaction_energy(setup, orc):
+    """vh_interaction_energy against the FP64 oracle on the same coefficients:
+    the same per-particle arithmetic, summed in another order.  The cloud has
+    particles in all three regions (smoothed fine, coarse spline, beyond both
+    grids)."""
+    s = setup
+    srec, occ, bnd, csol_f, csol_c = _forces_setup(s, orc, seed=7)
+    fine, coarse, sm = s["fine"], s["coarse"], s["sm"]
+    for enclosed in (0.0, 3.0):
+        got = s["acc"].interaction_energy(s["dmf"], s["dmc"], s["w"], enclosed)
+        want = orc.interaction_energy(srec, s["x0"], s["h"], fine.axis.knots, csol_f.astype(np.float64),
+                                      sm.overlap, float(sm.spacing), sm.nbdt, coarse.axis.knots,
+                                      csol_c.astype(np.float64), s["w"], enclosed)
+        assert abs(got - want) <= 1e-12 * abs(want)
+
+
+def test_step_with_energy_budget(config1, orc):
+    """One resident step with the budget: the Hartree energy on the step's
+    own Hartree potential, the kinetic energy of the push and the total
+    interaction at the pushed positions, against the FP64 oracle."""
+    p, fine, coarse, sm = config1
+    n = NPART
+    w = p.nelectrons / n
+    pos, mom = va.sample_thomas_fermi(va.read_radial_profile(), n, w, seed=-3)
+    proj = va.xenon_projectile(x0=-20.0, impact=10.0)
+    sim = va.Simulation(p, None, projectile=proj, tables=(fine, coarse, sm), positions=pos,
+                        momenta=mom)
+    try:
+        rec0 = sim.acc.download()
+        st = orc.OracleStep(fine, coarse, sm, va.Jellium.of(p.nions), w, sim.mass, p.dt, p.rcmax)
+        proj_before = va.physics.Projectile(**vars(sim.projectile))
+        ref = st.update_forces(rec0, proj_before, energy=True)
+        b = sim.step(energy=True)
+        f_gpu = sim.acc.forces()
+        assert relerr(f_gpu, ref["force"]) < 1e-4
+        # the push on the device's own forces is the oracle's to the bit
+        rec1, d = st.verlet(ref["srec"], f_gpu)
+        assert sim.acc.download().tobytes() == rec1.tobytes()
+        assert abs(b.kinetic - d[0]) <= 1e-10 * abs(d[0])
+        assert abs(b.hartree - ref["hartree"]) <= REL * abs(ref["hartree"])
+        total = st.energy(rec1, ref["csol_f"], ref["csol_c"])
+        scale = max(abs(total), abs(b.hartree))
+        # the mean field by difference, as the budget takes it; the Hartree and
+        # jellium parts of the total potential cancel to a fraction of either
+        assert abs(b.meanfield - (total - 2 * ref["hartree"])) <= 2 * REL * scale
+        assert b.total == b.ions + b.kinetic + b.hartree + b.meanfield
+        assert b.ions == 3 * p.nions ** 2 / (5 * va.Jellium.of(p.nions).radius)
+    finally:
+        sim.close()

@@ -246,3 +246,83 @@ def test_oracle_step_runs_and_conserves(orc, config1):
     assert abs(orc.total_charge(r["rho_c"], coarse.dual_moments[0]) - 196.0) < 1e-9
     assert np.all(np.isfinite(r["force"]))
     assert abs(r["mp"][0] - 196.0) < 1e-9
+
+
+def _hermite_dofs(ax, f, df):
+    """Coefficients of the cubic Hermite interpolant of f: value, slope per knot."""
+    a = np.zeros(ax.nbasis)
+    a[0::2] = f(ax.knots)
+    a[1::2] = df(ax.knots)
+    return a
+
+
+def _tensor(fx, fy, fz):
+    """Julia-order flat tensor product (x fastest)."""
+    return (fz[:, None, None] * fy[None, :, None] * fx[None, None, :]).ravel()
+
+
+def _separable(ax, terms):
+    """sum over terms of fx(x) fy(y) fz(z), each factor given as (f, df)."""
+    return sum(_tensor(*(_hermite_dofs(ax, *t) for t in term)) for term in terms)
+
+
+ONE = (lambda x: np.ones_like(x), lambda x: np.zeros_like(x))
+LIN = (lambda x: x, lambda x: np.ones_like(x))
+CUB = (lambda x: x ** 3 / 1e4, lambda x: 3 * x ** 2 / 1e4)
+SQR = (lambda x: -x ** 2 / 50, lambda x: -x / 25)
+
+
+def test_interaction_energy_closed_forms(orc, config1):
+    """interaction_energy (src/energy.jl:19-43), branch by branch:
+    * inside (knots[3], knots[end-2])^3 the smoothed potential of a linear
+      Phi is Phi itself (a symmetric kernel), up to the tables' normalisation
+      defect -- an error of one table column (h/1000) or one knot fails it;
+    * elsewhere the coarse spline reproduces a cubic exactly;
+    * beyond both grids, enclosed / |r|;
+    * the region test is the strict one on FP64 positions."""
+    p, fine, coarse, sm = config1
+    kf, kc = fine.axis.knots, coarse.axis.knots
+    x0, h = float(kf[0]), float((kf[-1] - kf[0]) / (len(kf) - 1))
+    lin3 = [(LIN, ONE, ONE), (ONE, LIN, ONE), (ONE, ONE, LIN)]
+    csol_f = _separable(fine.axis, [(LIN, ONE, ONE)]) + 2 * _separable(fine.axis, [(ONE, LIN, ONE)]) \
+        + 3 * _separable(fine.axis, [(ONE, ONE, LIN)])
+    cubic = [(CUB, ONE, ONE), (ONE, LIN, ONE), (ONE, ONE, SQR), (ONE, ONE, ONE)]
+    csol_c = _separable(coarse.axis, cubic)
+
+    def cubic_at(q):
+        return q[:, 0] ** 3 / 1e4 + q[:, 1] - q[:, 2] ** 2 / 50 + 1
+
+    rng = np.random.default_rng(17)
+    pos = np.concatenate([rng.uniform(-70, 70, (3000, 3)),          # mostly inside
+                          rng.uniform(-230, 230, (3000, 3)),        # mostly coarse
+                          rng.uniform(-500, 500, (500, 3)) + 600])  # beyond both
+    rec = va.physics.packed_records(pos, pos, x0, h)
+    q = va.physics.unsplit(rec["knode"], rec["delta"], x0, h)
+    w, encl = 0.37, 7.5
+    tot, phi = orc.interaction_energy(rec, x0, h, kf, csol_f, sm.overlap, float(sm.spacing), sm.nbdt,
+                                      kc, csol_c, w, encl, per_particle=True)
+    inside = np.all((q > kf[2]) & (q < kf[-3]), axis=1)
+    beyond = np.any((q < kc[0]) | (q > kc[-1]), axis=1)
+    coarse_pts = ~inside & ~beyond
+    assert inside.sum() > 1000 and coarse_pts.sum() > 1000 and beyond.sum() > 400
+    assert np.max(np.abs(phi[coarse_pts] - cubic_at(q[coarse_pts]))) < 1e-9
+    assert np.allclose(phi[beyond], encl / np.linalg.norm(q[beyond], axis=1), rtol=1e-14)
+    assert math.isclose(tot, w * phi.sum(), rel_tol=1e-12)
+    # the smoothed potential of a constant is the kernel's normalisation (~1)
+    one_f = _separable(fine.axis, [(ONE, ONE, ONE)])
+    _, phi1 = orc.interaction_energy(rec, x0, h, kf, one_f, sm.overlap, float(sm.spacing), sm.nbdt,
+                                     kc, np.zeros_like(csol_c), 1.0, 0.0, per_particle=True)
+    assert np.max(np.abs(phi1[inside] - 1.0)) < 5e-5
+    assert np.all(phi1[~inside] == 0.0)
+    # normalised by it, the linear potential comes back to Phi at the
+    # tabulated offset (the table samples the offset from the nearest knot
+    # every spacing/nbdt), up to the kernel's first-moment defect: far below
+    # the spacing/nbdt x |grad Phi| of a one-column slip or a knot slip
+    qi = q[inside]
+    ci = np.argmin(np.abs(qi[:, :, None] - kf[None, None, :]), axis=2)   # ties: first (left)
+    sp = float(sm.spacing)
+    col = np.floor((qi - kf[ci] + sp / 2) / sp * sm.nbdt + 0.5)
+    xq = kf[ci] + col * sp / sm.nbdt - sp / 2
+    lin_q = xq[:, 0] + 2 * xq[:, 1] + 3 * xq[:, 2]
+    err = np.max(np.abs(phi[inside] / phi1[inside] - lin_q))
+    assert err < 0.05 * sp / sm.nbdt

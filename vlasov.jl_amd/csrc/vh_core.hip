@@ -268,7 +268,7 @@ extern "C" int vh_accel_destroy(vh_accel *a) {
   if (!a) return VH_OK;
   hipSetDevice(a->ctx->device);
   hipStreamSynchronize(a->ctx->stream);
-  void *ptrs[] = {a->ovl, a->grad, a->nodes, a->parts[0], a->parts[1], a->force,
+  void *ptrs[] = {a->ovl, a->grad, a->nodes, a->ovl64, a->knots64, a->parts[0], a->parts[1], a->force,
                   a->keys[0], a->keys[1], a->vals[0], a->vals[1], a->counts, a->scan_tmp,
                   a->occupied, a->bounds, a->cellmap, a->fitlist, a->nonfit, a->nonfit_prefix,
                   a->outlist, a->counters, a->flags, a->tiles, a->dpart, a->dscal, a->cellscratch};
@@ -305,6 +305,9 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
   a->x0 = x0;
   a->h = h;
   a->spacing = (float)spacing;
+  a->spacing64 = spacing;
+  a->klo = knots[2];
+  a->khi = knots[nk - 3];
   a->lo = (float)(knots[1] + spacing / 2);
   a->hi = (float)(knots[nk - 2] - spacing / 2);
   a->nbdt = nbdt;
@@ -333,6 +336,8 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
   MK(upload_converted(ctx, &a->ovl, overlap, (size_t)10 * a->ncol));
   MK(upload_converted(ctx, &a->grad, gradient, (size_t)10 * a->ncol));
   MK(upload_converted(ctx, &a->nodes, nodes, (size_t)8 * a->ncol));
+  MK(upload_converted(ctx, &a->ovl64, overlap, (size_t)10 * a->ncol));
+  MK(upload_converted(ctx, &a->knots64, knots, (size_t)nk));
   size_t np = (size_t)a->npad;
   MK(vh_alloc((void **)&a->parts[0], np * 4 * PS_COUNT));
   MK(vh_alloc((void **)&a->parts[1], np * 4 * PS_COUNT));
@@ -742,6 +747,22 @@ extern "C" int vh_forces(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double wei
   if (red) VH_CHECK_HIP(hipMemcpyAsync(red, a->dscal + 8, 32, hipMemcpyDeviceToHost, a->ctx->stream));
   if (nout) VH_CHECK_HIP(hipMemcpyAsync(nout, a->counters + CNT_OUTCOUNT, 4, hipMemcpyDeviceToHost, a->ctx->stream));
   if (red || nout) VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
+  return VH_OK;
+}
+
+extern "C" int vh_interaction_energy(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight,
+                                     double enclosed, double *energy) {
+  VH_REQUIRE(a && fine && coarse && energy, "vh_interaction_energy: null argument");
+  VH_REQUIRE(fine->n == a->n, "vh_interaction_energy: fine csol has n=%d, accelerator n=%d",
+             fine->n, a->n);
+  VH_REQUIRE(fine->ctx == a->ctx && coarse->ctx == a->ctx,
+             "vh_interaction_energy: objects on different contexts");
+  VH_REQUIRE(std::isfinite(weight) && std::isfinite(enclosed),
+             "vh_interaction_energy: weight and enclosed charge must be finite");
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  VH_TRY(vh_energy_impl(a, fine, coarse, weight, enclosed));
+  VH_CHECK_HIP(hipMemcpyAsync(energy, a->dscal + 18, 8, hipMemcpyDeviceToHost, a->ctx->stream));
+  VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
   return VH_OK;
 }
 

@@ -317,6 +317,137 @@ __global__ void k_reduce_cols(const double *__restrict__ part, int nparts, int n
   }
 }
 
+// ---------------------------------------------------------------------------
+// interaction energy sum_i w phi(r_i), FP64 (src/energy.jl:19-43)
+// ---------------------------------------------------------------------------
+struct vh_energy_args {
+  double x0, h;        // PackedPositions x0, h: r = x0 + (k - 1) h + delta
+  double lo, hi;       // fine knots[3], knots[end-2]: the smoothed-potential region
+  double spacing;      // smoothing table spacing
+  double nbdt;
+  double w;            // weight
+  double enclosed;     // charge seen beyond both grids
+  int nkf, nf;         // fine knots, fine grid size
+  int nkc, nc;         // coarse knots, coarse grid size
+  int ncol;
+};
+
+// nearest_knot (src/fields.jl:224-227), 0-based: searchsortedlast, clamped to
+// a cell, then the closer end of that cell
+__device__ __forceinline__ int nearest_knot_d(const double *__restrict__ g, int nk, double x) {
+  int lo = 0, hi = nk;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (g[mid] <= x) lo = mid + 1;
+    else hi = mid;
+  }
+  const int i = lo < 1 ? 1 : (lo > nk - 1 ? nk - 1 : lo);  // 1-based searchsortedlast
+  return (g[i] - x) < (x - g[i - 1]) ? i : i - 1;
+}
+
+// One lane per particle, in the buffer's current order (sorted by cell after
+// the step's sort, so neighbouring lanes read the same csol rows).  Inside
+// (lo, hi)^3: smoothed_potential (src/fields.jl:405-428) of the fine csol;
+// elsewhere spline_potential (src/fields.jl:440-466) of the coarse csol, or
+// enclosed / |r| beyond both grids.  Every product and sum in the reference's
+// order, in FP64, on the FP32 coefficients widened exactly.
+__global__ void __launch_bounds__(256)
+    k_energy(const int32_t *__restrict__ P, int64_t npad, int64_t npart,
+             const float *__restrict__ csolf, const double *__restrict__ kf,
+             const double *__restrict__ ovl, const float *__restrict__ csolc,
+             const double *__restrict__ kc, vh_energy_args ea, double *__restrict__ part) {
+  __shared__ double lds[8];
+  const float *D = (const float *)P;
+  const int64_t nf = ea.nf, nc = ea.nc;
+  double total = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npart;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double p[3];
+    bool inside = true;
+    for (int d = 0; d < 3; ++d) {
+      p[d] = ea.x0 + (double)(P[(PS_KX + d) * npad + i] - 1) * ea.h + (double)D[(PS_DX + d) * npad + i];
+      inside = inside && ea.lo < p[d] && p[d] < ea.hi;
+    }
+    double phi = 0.0;
+    if (inside) {
+      int64_t b[3];
+      const double *o[3];
+      for (int d = 0; d < 3; ++d) {
+        const int ci = nearest_knot_d(kf, ea.nkf, p[d]);
+        b[d] = 2 * ci - 4;  // 0-based 2ci - 5
+        int col = (int)floor((p[d] - kf[ci] + ea.spacing / 2) / ea.spacing * ea.nbdt + 0.5) + 1;
+        col = col < 1 ? 1 : (col > ea.ncol ? ea.ncol : col);
+        o[d] = ovl + 10 * (col - 1);
+      }
+      for (int kk = 0; kk < 10; ++kk) {
+        const double oz = o[2][kk];
+        for (int jj = 0; jj < 10; ++jj) {
+          const double wgt = o[1][jj] * oz;
+          const float *row = csolf + b[0] + nf * ((b[1] + jj) + nf * (b[2] + kk));
+          double s = 0.0;
+#pragma unroll
+          for (int ii = 0; ii < 10; ++ii) s += (double)row[ii] * o[0][ii];
+          phi += s * wgt;
+        }
+      }
+    } else {
+      int cell[3];
+      bool out = false;
+      for (int d = 0; d < 3; ++d) {
+        cell[d] = cell_index_d(kc, ea.nkc, p[d]);
+        out = out || cell[d] < 0;
+      }
+      if (out) {
+        phi = ea.enclosed / sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+      } else {
+        double v[3][4], dv;
+        for (int d = 0; d < 3; ++d)
+          for (int a = 0; a < 4; ++a) hermite_vd(kc, ea.nkc, 2 * cell[d] + a, p[d], v[d][a], dv);
+        for (int ak = 0; ak < 4; ++ak) {
+          const int64_t k = 2 * cell[2] + ak;
+          for (int aj = 0; aj < 4; ++aj) {
+            const int64_t j = 2 * cell[1] + aj;
+            const double wgt = v[1][aj] * v[2][ak];
+            for (int ai = 0; ai < 4; ++ai)
+              phi += (double)csolc[2 * cell[0] + ai + nc * (j + nc * k)] * v[0][ai] * wgt;
+          }
+        }
+      }
+    }
+    total += ea.w * phi;
+  }
+  const double s = vh_block_sum_d(total, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+#define ENERGY_BLOCKS 2048
+
+int vh_energy_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, double enclosed) {
+  vh_context *c = a->ctx;
+  vh_energy_args ea;
+  ea.x0 = a->x0;
+  ea.h = a->h;
+  ea.lo = a->klo;
+  ea.hi = a->khi;
+  ea.spacing = a->spacing64;
+  ea.nbdt = (double)a->nbdt;
+  ea.w = weight;
+  ea.enclosed = enclosed;
+  ea.nkf = a->nk;
+  ea.nf = a->n;
+  ea.nkc = coarse->nk;
+  ea.nc = coarse->n;
+  ea.ncol = a->ncol;
+  hipLaunchKernelGGL(k_energy, dim3(ENERGY_BLOCKS), dim3(256), 0, c->stream, a->parts[a->cur],
+                     a->npad, a->npart, fine->csol, a->knots64, a->ovl64, coarse->csol,
+                     coarse->knots64, ea, a->dpart);
+  VH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, c->stream, a->dpart, ENERGY_BLOCKS, 1,
+                     a->dscal + 18);
+  VH_LAUNCH_CHECK();
+  return VH_OK;
+}
+
 int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, const double *proj,
                    int soft_mode, double ball_radius) {
   vh_context *c = a->ctx;

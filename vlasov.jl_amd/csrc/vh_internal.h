@@ -117,6 +117,10 @@ struct vh_accel {
   float spacing;
   int nbdt, ncol;
   float *ovl, *grad, *nodes;   // [col*10 + a], [col*8 + a]
+  double *ovl64;               // overlap in FP64 (energy budget)
+  double *knots64;             // fine knots in FP64
+  double spacing64;            // smoothing spacing in FP64
+  double klo, khi;             // knots[3], knots[end-2]: smoothed-potential region
   int32_t *parts[2];           // 12 streams each
   int cur;
   float *force;                // 3 streams (fx, fy, fz), slot order
@@ -204,3 +208,4 @@ int vh_total_charge_impl(vh_mesh *m, const float *rho, double *dq);
 int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, const double *proj,
                    int soft_mode, double ball_radius);
 int vh_verlet_impl(vh_accel *a, double dt, double mass, double rcmax);
+int vh_energy_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, double enclosed);

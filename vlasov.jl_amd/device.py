@@ -241,6 +241,17 @@ class DeviceAccelerator:
         _lib.call("vh_verlet", self.handle, float(dt), float(mass), float(rcmax), d.ctypes.data)
         return {"kinetic": d[0], "escaped": d[1], "angular": (d[2], d[3], d[4])}
 
+    def interaction_energy(self, fine: DeviceMesh, coarse: DeviceMesh, weight: float,
+                           enclosed: float = 0.0) -> float:
+        """interaction_energy(cloud, fine, csol_f, coarse, csol_c, sm; enclosed)
+        (src/energy.jl:19-43) on the csol grids the two meshes hold now."""
+        if fine.n != self.n:
+            raise ValueError(f"accelerator sized for n={self.n}")
+        e = C.c_double()
+        _lib.call("vh_interaction_energy", self.handle, fine.handle, coarse.handle, float(weight),
+                  float(enclosed), C.byref(e))
+        return float(e.value)
+
     def enclosed_count(self, center, radius: float) -> int:
         c = np.ascontiguousarray(np.asarray(center, dtype=np.float64))
         cnt = C.c_int64()

@@ -50,6 +50,33 @@ class Jellium:
         return uniform_sphere_potential(self.nions, self.radius, r)
 
 
+def ion_self_energy(jel: Jellium) -> float:
+    """3 N^2 / 5 r0, the uniformly charged ball (src/energy.jl:51)."""
+    return 3 * jel.nions ** 2 / (5 * jel.radius)
+
+
+@dataclass
+class EnergyBudget:
+    """src/energy.jl:69-80: total = ions + kinetic + hartree + meanfield;
+    `escaped` is the kinetic share beyond rcmax and enters no sum."""
+    total: float
+    kinetic: float
+    hartree: float
+    meanfield: float
+    ions: float
+    escaped: float = 0.0
+
+
+def energy_budget(jel: Jellium, kinetic: float, hartree: float, total_interaction: float,
+                  escaped: float = 0.0) -> EnergyBudget:
+    """energy_budget (src/energy.jl:114-120): the mean-field term by difference,
+    total interaction - 2 * hartree."""
+    meanfield = total_interaction - 2 * hartree
+    ions = ion_self_energy(jel)
+    return EnergyBudget(ions + kinetic + hartree + meanfield, kinetic, hartree, meanfield, ions,
+                        escaped)
+
+
 def uniform_sphere_potential(Q, R, r):
     """-Q (3 - (r/R)^2) / 2R inside, -Q/r outside (src/meanfield.jl:235-237)."""
     return -Q * (3 - (r / R) ** 2) / (2 * R) if r < R else -Q / r

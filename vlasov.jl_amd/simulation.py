@@ -20,8 +20,8 @@ import numpy as np
 
 from . import grids
 from .device import Context, DeviceAccelerator, DeviceMesh, poisson_nested
-from .physics import (ELECTRON_MASS, Jellium, Projectile, jellium_force, packed_records,
-                      record_positions, sample_thomas_fermi, split)
+from .physics import (ELECTRON_MASS, Jellium, Projectile, energy_budget, jellium_force,
+                      packed_records, record_positions, sample_thomas_fermi, split)
 
 
 @dataclass
@@ -120,19 +120,24 @@ class Simulation:
         self.acc.upload(rec)
 
     # --- the step ------------------------------------------------------------------
-    def update_forces(self, advance: bool = True):
-        """_update_forces_resident! (src/simulation.jl:472-531), energy = false."""
+    def update_forces(self, advance: bool = True, energy: bool = False):
+        """_update_forces_resident! (src/simulation.jl:472-531).  With `energy`,
+        returns the Hartree energy (half the interaction energy on the
+        coefficients of the Hartree potential alone, before the mean field is
+        added to them); otherwise None."""
         w = self.weight
         self.nout = self.acc.deposit_smoothed(self.dmf, w)
         self.acc.deposit_cic(self.dmc, w)
         poisson_nested(self.dmf, self.dmc)        # leaves csol_c = S^-1 phi_c
         self.dmf.spline_coefficients()
+        hartree = self.acc.interaction_energy(self.dmf, self.dmc, w) / 2 if energy else None
         self.dmf.effective_potential(self.jellium)
         self.dmc.effective_potential(self.jellium)
         self.outcount, self.red = self.acc.compute_forces(self.dmf, self.dmc, w, self.projectile,
                                                           self.softening_mode)
         if advance and self.projectile is not None:
             self.advance_projectile()
+        return hartree
 
     def projectile_forces(self):
         """projectile_forces!(cloud, acc, proj, jel) (src/accelerator.jl:746-756)."""
@@ -145,18 +150,34 @@ class Simulation:
         force, _, _ = self.projectile_forces()
         self.projectile.step(force, self.params.dt)
 
-    def step(self):
-        """step!(sim; energy = false) (src/simulation.jl:382-396)."""
-        self.update_forces(advance=True)
+    def step(self, energy: bool = False):
+        """step!(sim; energy) (src/simulation.jl:382-396).  With `energy`,
+        returns the EnergyBudget: the Hartree energy measured before the
+        push, the kinetic energy of the push, and the total interaction on the
+        step's total potential at the pushed positions (the host path's
+        order, src/simulation.jl:309-351).  Without, returns None."""
+        hartree = self.update_forces(advance=True, energy=energy)
         self.diagnostics = self.acc.verlet(self.params.dt, self.mass, self.params.rcmax)
         self.step_count += 1
-        return self.diagnostics
+        if not energy:
+            return None
+        total = self.acc.interaction_energy(self.dmf, self.dmc, self.weight)
+        return energy_budget(self.jellium, self.diagnostics["kinetic"], hartree, total,
+                             self.diagnostics["escaped"])
 
-    def run(self, nsteps: int, callback=None):
+    def run(self, nsteps: int, energy_every: int = 0, callback=None):
+        """run! (src/simulation.jl:429-441): the budget history of the steps
+        where it is taken (every `energy_every` steps, none when 0)."""
+        if energy_every < 0:
+            raise ValueError("`energy_every` must be at least 0")
+        history = []
         for i in range(nsteps):
-            d = self.step()
+            b = self.step(energy=energy_every > 0 and i % energy_every == 0)
+            if b is not None:
+                history.append(b)
             if callback is not None:
-                callback(i + 1, d)
+                callback(i + 1, b)
+        return history
 
     # --- observables -------------------------------------------------------------
     def enclosed_charge(self, radius: float = 8.0) -> float:
