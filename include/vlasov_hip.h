@@ -124,6 +124,16 @@ int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const double *knots,
                     const double *gradient, const double *nodes, vh_accel **out);
 int vh_accel_destroy(vh_accel *a);
 
+/* Options of an accelerator (no counterpart in the reference, whose device
+ * kernel has one contraction order).
+ *   VH_OPT_EXACT_FIELD  0 (default): the 10^3 field contraction runs its
+ *                       first stage on the matrix cores, summed in another
+ *                       order than contract_spline_10 (FP32-rounding
+ *                       agreement); 1: the reference's exact FMA order
+ *                       (src/fields.jl:305-326), bit-identical to it. */
+enum { VH_OPT_EXACT_FIELD = 0 };
+int vh_accel_set_option(vh_accel *a, int option, int value);
+
 /* upload!/download! of the particle buffer (src/accelerator.jl:48, 64-68):
  * npart PackedParticle{Float32} records, in the device's current (sorted)
  * order. */

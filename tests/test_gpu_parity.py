@@ -2,9 +2,10 @@
 (nfine = 28: 58^3 fine and coarse grids) on a seeded Thomas-Fermi cloud.
 
 Bars: bit-exact for the sort (records, occupied cells, bounds, out-of-stencil
-list), the table columns, the Float32 Verlet and the Float32 contraction of
-the field; 1e-5 relative (max |diff| / max |ref|) for FP32 densities,
-potentials and forces against the FP64 oracle."""
+list), the table columns, the Float32 Verlet and the exact-order Float32
+contraction of the field; 1e-5 relative (max |diff| / max |ref|) for FP32
+densities, potentials and forces (the default matrix-core contraction
+included) against the FP64 oracle; 1e-12 for the FP64 interaction energy."""
 import numpy as np
 import pytest
 
@@ -179,10 +180,15 @@ def _forces_setup(s, orc, seed=5):
 
 
 def test_field_contraction_bit_exact(setup, orc):
+    """The exact-order variant: bit-identical to contract_spline_10 in FP32."""
     s = setup
     srec, occ, bnd, csol_f, csol_c = _forces_setup(s, orc)
     sm, ost = s["sm"], s["ost"]
-    nout, red = s["acc"].compute_forces(s["dmf"], s["dmc"], s["w"], None)
+    s["acc"].set_exact_field(True)
+    try:
+        nout, red = s["acc"].compute_forces(s["dmf"], s["dmc"], s["w"], None)
+    finally:
+        s["acc"].set_exact_field(False)
     f = s["acc"].forces()
     want, _ = orc.field(srec, occ, bnd, ost.nk, s["fine"].n, csol_f.astype(np.float64),
                         sm.overlap, sm.gradient, ost.x0, ost.h, ost.spacing, sm.nbdt, sm.nbdt + 1,
@@ -196,6 +202,38 @@ def test_field_contraction_bit_exact(setup, orc):
     wfb = orc.fallback(srec, outl, ost.x0, ost.h, s["coarse"].axis.knots, csol_c.astype(np.float64),
                        s["w"], None)
     assert relerr(f[outl], wfb[outl]) < REL
+
+
+def test_field_matrix_cores(setup, orc):
+    """The default variant (first stage on MFMA): within FP32 rounding of the
+    FP64 contraction, identical to the bit from one call to the next, and
+    within a few FP32 ulps of the exact-order variant."""
+    s = setup
+    srec, occ, bnd, _, csol_c = _forces_setup(s, orc, seed=8)
+    sm, ost = s["sm"], s["ost"]
+    acc = s["acc"]
+    # a zero-mean rough potential: no constant part whose cancelling
+    # contributions would make the rounding of any summation order look large
+    rng = np.random.default_rng(18)
+    csol_f = (rng.standard_normal(s["fine"].n ** 3) * 0.01).astype(np.float32)
+    s["dmf"].upload(va._lib.VH_GRID_CSOL, csol_f)
+    acc.compute_forces(s["dmf"], s["dmc"], s["w"], None)
+    f = acc.forces()
+    want, _ = orc.field(srec, occ, bnd, ost.nk, s["fine"].n, csol_f.astype(np.float64),
+                        sm.overlap, sm.gradient, ost.x0, ost.h, ost.spacing, sm.nbdt, sm.nbdt + 1,
+                        s["w"], None, 0, 0.0, f32=False)
+    inside = ~np.isnan(want[:, 0])
+    assert np.all(np.isfinite(f))
+    assert relerr(f[inside], want[inside]) < REL
+    acc.compute_forces(s["dmf"], s["dmc"], s["w"], None)
+    assert acc.forces().tobytes() == f.tobytes()
+    acc.set_exact_field(True)
+    try:
+        acc.compute_forces(s["dmf"], s["dmc"], s["w"], None)
+        fe = acc.forces()
+    finally:
+        acc.set_exact_field(False)
+    assert relerr(f[inside], fe[inside]) < 1e-6
 
 
 @pytest.mark.parametrize("mode", ["device", "ball"])
@@ -241,6 +279,10 @@ def test_bad_arguments_refused_before_launch(setup):
                      None, None)
     with pytest.raises(ValueError):
         s["acc"].upload(np.zeros(3, dtype=va.PACKED_DTYPE))
+    with pytest.raises(va.VlasovHIPError):
+        va._lib.call("vh_accel_set_option", s["acc"].handle, 99, 1)
+    with pytest.raises(va.VlasovHIPError):
+        va._lib.call("vh_accel_set_option", s["acc"].handle, va._lib.VH_OPT_EXACT_FIELD, 2)
 
 
 def test_interaction_energy(setup, orc):

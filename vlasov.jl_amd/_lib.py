@@ -14,6 +14,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libvlasov_hip.so"
 
 VH_GRID_RHO, VH_GRID_PHI, VH_GRID_CSOL = 0, 1, 2
 VH_SOFT_DEVICE, VH_SOFT_BALL = 0, 1
+VH_OPT_EXACT_FIELD = 0
 STAGES = ("sort", "deposit_fine", "deposit_cic", "poisson", "spline", "meanfield", "field",
           "verlet", "field_kernel", "deposit_kernel")
 
@@ -47,6 +48,7 @@ _SIGNATURES = {
     "vh_accel_create": [_vp, C.c_int64, C.c_int, _f64p, C.c_double, C.c_int, _f64p, _f64p, _f64p,
                         C.POINTER(_vp)],
     "vh_accel_destroy": [_vp],
+    "vh_accel_set_option": [_vp, C.c_int, C.c_int],
     "vh_particles_upload": [_vp, _vp],
     "vh_particles_download": [_vp, _vp],
     "vh_forces_download": [_vp, _f32p],

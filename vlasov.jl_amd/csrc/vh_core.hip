@@ -375,6 +375,19 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
   return VH_OK;
 }
 
+extern "C" int vh_accel_set_option(vh_accel *a, int option, int value) {
+  VH_REQUIRE(a != nullptr, "vh_accel_set_option: null accelerator");
+  switch (option) {
+    case VH_OPT_EXACT_FIELD:
+      VH_REQUIRE(value == 0 || value == 1, "vh_accel_set_option: exact field takes 0 or 1");
+      a->exact_field = value;
+      return VH_OK;
+    default:
+      VH_REQUIRE(false, "vh_accel_set_option: unknown option %d", option);
+  }
+  return VH_EINVAL;
+}
+
 // AoS record <-> SoA streams
 __global__ void k_aos_to_soa(const int32_t *__restrict__ rec, int32_t *__restrict__ soa, int64_t n,
                              int64_t npad) {

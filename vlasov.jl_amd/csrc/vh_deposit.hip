@@ -181,7 +181,9 @@ __global__ void k_dep_scale(float *__restrict__ rho, int64_t len, const double *
 int vh_deposit_fine_impl(vh_accel *a, vh_mesh *fine, double charge) {
   vh_context *c = a->ctx;
   hipStream_t st = c->stream;
-  const int grid = (int)(a->maxocc < 16384 ? a->maxocc : 16384);
+  // one wave per fitting cell up to 65536 cells: the dispatcher balances
+  // cells of very different weights
+  const int grid = (int)(a->maxocc < 65536 ? a->maxocc : 65536);
   int tok = vh_prof_begin(c, VH_ST_DEPOSIT_KERNEL);
   hipLaunchKernelGGL(k_dep_tiles, dim3(grid), dim3(64), 0, st, a->parts[a->cur], a->npad, a->nodes,
                      a->fitlist, a->bounds, a->counters, a->spacing, (float)a->nbdt, a->ncol,

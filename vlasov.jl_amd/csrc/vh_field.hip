@@ -1,13 +1,16 @@
 // Smoothed field at the pseudo-particles with the projectile fused in, the
 // out-of-stencil fallback, and the packed position Verlet.
 //
-// Field (src/kernels.jl:339-470).  One wave per occupied cell: the cell's
-// 10^3 block of csol is staged once in LDS (read by every lane at the same
-// address, i.e. broadcast), each lane takes one particle at a time, keeps
-// the 60 tabulated overlap/gradient values of its particle in registers and
-// runs contract_spline_10 (src/fields.jl:305-326) in the reference's fma
-// order.  The projectile's per-particle reaction and pair potential are
-// summed per lane in FP64 and reduced in a fixed order.
+// Field (src/kernels.jl:339-470).  One wave per occupied cell, two variants:
+//   k_field_mfma (default): the first stage of the 10^3 contraction on the
+//     matrix cores, 16 particles per product (see the kernel);
+//   k_field (exact order): each lane takes one particle, keeps 40 of its
+//     tabulated overlap/gradient values in registers and runs
+//     contract_spline_10 (src/fields.jl:305-326) in the reference's fma order
+//     on the cell's 10^3 block of csol, read at wave-uniform addresses (scalar
+//     loads feeding the FMAs from SGPRs).
+// The projectile's per-particle reaction and pair potential are summed per
+// lane in FP64 and reduced in a fixed order.
 //
 // Fallback (src/accelerator.jl:679-731).  The reference downloads the whole
 // force buffer and loops on the host over the particles whose stencil leaves
@@ -188,6 +191,245 @@ __global__ void __launch_bounds__(64)
 }
 
 // ---------------------------------------------------------------------------
+// field kernel on the matrix cores
+// ---------------------------------------------------------------------------
+// The contraction F = sum c[ii,jj,kk] X[ii] Y[jj] Z[kk] in three stages:
+//   stage 1 (MFMA): T_o[p, r] = sum_ii c[ii, r] ox[p, ii] and T_g[p, r] with
+//     gx, for the 100 rows r = (jj, kk) of the cell's block -- a product of
+//     the block (rows x ii, the same for every particle of the cell) by the
+//     particles' x columns (ii x particles), 16 particles at a time;
+//   stage 2 (VALU): per particle, the y contraction of each kk plane;
+//   stage 3 (VALU): the z contraction, then the sum over the four lanes that
+//     share a particle.
+// v_mfma_f32_16x16x4_f32: lane l supplies A[l%16][l/16] and B[l/16][l%16] and
+// receives D[4(l/16)+v][l%16], v = 0..3.  Rows are laid out so that the four
+// D registers and 8 tiles of a lane hold, for its particle, the rows
+// q = 4t + v (q < 30) as (jj = q % 10, kk = l/16 + 4 (q / 10)): jj is known at
+// compile time, kk depends on the lane group only.  K = ii runs over 3 steps
+// of 4 (10 of 12 used), rows over 8 tiles of 16 (100 of 128 used).
+// The sums run in another order than the reference's FMA chain, so the result
+// agrees to FP32 rounding, not to the bit; k_field keeps the exact order.
+//
+// Per chunk of 64 particles: phase A (a lane per particle) stages the six
+// table columns of each particle in LDS, rows of FT_STRIDE floats (odd, so
+// that consecutive particles fall in different banks): ox, gx, oy, gy, oz, gz
+// at offsets 0, 10, 20, 30, 40, 50; phase B runs the four sub-groups of 16
+// particles; phase C stores.
+#define FT_STRIDE 61
+
+// Operands of one sub-group for the lane group `grp`, from its particle's LDS
+// row: b = {ox, ox, ox, gx, gx, gx} at ii = 4k + grp (0 beyond 9), y = oy then
+// gy (all ten), z = {oz, oz, oz, gz, gz, gz} at kk = grp + 4m (0 beyond 9).
+// Every read is unconditional (index 8 + grp exists for grp < 2 only, the
+// other lanes read an in-bounds stand-in and select 0): conditional reads
+// would become exec-masked branches, each waiting for all loads in flight.
+__device__ __forceinline__ void ft_operands(const float *row, int grp, float b[6], float y[20],
+                                            float z[6]) {
+  const bool top = grp < 2;
+  const int gt = top ? grp : 0;
+  const float b2 = row[8 + gt], b5 = row[18 + gt], z2 = row[48 + gt], z5 = row[58 + gt];
+  b[0] = row[grp];
+  b[1] = row[4 + grp];
+  b[2] = top ? b2 : 0.f;
+  b[3] = row[10 + grp];
+  b[4] = row[14 + grp];
+  b[5] = top ? b5 : 0.f;
+#pragma unroll
+  for (int a = 0; a < 20; ++a) y[a] = row[20 + a];
+  z[0] = row[40 + grp];
+  z[1] = row[44 + grp];
+  z[2] = top ? z2 : 0.f;
+  z[3] = row[50 + grp];
+  z[4] = row[54 + grp];
+  z[5] = top ? z5 : 0.f;
+}
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+    k_field_mfma(const int32_t *__restrict__ P, int64_t npad, const float *__restrict__ csol, int n,
+                 const float *__restrict__ ovl, const float *__restrict__ grad,
+                 const int32_t *__restrict__ occupied, const int32_t *__restrict__ bounds,
+                 const int32_t *__restrict__ cnt, int nk, float x0, float h, float spacing,
+                 float nbdt, float w, int ncol, vh_proj_args pa, float *__restrict__ force,
+                 double *__restrict__ part) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  __shared__ float fld[3][64];
+  __shared__ float tab[64 * FT_STRIDE];
+  const float *D = (const float *)P;
+  const int l = threadIdx.x;
+  const int j16 = l & 15, grp = l >> 4;
+  const float half = spacing * 0.5f;
+  const float nan = __int_as_float(0x7fc00000);
+  const int nocc = cnt[CNT_NOCC];
+  const int64_t nn = n;
+  double sax = 0.0, say = 0.0, saz = 0.0, sae = 0.0;
+  for (int g = blockIdx.x; g < nocc; g += gridDim.x) {
+    const int key = occupied[g];
+    const int kx = key % nk + 1, ky = (key / nk) % nk + 1, kz = key / (nk * nk) + 1;
+    const int bx = 2 * kx - 5, by = 2 * ky - 5, bz = 2 * kz - 5;
+    const bool ok = bx >= 1 && by >= 1 && bz >= 1 && bx + 9 <= n && by + 9 <= n && bz + 9 <= n;
+    const int lo = bounds[g], hi = bounds[g + 1];
+    // A operands of the cell: row j16 of tile t, k = ii = 4 s + grp
+    float A[8][3];
+    if (ok) {
+      const float *tb = csol + (bx - 1) + nn * ((by - 1) + nn * (bz - 1));
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int q = 4 * t + (j16 & 3), m = q / 10, jj = q % 10, kk = (j16 >> 2) + 4 * m;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          const int ii = 4 * s + grp;
+          const bool in = q < 30 && kk < 10 && ii < 10;
+          const float v = tb[in ? ii + nn * (jj + nn * kk) : 0];  // unconditional read
+          A[t][s] = in ? v : 0.f;
+        }
+      }
+    }
+    for (int base = lo; base < hi; base += 64) {
+      const int nlive = min(64, hi - base);
+      // phase A, one lane per particle: offsets, the six table columns of the
+      // particle staged in LDS, and the projectile term (independent of the
+      // contraction, so its latency overlaps the table loads)
+      const int s = base + l;
+      const bool mine = l < nlive;
+      const int sr = mine ? s : lo;
+      const float dx0 = D[PS_DX * npad + sr], dy0 = D[PS_DY * npad + sr], dz0 = D[PS_DZ * npad + sr];
+      if (ok) {
+        int cx = (int)floorf((dx0 + half) / spacing * nbdt + 0.5f) + 1;
+        int cy = (int)floorf((dy0 + half) / spacing * nbdt + 0.5f) + 1;
+        int cz = (int)floorf((dz0 + half) / spacing * nbdt + 0.5f) + 1;
+        cx = min(max(cx, 1), ncol) - 1;
+        cy = min(max(cy, 1), ncol) - 1;
+        cz = min(max(cz, 1), ncol) - 1;
+        // 10-float columns start at even offsets: 8-byte loads
+        const float2 *src[6] = {(const float2 *)(ovl + cx * 10), (const float2 *)(grad + cx * 10),
+                                (const float2 *)(ovl + cy * 10), (const float2 *)(grad + cy * 10),
+                                (const float2 *)(ovl + cz * 10), (const float2 *)(grad + cz * 10)};
+        float *row = tab + l * FT_STRIDE;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+#pragma unroll
+          for (int a = 0; a < 5; ++a) {
+            const float2 v = src[c][a];
+            row[10 * c + 2 * a] = v.x;
+            row[10 * c + 2 * a + 1] = v.y;
+          }
+        }
+      }
+      float fx2 = 0.f, fy2 = 0.f, fz2 = 0.f;
+      if (mine && pa.coef != 0.f) {
+        float px, py, pz;
+        if (pa.ball) {  // the particle's own position
+          px = x0 + (float)(P[PS_KX * npad + s] - 1) * h + dx0;
+          py = x0 + (float)(P[PS_KY * npad + s] - 1) * h + dy0;
+          pz = x0 + (float)(P[PS_KZ * npad + s] - 1) * h + dz0;
+        } else {        // rebuilt from the cell, as the reference's kernel does
+          px = x0 + (float)(kx - 1) * h + dx0;
+          py = x0 + (float)(ky - 1) * h + dy0;
+          pz = x0 + (float)(kz - 1) * h + dz0;
+        }
+        const float dx = pa.px - px, dy = pa.py - py, dz = pa.pz - pz;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        const float mm = pa.coef * (pa.ball ? ball_force_kernel_f(d2, pa.radius)
+                                            : gaussian_force_kernel_f(d2, pa.sigma));
+        fx2 = mm * dx;
+        fy2 = mm * dy;
+        fz2 = mm * dz;
+        const float r = sqrtf(d2);
+        sax += fx2;
+        say += fy2;
+        saz += fz2;
+        if (pa.ball) {
+          const float R = pa.radius;
+          sae += r < R ? -(3.0f - (r / R) * (r / R)) / (2.0f * R) : -1.0f / r;
+        } else {
+          sae += r < 1.0e-4f * pa.sigma ? 0.7978845608f / pa.sigma
+                                        : erff(r / 1.4142135624f / pa.sigma) / r;
+        }
+      }
+      __syncthreads();
+      // phase B: the contraction, 16 particles per sub-group; each sub-group's
+      // operands are read from LDS one sub-group ahead
+      if (ok) {
+        const int nsg = (nlive + 15) >> 4;
+#pragma unroll 1
+        for (int sg = 0; sg < nsg; ++sg) {
+          float b[6], y[20], z[6];
+          ft_operands(tab + (16 * sg + j16) * FT_STRIDE, grp, b, y, z);
+          // stage 1
+          f4 To[8], Tg[8];
+#pragma unroll
+          for (int t = 0; t < 8; ++t) {
+            To[t] = f4{0.f, 0.f, 0.f, 0.f};
+            Tg[t] = f4{0.f, 0.f, 0.f, 0.f};
+          }
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+              To[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[t][k], b[k], To[t], 0, 0, 0);
+              Tg[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[t][k], b[3 + k], Tg[t], 0, 0, 0);
+            }
+          }
+          // stage 2: y contraction per kk plane (m = 0..2 of this lane group);
+          // y[0..9] = oy, y[10..19] = gy
+          float ux[3] = {0.f, 0.f, 0.f}, uy[3] = {0.f, 0.f, 0.f}, uz[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+          for (int t = 0; t < 8; ++t) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int q = 4 * t + v;
+              if (q < 30) {
+                const int m = q / 10, jj = q % 10;
+                ux[m] = __builtin_fmaf(y[jj], Tg[t][v], ux[m]);
+                uy[m] = __builtin_fmaf(y[10 + jj], To[t][v], uy[m]);
+                uz[m] = __builtin_fmaf(y[jj], To[t][v], uz[m]);
+              }
+            }
+          }
+          // stage 3: z contraction of this lane group's planes kk = grp + 4m
+          // (z[m] = oz[kk], z[3 + m] = gz[kk], zero where kk > 9)
+          float fx = 0.f, fy = 0.f, fz = 0.f;
+#pragma unroll
+          for (int m = 0; m < 3; ++m) {
+            fx = __builtin_fmaf(z[m], ux[m], fx);
+            fy = __builtin_fmaf(z[m], uy[m], fy);
+            fz = __builtin_fmaf(z[3 + m], uz[m], fz);
+          }
+          fx += __shfl_xor(fx, 16, 64);
+          fy += __shfl_xor(fy, 16, 64);
+          fz += __shfl_xor(fz, 16, 64);
+          fx += __shfl_xor(fx, 32, 64);
+          fy += __shfl_xor(fy, 32, 64);
+          fz += __shfl_xor(fz, 32, 64);
+          if (grp == 0) {
+            fld[0][16 * sg + j16] = -w * fx;
+            fld[1][16 * sg + j16] = -w * fy;
+            fld[2][16 * sg + j16] = -w * fz;
+          }
+        }
+      }
+      __syncthreads();
+      // phase C: field minus projectile term, coalesced stores
+      if (mine) {
+        force[s] = ok ? fld[0][l] - fx2 : nan;
+        force[npad + s] = ok ? fld[1][l] - fy2 : nan;
+        force[2 * npad + s] = ok ? fld[2][l] - fz2 : nan;
+      }
+    }
+  }
+  sax = vh_wave_sum(sax);
+  say = vh_wave_sum(say);
+  saz = vh_wave_sum(saz);
+  sae = vh_wave_sum(sae);
+  if (l == 0) {
+    part[blockIdx.x] = sax;
+    part[gridDim.x + blockIdx.x] = say;
+    part[2 * gridDim.x + blockIdx.x] = saz;
+    part[3 * gridDim.x + blockIdx.x] = sae;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // out-of-stencil fallback, FP64
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void hermite_vd(const double *__restrict__ g, int nk, int b, double x,
@@ -306,10 +548,12 @@ __global__ void k_fallback(const int32_t *__restrict__ P, int64_t npad, const in
   }
 }
 
+// Sums each of `nvals` columns of `nparts` partials in a fixed order; one
+// block per column (any grid works, columns are strided over the blocks).
 __global__ void k_reduce_cols(const double *__restrict__ part, int nparts, int nvals,
                               double *__restrict__ out) {
   __shared__ double lds[16];
-  for (int v = 0; v < nvals; ++v) {
+  for (int v = blockIdx.x; v < nvals; v += gridDim.x) {
     double s = 0.0;
     for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[(int64_t)v * nparts + i];
     const double r = vh_block_sum_d(s, lds);
@@ -466,16 +710,19 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
     pa.sigma = (float)proj[4];
     pa.radius = (float)(ball_radius > 0 ? ball_radius : proj[4]);
   }
-  int grid = (int)(a->maxocc < 16384 ? a->maxocc : 16384);
-  if (grid > a->dpart_len / 4) grid = (int)(a->dpart_len / 4);
+  // One block per occupied cell up to 65536 cells (the dispatcher then
+  // balances cells of very different weights); beyond, blocks stride over
+  // cells.  Four FP64 partials per block fit the 8 x dpart_len buffer.
+  int grid = (int)(a->maxocc < 65536 ? a->maxocc : 65536);
+  if (4 * (int64_t)grid > 8 * a->dpart_len) grid = (int)(2 * a->dpart_len);
   int tok = vh_prof_begin(c, VH_ST_FIELD_KERNEL);
-  hipLaunchKernelGGL(k_field, dim3(grid), dim3(64), 0, st, a->parts[a->cur], a->npad, fine->csol,
-                     a->n, a->ovl, a->grad, a->occupied, a->bounds, a->counters, a->nk,
-                     (float)a->x0, (float)a->h, a->spacing, (float)a->nbdt, (float)weight, a->ncol,
-                     pa, a->force, a->dpart);
+  hipLaunchKernelGGL(a->exact_field ? k_field : k_field_mfma, dim3(grid), dim3(64), 0, st,
+                     a->parts[a->cur], a->npad, fine->csol, a->n, a->ovl, a->grad, a->occupied,
+                     a->bounds, a->counters, a->nk, (float)a->x0, (float)a->h, a->spacing,
+                     (float)a->nbdt, (float)weight, a->ncol, pa, a->force, a->dpart);
   vh_prof_end(c, tok);
   VH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, st, a->dpart, grid, 4, a->dscal + 8);
+  hipLaunchKernelGGL(k_reduce_cols, dim3(4), dim3(1024), 0, st, a->dpart, grid, 4, a->dscal + 8);
   VH_LAUNCH_CHECK();
   vh_fallback_args fa;
   fa.px = proj ? proj[0] : 0.0;
@@ -567,7 +814,7 @@ int vh_verlet_impl(vh_accel *a, double dt, double mass, double rcmax) {
   hipLaunchKernelGGL(k_verlet, dim3(VERLET_BLOCKS), dim3(256), 0, c->stream, a->parts[a->cur],
                      a->npad, a->npart, a->force, fa, fh, fx0, pfac, r2max, inv2M, a->dpart);
   VH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, c->stream, a->dpart, VERLET_BLOCKS, 5,
+  hipLaunchKernelGGL(k_reduce_cols, dim3(5), dim3(256), 0, c->stream, a->dpart, VERLET_BLOCKS, 5,
                      a->dscal);
   VH_LAUNCH_CHECK();
   return VH_OK;

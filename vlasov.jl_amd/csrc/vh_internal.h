@@ -149,6 +149,7 @@ struct vh_accel {
   int64_t maxocc;              // min(ncell, npart)
   uint32_t *cellscratch;       // 3 x (maxocc + 1) scan scratch of the cell classification
   int sorted_once;             // host flag: the lists of a sort exist
+  int exact_field;             // VH_OPT_EXACT_FIELD: the reference's FMA order
 };
 
 // ---------------------------------------------------------------------------

@@ -157,6 +157,12 @@ class DeviceAccelerator:
         self.nk = len(knots)
         self.n = 2 * len(knots)
 
+    def set_exact_field(self, exact: bool):
+        """True: the field contraction in the reference's exact FMA order
+        (bit-identical to contract_spline_10 in FP32); False (default): its
+        first stage on the matrix cores."""
+        _lib.call("vh_accel_set_option", self.handle, _lib.VH_OPT_EXACT_FIELD, 1 if exact else 0)
+
     # --- buffers -------------------------------------------------------------
     def upload(self, records: np.ndarray):
         r = np.ascontiguousarray(records)
