@@ -27,8 +27,11 @@ HEADERS = sorted(list(CSRC.glob("*.h")) + [HERE.parent / "include" / "vlasov_hip
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
           "-Wno-unused-variable", "-Wno-unused-but-set-variable"]
+# -amdgpu-mfma-vgpr-form: matrix-core results go to ordinary VGPRs, which the
+# epilogues read directly, instead of accumulation registers copied out one
+# v_accvgpr_read at a time.
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-fno-gpu-rdc"]
+          "-mllvm", "-amdgpu-mfma-vgpr-form", "-fno-gpu-rdc"]
 
 
 def hipcc() -> str:

@@ -139,8 +139,8 @@ extern "C" int vh_mesh_destroy(vh_mesh *m) {
   if (!m) return VH_OK;
   hipSetDevice(m->ctx->device);
   hipStreamSynchronize(m->ctx->stream);
-  void *ptrs[] = {m->colloc, m->knots, m->colloc64, m->knots64, m->lap, m->Minv_rm, m->M_rm,
-                  m->Sinv_rm, m->invlsum, m->dual, m->dual32, m->duallen, m->loc_cells,
+  void *ptrs[] = {m->colloc, m->knots, m->colloc64, m->knots64, m->lap, m->Minv_op, m->M_op,
+                  m->Sinv_op, m->invlsum, m->dual, m->dual32, m->duallen, m->loc_cells,
                   m->rho, m->phi, m->csol, m->extra, m->work, m->inner, m->work1, m->work2,
                   m->partials, m->scal, m->acc64};
   for (void *p : ptrs) vh_free(p);
@@ -181,9 +181,9 @@ extern "C" int vh_mesh_create(vh_context *ctx, int n, int nk, const double *coll
   MK(upload_converted(ctx, &m->colloc64, colloc, n));
   MK(upload_converted(ctx, &m->knots64, knots, nk));
   MK(upload_rowmajor(ctx, &m->lap, lap, n));
-  MK(upload_rowmajor(ctx, &m->Minv_rm, Minv, ni));
-  MK(upload_rowmajor(ctx, &m->M_rm, M, ni));
-  MK(upload_rowmajor(ctx, &m->Sinv_rm, Sinv, n));
+  MK(vh_gemm_pack(ctx, &m->Minv_op, Minv, ni));
+  MK(vh_gemm_pack(ctx, &m->M_op, M, ni));
+  MK(vh_gemm_pack(ctx, &m->Sinv_op, Sinv, n));
   MK(upload_converted(ctx, &m->dual, dual, 3 * (size_t)n));
   MK(upload_converted(ctx, &m->dual32, dual, 3 * (size_t)n));
   MK(upload_converted(ctx, &m->duallen, duallen, n));

@@ -75,9 +75,9 @@ struct vh_mesh {
   float *colloc, *knots;
   double *colloc64, *knots64;
   float *lap;        // full D, row-major [r*n + c]
-  float *Minv_rm;    // ni x ni row-major, A operand of the rotating product
-  float *M_rm;
-  float *Sinv_rm;    // n x n row-major
+  float *Minv_op;    // ni x ni, A operand of the rotating product (vh_gemm_pack order)
+  float *M_op;
+  float *Sinv_op;    // n x n, same order
   float *invlsum;    // ni^3, i fastest
   double *dual;      // 3 x n (order k at k*n), FP64
   float *dual32;     // same, narrowed
@@ -193,9 +193,11 @@ int vh_fill_f32(vh_context *c, float *p, int64_t n, float v);
 int vh_scan_exclusive(vh_context *c, uint32_t *data, int64_t n, uint32_t *tmp, int64_t tmp_len,
                       uint32_t *total_out);
 
-// rotating product: out[i' m + j] = sum_i A[i' n + i] src[j n + i], m = len/n
+// rotating product: out[i' m + j] = sum_i A[i', i] src[j n + i], m = len/n, with
+// A given in MFMA operand order by vh_gemm_pack (device copy allocated there)
 enum { EPI_STORE = 0, EPI_HADAMARD = 1, EPI_FILL_INTERIOR = 2, EPI_ACCUMULATE = 3 };
-int vh_rotate_gemm(vh_context *c, const float *A, const float *src, float *out, int n, int64_t m,
+int vh_gemm_pack(vh_context *c, float **dst, const double *colmajor, int n);
+int vh_rotate_gemm(vh_context *c, const float *Aop, const float *src, float *out, int n, int64_t m,
                    int epi, const float *aux, int nfull);
 
 int vh_sort_impl(vh_accel *a);

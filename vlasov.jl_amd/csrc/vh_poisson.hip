@@ -6,7 +6,7 @@
 // by A along its first dimension and written transposed, which rotates the
 // dimensions; three in a row apply A along all three axes.  Here it is one
 // MFMA kernel (v_mfma_f32_16x16x4_f32, FP32 in and out) whose block owns all
-// n output rows of 64 output columns, so every source column is read once;
+// n output rows of 32 output columns, so every source column is read once;
 // the Hadamard product by 1/(l_i+l_j+l_k) (src/tensorsolver.jl:291), the copy
 // into the interior of phi (src/kernels.jl:804-813) and the `csol .+= ...` of
 // the mean field (src/devicemesh.jl:182) are fused into its epilogue.
@@ -16,89 +16,154 @@
 
 typedef float vh_f4 __attribute__((ext_vector_type(4)));
 
-// One block owns 32 output columns j (2 MFMA column tiles) and all n output
-// rows: the 32 x n source panel is staged in LDS once (k contiguous, padded
-// stride), and the four waves split the MT row tiles; A (n x n, at most
-// 266 KB) is read through the L1 at wave-uniform rows.
-template <int MT>
-__global__ void __launch_bounds__(256)
+// The product C = A S (C n x m, row-major out[i' m + j]; S[k][j] = src[j n + k])
+// by v_mfma_f32_16x16x4_f32 (lane l: A[l%16][k], B[k][l%16], k from l/16).
+// A block owns 32 output columns (two column tiles) and W row tiles of 16,
+// one wave each (W = all MT row tiles up to 9, else half of them, so at most
+// 9 waves and 170 VGPRs each):
+//   * each wave keeps the A operands of its row tile for the whole K range in
+//     registers, with K permuted so that lane group g = l/16 owns the
+//     contiguous slice k = g KQ + s, s < KQ (KQ odd, >= n/4; k >= n padded
+//     with zero A).  A is stored once, at mesh creation, in that operand
+//     order (vh_gemm_pack): step s of row tile r is 64 consecutive floats,
+//     one per lane, so the wave's reads are coalesced;
+//   * the 32 source columns are contiguous in memory: the block reads them
+//     with coalesced 8-byte loads into LDS, column stride ldb = 2 (mod 32),
+//     so that the 32 lanes of a half-wave (16 columns x 2 lane groups, odd
+//     KQ) hit 32 different banks;
+//   * two independent accumulation chains (one per column tile).
+template <int KQ, int NCT>
+__global__ void __launch_bounds__(576)
     k_rot_gemm(const float *__restrict__ A, const float *__restrict__ src, float *__restrict__ out,
-               int n, int64_t m, int epi, const float *__restrict__ aux, int nfull) {
+               int n, int64_t m, int ldb, int epi, const float *__restrict__ aux, int nfull) {
   extern __shared__ float Bs[];
-  constexpr int RT = (MT + 3) / 4;
+  constexpr int NC = 16 * NCT;  // output columns of the block
+  const int nthr = blockDim.x;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int ldb = n + 1;
-  const int64_t j0 = (int64_t)blockIdx.x * 32;
-  const int64_t ncols = (m - j0) < 32 ? (m - j0) : 32;
-  const float *panel = src + j0 * n;
-  for (int e = threadIdx.x; e < 32 * n; e += 256) {
-    const int c = e / n, k = e - c * n;
-    Bs[c * ldb + k] = (c < ncols) ? panel[e] : 0.f;
-  }
-  __syncthreads();
-  vh_f4 acc[RT][2];
-#pragma unroll
-  for (int rr = 0; rr < RT; ++rr) acc[rr][0] = acc[rr][1] = (vh_f4){0.f, 0.f, 0.f, 0.f};
   const int lc = l & 15, lk = l >> 4;
-  for (int ks = 0; ks < n; ks += 4) {
-    const int k = ks + lk;
-    const bool kin = k < n;
-    const float b0 = kin ? Bs[lc * ldb + k] : 0.f;
-    const float b1 = kin ? Bs[(16 + lc) * ldb + k] : 0.f;
+  const int mt = (n + 15) >> 4;
+  const int r = blockIdx.y * (nthr >> 6) + w;
+  const int64_t j0 = (int64_t)blockIdx.x * NC;
+  // A operands of row tile r (waves past the last tile read tile mt - 1 and
+  // leave after the barrier)
+  float a[KQ];
+  const float *ar = A + ((int64_t)(r < mt ? r : mt - 1) * KQ) * 64 + l;
 #pragma unroll
-    for (int rr = 0; rr < RT; ++rr) {
-      const int r = w + 4 * rr;
-      if (r >= MT) break;
-      const int row = r * 16 + lc;
-      const float av = (row < n && kin) ? A[row * n + k] : 0.f;
-      acc[rr][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b0, acc[rr][0], 0, 0, 0);
-      acc[rr][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b1, acc[rr][1], 0, 0, 0);
+  for (int s = 0; s < KQ; ++s) a[s] = ar[64 * s];
+  // the NC x n panel (n even: pairs never straddle a column), eight pairs
+  // per thread per round, all eight loads issued before the LDS writes;
+  // out-of-range pairs read pair 0 and are replaced by zeros (the select
+  // tests the address's condition, so the load stays unconditional)
+  const float2 *src2 = (const float2 *)src;
+  const int64_t tot2 = m * n / 2, base2 = j0 * n / 2;
+  const int per2 = NC / 2 * n;
+  for (int e0 = 0; e0 < per2; e0 += 8 * nthr) {
+    float2 pv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + threadIdx.x + u * nthr;
+      const int64_t g = base2 + e;
+      const bool in = e < per2 && g < tot2;
+      const float2 v = src2[in ? g : 0];
+      pv[u] = in ? v : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + threadIdx.x + u * nthr;
+      if (e < per2) {
+        const int f = 2 * e, cc = f / n, k = f - cc * n;
+        *(float2 *)(Bs + cc * ldb + k) = pv[u];
+      }
     }
   }
+  __syncthreads();
+  if (r >= mt) return;
+  vh_f4 acc[NCT];
 #pragma unroll
-  for (int rr = 0; rr < RT; ++rr) {
-    const int r = w + 4 * rr;
-    if (r >= MT) break;
+  for (int ct = 0; ct < NCT; ++ct) acc[ct] = (vh_f4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int ct = 0; ct < 2; ++ct) {
-      const int64_t j = j0 + ct * 16 + lc;
-      if (j >= m) continue;
+  for (int s = 0; s < KQ; ++s) {
+    int k = lk * KQ + s;
+    k = k < n ? k : n - 1;  // A is zero there
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int row = r * 16 + 4 * lk + v;
-        if (row >= n) continue;
-        const int64_t idx = (int64_t)row * m + j;
-        const float val = acc[rr][ct][v];
-        if (epi == EPI_STORE) {
-          out[idx] = val;
-        } else if (epi == EPI_HADAMARD) {
-          out[idx] = val * aux[idx];
-        } else if (epi == EPI_ACCUMULATE) {
-          out[idx] = out[idx] + val;
-        } else {  // EPI_FILL_INTERIOR: idx is (i, j, k) of the interior, i fastest
-          const int64_t nn = n;
-          const int64_t i = idx % nn, jj = (idx / nn) % nn, kz = idx / (nn * nn);
-          out[(i + 1) + nfull * ((jj + 1) + (int64_t)nfull * (kz + 1))] = val;
-        }
+    for (int ct = 0; ct < NCT; ++ct)
+      acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], Bs[(16 * ct + lc) * ldb + k], acc[ct], 0, 0, 0);
+  }
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct) {
+    const int64_t j = j0 + ct * 16 + lc;
+    if (j >= m) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int orow = r * 16 + 4 * lk + v;
+      if (orow >= n) continue;
+      const int64_t idx = (int64_t)orow * m + j;
+      const float val = acc[ct][v];
+      if (epi == EPI_STORE) {
+        out[idx] = val;
+      } else if (epi == EPI_HADAMARD) {
+        out[idx] = val * aux[idx];
+      } else if (epi == EPI_ACCUMULATE) {
+        out[idx] = out[idx] + val;
+      } else {  // EPI_FILL_INTERIOR: idx is (i, j, k) of the interior, i fastest
+        const int64_t nn = n;
+        const int64_t i = idx % nn, jj = (idx / nn) % nn, kz = idx / (nn * nn);
+        out[(i + 1) + nfull * ((jj + 1) + (int64_t)nfull * (kz + 1))] = val;
       }
     }
   }
 }
 
+// K slice per lane group: the smallest odd bucket >= n/4 (0 beyond n = 276)
+static int gemm_kq(int n) {
+  static const int buckets[] = {5, 9, 15, 17, 25, 33, 41, 49, 57, 65, 69};
+  const int need = (n + 3) / 4;
+  for (int b : buckets)
+    if (b >= need) return b;
+  return 0;
+}
+
+// A (n x n, column-major FP64) in operand order: element
+// [(r KQ + s) 64 + l] = A[16 r + l % 16][(l / 16) KQ + s], zero outside.
+int vh_gemm_pack(vh_context *c, float **dst, const double *colmajor, int n) {
+  const int mt = (n + 15) / 16, kq = gemm_kq(n);
+  VH_REQUIRE(n >= 2 && mt <= 17 && kq > 0, "rotating product: n = %d beyond 272", n);
+  std::vector<float> op((size_t)mt * kq * 64);
+  for (int r = 0; r < mt; ++r)
+    for (int s = 0; s < kq; ++s)
+      for (int l = 0; l < 64; ++l) {
+        const int row = 16 * r + (l & 15), k = (l >> 4) * kq + s;
+        op[((size_t)r * kq + s) * 64 + l] =
+            (row < n && k < n) ? (float)colmajor[row + (size_t)n * k] : 0.f;
+      }
+  VH_TRY(vh_alloc((void **)dst, op.size() * sizeof(float)));
+  VH_CHECK_HIP(hipMemcpy(*dst, op.data(), op.size() * sizeof(float), hipMemcpyHostToDevice));
+  return VH_OK;
+}
+
+// launch shape: column tiles per block, and waves (row tiles) per block
+#define GEMM_NCT 2
+#define GEMM_WAVES(mt) ((mt) <= 9 ? (mt) : ((mt) + 1) / 2)
+
 int vh_rotate_gemm(vh_context *c, const float *A, const float *src, float *out, int n, int64_t m,
                    int epi, const float *aux, int nfull) {
   const int mt = (n + 15) / 16;
   VH_REQUIRE(mt >= 1 && mt <= 17, "rotating product: n = %d beyond 272", n);
-  dim3 grid((unsigned)((m + 31) / 32)), block(256);
-  const size_t lds = sizeof(float) * 32 * (size_t)(n + 1);
-  switch (mt) {
-#define CASE(T)                                                                                 \
-  case T:                                                                                       \
-    hipLaunchKernelGGL(k_rot_gemm<T>, grid, block, lds, c->stream, A, src, out, n, m, epi, aux, \
-                       nfull);                                                                  \
+  VH_REQUIRE(n % 2 == 0 && ((uintptr_t)src & 7) == 0,
+             "rotating product: n (%d) must be even and the source 8-byte aligned", n);
+  const int kq = gemm_kq(n);
+  const int ldb = n + ((2 - n % 32) + 32) % 32;  // smallest >= n with ldb = 2 (mod 32)
+  const int wv = GEMM_WAVES(mt);
+  constexpr int nc = 16 * GEMM_NCT;
+  dim3 grid((unsigned)((m + nc - 1) / nc), (unsigned)((mt + wv - 1) / wv)), block(64 * wv);
+  const size_t lds = sizeof(float) * nc * (size_t)ldb;
+  switch (kq) {
+#define CASE(T)                                                                                    \
+  case T:                                                                                          \
+    hipLaunchKernelGGL((k_rot_gemm<T, GEMM_NCT>), grid, block, lds, c->stream, A, src, out, n, m,  \
+                       ldb, epi, aux, nfull);                                                      \
     break;
-    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9)
-    CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17)
+    CASE(5) CASE(9) CASE(15) CASE(17) CASE(25) CASE(33) CASE(41) CASE(49) CASE(57) CASE(65) CASE(69)
 #undef CASE
   }
   VH_LAUNCH_CHECK();
@@ -402,10 +467,10 @@ static int solve_interior(vh_mesh *m) {
                      m->lap, m->n);
   VH_LAUNCH_CHECK();
   // forward transform, Hadamard fused into the third product
-  VH_TRY(apply_all(c, m->Minv_rm, ni, m->inner, m->work1, m->work2, m->work1, EPI_HADAMARD,
+  VH_TRY(apply_all(c, m->Minv_op, ni, m->inner, m->work1, m->work2, m->work1, EPI_HADAMARD,
                    m->invlsum, 0));
   // inverse transform, the copy into phi's interior fused into the third product
-  VH_TRY(apply_all(c, m->M_rm, ni, m->work1, m->inner, m->work2, m->phi, EPI_FILL_INTERIOR,
+  VH_TRY(apply_all(c, m->M_op, ni, m->work1, m->inner, m->work2, m->phi, EPI_FILL_INTERIOR,
                    nullptr, m->n));
   return VH_OK;
 }
@@ -422,7 +487,7 @@ int vh_poisson_single_impl(vh_mesh *m) {
 
 int vh_spline_coefficients_impl(vh_mesh *m, const float *src, float *dst) {
   // src -> dst -> work -> dst, as apply_all_rotating! alternates (src/tensorsolver.jl:243-254)
-  VH_TRY(apply_all(m->ctx, m->Sinv_rm, m->n, src, dst, m->work, dst, EPI_STORE, nullptr, 0));
+  VH_TRY(apply_all(m->ctx, m->Sinv_op, m->n, src, dst, m->work, dst, EPI_STORE, nullptr, 0));
   return VH_OK;
 }
 
@@ -466,7 +531,7 @@ int vh_effective_potential_impl(vh_mesh *m, double nions, double radius) {
                      m->n, (float)nions, (float)radius);
   VH_LAUNCH_CHECK();
   // extra -> work -> extra -> csol (+=)
-  VH_TRY(apply_all(c, m->Sinv_rm, m->n, m->extra, m->work, m->extra, m->csol, EPI_ACCUMULATE,
+  VH_TRY(apply_all(c, m->Sinv_op, m->n, m->extra, m->work, m->extra, m->csol, EPI_ACCUMULATE,
                    nullptr, 0));
   return VH_OK;
 }
