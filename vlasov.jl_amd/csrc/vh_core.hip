@@ -268,7 +268,7 @@ extern "C" int vh_accel_destroy(vh_accel *a) {
   if (!a) return VH_OK;
   hipSetDevice(a->ctx->device);
   hipStreamSynchronize(a->ctx->stream);
-  void *ptrs[] = {a->ovl, a->grad, a->nodes, a->ovl64, a->knots64, a->parts[0], a->parts[1], a->force,
+  void *ptrs[] = {a->ovl, a->grad, a->nodes, a->og, a->ovl64, a->knots64, a->parts[0], a->parts[1], a->force,
                   a->keys[0], a->keys[1], a->vals[0], a->vals[1], a->counts, a->scan_tmp,
                   a->occupied, a->bounds, a->cellmap, a->fitlist, a->nonfit, a->nonfit_prefix,
                   a->outlist, a->counters, a->flags, a->tiles, a->dpart, a->dscal, a->cellscratch};
@@ -337,6 +337,15 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
   MK(upload_converted(ctx, &a->grad, gradient, (size_t)10 * a->ncol));
   MK(upload_converted(ctx, &a->nodes, nodes, (size_t)8 * a->ncol));
   MK(upload_converted(ctx, &a->ovl64, overlap, (size_t)10 * a->ncol));
+  {  // overlap and gradient of a column side by side, 80 bytes per column
+    std::vector<double> og((size_t)20 * a->ncol);
+    for (int c = 0; c < a->ncol; ++c)
+      for (int k = 0; k < 10; ++k) {
+        og[(size_t)20 * c + k] = overlap[(size_t)10 * c + k];
+        og[(size_t)20 * c + 10 + k] = gradient[(size_t)10 * c + k];
+      }
+    MK(upload_converted(ctx, &a->og, og.data(), og.size()));
+  }
   MK(upload_converted(ctx, &a->knots64, knots, (size_t)nk));
   size_t np = (size_t)a->npad;
   MK(vh_alloc((void **)&a->parts[0], np * 4 * PS_COUNT));

@@ -217,41 +217,133 @@ __global__ void __launch_bounds__(64)
 // particles; phase C stores.
 #define FT_STRIDE 61
 
-// Operands of one sub-group for the lane group `grp`, from its particle's LDS
-// row: b = {ox, ox, ox, gx, gx, gx} at ii = 4k + grp (0 beyond 9), y = oy then
-// gy (all ten), z = {oz, oz, oz, gz, gz, gz} at kk = grp + 4m (0 beyond 9).
-// Every read is unconditional (index 8 + grp exists for grp < 2 only, the
-// other lanes read an in-bounds stand-in and select 0): conditional reads
-// would become exec-masked branches, each waiting for all loads in flight.
-__device__ __forceinline__ void ft_operands(const float *row, int grp, float b[6], float y[20],
-                                            float z[6]) {
+typedef float ft_f4 __attribute__((ext_vector_type(4)));
+
+// B operands of one sub-group for the lane group `grp`, from its particle's
+// LDS row: {ox, ox, ox, gx, gx, gx} at ii = 4k + grp (0 beyond 9).  Every
+// read is unconditional (index 8 + grp exists for grp < 2 only, the other
+// lanes read an in-bounds stand-in and select 0): conditional reads would
+// become exec-masked branches, each waiting for all loads in flight.
+__device__ __forceinline__ void ft_bops(const float *row, int grp, float b[6]) {
   const bool top = grp < 2;
   const int gt = top ? grp : 0;
-  const float b2 = row[8 + gt], b5 = row[18 + gt], z2 = row[48 + gt], z5 = row[58 + gt];
+  const float b2 = row[8 + gt], b5 = row[18 + gt];
   b[0] = row[grp];
   b[1] = row[4 + grp];
   b[2] = top ? b2 : 0.f;
   b[3] = row[10 + grp];
   b[4] = row[14 + grp];
   b[5] = top ? b5 : 0.f;
+}
+
+// The contraction of NS sub-groups at once (sg0 .. sg0 + NS - 1), pipelined
+// over the 8 row tiles: tile t's products (T_o and T_g chains of every
+// sub-group interleaved over the three K steps) are issued, then the y
+// contraction of tile t - 1, whose products are complete, runs on the VALU
+// while tile t's are in flight.  Slot q = 4t + v of a lane holds plane
+// kk = grp + 4 (q / 10), jj = q % 10.  Then the z contraction, the sum over
+// the four lane groups of a particle, and the store of -w F into the chunk's
+// slot (all four lanes store the same value).
+template <int NS>
+__device__ __forceinline__ void ft_contract(const float A[8][3], const float *tabj, int sg0, int grp,
+                                            int j16, float w, float (*fld)[64]) {
+  const bool top = grp < 2;
+  const int gt = top ? grp : 0;
+  float b[NS][6], y[NS][20], z[NS][6];
 #pragma unroll
-  for (int a = 0; a < 20; ++a) y[a] = row[20 + a];
-  z[0] = row[40 + grp];
-  z[1] = row[44 + grp];
-  z[2] = top ? z2 : 0.f;
-  z[3] = row[50 + grp];
-  z[4] = row[54 + grp];
-  z[5] = top ? z5 : 0.f;
+  for (int u = 0; u < NS; ++u) {
+    const float *row = tabj + 16 * (sg0 + u) * FT_STRIDE;
+    ft_bops(row, grp, b[u]);
+#pragma unroll
+    for (int a = 0; a < 20; ++a) y[u][a] = row[20 + a];  // oy, then gy
+    const float z2 = row[48 + gt], z5 = row[58 + gt];
+    z[u][0] = row[40 + grp];
+    z[u][1] = row[44 + grp];
+    z[u][2] = top ? z2 : 0.f;
+    z[u][3] = row[50 + grp];
+    z[u][4] = row[54 + grp];
+    z[u][5] = top ? z5 : 0.f;
+  }
+  float ux[NS][3], uy[NS][3], uz[NS][3];
+#pragma unroll
+  for (int u = 0; u < NS; ++u)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) ux[u][m] = uy[u][m] = uz[u][m] = 0.f;
+  ft_f4 To[NS][8], Tg[NS][8];
+#pragma unroll
+  for (int t = 0; t <= 8; ++t) {
+    if (t < 8) {
+#pragma unroll
+      for (int u = 0; u < NS; ++u) {
+        To[u][t] = ft_f4{0.f, 0.f, 0.f, 0.f};
+        Tg[u][t] = ft_f4{0.f, 0.f, 0.f, 0.f};
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+          To[u][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[t][k], b[u][k], To[u][t], 0, 0, 0);
+          Tg[u][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[t][k], b[u][3 + k], Tg[u][t], 0, 0, 0);
+        }
+      }
+    }
+    if (t >= 1) {
+      const int tp = t - 1;
+#pragma unroll
+      for (int u = 0; u < NS; ++u) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int q = 4 * tp + v;
+          if (q < 30) {
+            const int m = q / 10, jj = q % 10;
+            ux[u][m] = __builtin_fmaf(y[u][jj], Tg[u][tp][v], ux[u][m]);
+            uy[u][m] = __builtin_fmaf(y[u][10 + jj], To[u][tp][v], uy[u][m]);
+            uz[u][m] = __builtin_fmaf(y[u][jj], To[u][tp][v], uz[u][m]);
+          }
+        }
+      }
+    }
+  }
+  // schedule: the LDS reads, tile 0's products, then one product per two
+  // VALU instructions (left to itself the compiler issues all the products
+  // back to back and the y contraction after them)
+  __builtin_amdgcn_sched_group_barrier(0x100, 16 * NS, 0);  // DS reads
+  __builtin_amdgcn_sched_group_barrier(0x008, 6 * NS, 0);   // MFMA
+#pragma unroll
+  for (int i = 0; i < 42 * NS; ++i) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+    __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // VALU
+  }
+#pragma unroll
+  for (int u = 0; u < NS; ++u) {
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      fx = __builtin_fmaf(z[u][m], ux[u][m], fx);
+      fy = __builtin_fmaf(z[u][m], uy[u][m], fy);
+      fz = __builtin_fmaf(z[u][3 + m], uz[u][m], fz);
+    }
+    fx += __shfl_xor(fx, 16, 64);
+    fy += __shfl_xor(fy, 16, 64);
+    fz += __shfl_xor(fz, 16, 64);
+    fx += __shfl_xor(fx, 32, 64);
+    fy += __shfl_xor(fy, 32, 64);
+    fz += __shfl_xor(fz, 32, 64);
+    const int slot = 16 * (sg0 + u) + j16;
+    fld[0][slot] = -w * fx;
+    fld[1][slot] = -w * fy;
+    fld[2][slot] = -w * fz;
+  }
 }
 
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     k_field_mfma(const int32_t *__restrict__ P, int64_t npad, const float *__restrict__ csol, int n,
-                 const float *__restrict__ ovl, const float *__restrict__ grad,
+                 const float4 *__restrict__ og,
                  const int32_t *__restrict__ occupied, const int32_t *__restrict__ bounds,
                  const int32_t *__restrict__ cnt, int nk, float x0, float h, float spacing,
                  float nbdt, float w, int ncol, vh_proj_args pa, float *__restrict__ force,
                  double *__restrict__ part) {
-  typedef float f4 __attribute__((ext_vector_type(4)));
+  typedef ft_f4 f4;
   __shared__ float fld[3][64];
   __shared__ float tab[64 * FT_STRIDE];
   const float *D = (const float *)P;
@@ -300,18 +392,19 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
         cx = min(max(cx, 1), ncol) - 1;
         cy = min(max(cy, 1), ncol) - 1;
         cz = min(max(cz, 1), ncol) - 1;
-        // 10-float columns start at even offsets: 8-byte loads
-        const float2 *src[6] = {(const float2 *)(ovl + cx * 10), (const float2 *)(grad + cx * 10),
-                                (const float2 *)(ovl + cy * 10), (const float2 *)(grad + cy * 10),
-                                (const float2 *)(ovl + cz * 10), (const float2 *)(grad + cz * 10)};
+        // each axis: overlap and gradient side by side, five 16-byte loads
+        const int col[3] = {cx, cy, cz};
         float *row = tab + l * FT_STRIDE;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
+        for (int d = 0; d < 3; ++d) {
+          const float4 *src = og + 5 * col[d];
 #pragma unroll
           for (int a = 0; a < 5; ++a) {
-            const float2 v = src[c][a];
-            row[10 * c + 2 * a] = v.x;
-            row[10 * c + 2 * a + 1] = v.y;
+            const float4 v = src[a];
+            row[20 * d + 4 * a] = v.x;
+            row[20 * d + 4 * a + 1] = v.y;
+            row[20 * d + 4 * a + 2] = v.z;
+            row[20 * d + 4 * a + 3] = v.w;
           }
         }
       }
@@ -347,66 +440,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
         }
       }
       __syncthreads();
-      // phase B: the contraction, 16 particles per sub-group; each sub-group's
-      // operands are read from LDS one sub-group ahead
+      // phase B: the contraction, one sub-group of 16 particles at a time
+      // (two at a time, ft_contract<2>, measured no faster at twice the
+      // registers)
       if (ok) {
         const int nsg = (nlive + 15) >> 4;
+        const float *tabj = tab + j16 * FT_STRIDE;
 #pragma unroll 1
-        for (int sg = 0; sg < nsg; ++sg) {
-          float b[6], y[20], z[6];
-          ft_operands(tab + (16 * sg + j16) * FT_STRIDE, grp, b, y, z);
-          // stage 1
-          f4 To[8], Tg[8];
-#pragma unroll
-          for (int t = 0; t < 8; ++t) {
-            To[t] = f4{0.f, 0.f, 0.f, 0.f};
-            Tg[t] = f4{0.f, 0.f, 0.f, 0.f};
-          }
-#pragma unroll
-          for (int k = 0; k < 3; ++k) {
-#pragma unroll
-            for (int t = 0; t < 8; ++t) {
-              To[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[t][k], b[k], To[t], 0, 0, 0);
-              Tg[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[t][k], b[3 + k], Tg[t], 0, 0, 0);
-            }
-          }
-          // stage 2: y contraction per kk plane (m = 0..2 of this lane group);
-          // y[0..9] = oy, y[10..19] = gy
-          float ux[3] = {0.f, 0.f, 0.f}, uy[3] = {0.f, 0.f, 0.f}, uz[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-          for (int t = 0; t < 8; ++t) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-              const int q = 4 * t + v;
-              if (q < 30) {
-                const int m = q / 10, jj = q % 10;
-                ux[m] = __builtin_fmaf(y[jj], Tg[t][v], ux[m]);
-                uy[m] = __builtin_fmaf(y[10 + jj], To[t][v], uy[m]);
-                uz[m] = __builtin_fmaf(y[jj], To[t][v], uz[m]);
-              }
-            }
-          }
-          // stage 3: z contraction of this lane group's planes kk = grp + 4m
-          // (z[m] = oz[kk], z[3 + m] = gz[kk], zero where kk > 9)
-          float fx = 0.f, fy = 0.f, fz = 0.f;
-#pragma unroll
-          for (int m = 0; m < 3; ++m) {
-            fx = __builtin_fmaf(z[m], ux[m], fx);
-            fy = __builtin_fmaf(z[m], uy[m], fy);
-            fz = __builtin_fmaf(z[3 + m], uz[m], fz);
-          }
-          fx += __shfl_xor(fx, 16, 64);
-          fy += __shfl_xor(fy, 16, 64);
-          fz += __shfl_xor(fz, 16, 64);
-          fx += __shfl_xor(fx, 32, 64);
-          fy += __shfl_xor(fy, 32, 64);
-          fz += __shfl_xor(fz, 32, 64);
-          if (grp == 0) {
-            fld[0][16 * sg + j16] = -w * fx;
-            fld[1][16 * sg + j16] = -w * fy;
-            fld[2][16 * sg + j16] = -w * fz;
-          }
-        }
+        for (int sg = 0; sg < nsg; ++sg) ft_contract<1>(A, tabj, sg, grp, j16, w, fld);
       }
       __syncthreads();
       // phase C: field minus projectile term, coalesced stores
@@ -716,10 +757,16 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
   int grid = (int)(a->maxocc < 65536 ? a->maxocc : 65536);
   if (4 * (int64_t)grid > 8 * a->dpart_len) grid = (int)(2 * a->dpart_len);
   int tok = vh_prof_begin(c, VH_ST_FIELD_KERNEL);
-  hipLaunchKernelGGL(a->exact_field ? k_field : k_field_mfma, dim3(grid), dim3(64), 0, st,
-                     a->parts[a->cur], a->npad, fine->csol, a->n, a->ovl, a->grad, a->occupied,
-                     a->bounds, a->counters, a->nk, (float)a->x0, (float)a->h, a->spacing,
-                     (float)a->nbdt, (float)weight, a->ncol, pa, a->force, a->dpart);
+  if (a->exact_field)
+    hipLaunchKernelGGL(k_field, dim3(grid), dim3(64), 0, st, a->parts[a->cur], a->npad,
+                       fine->csol, a->n, a->ovl, a->grad, a->occupied, a->bounds, a->counters,
+                       a->nk, (float)a->x0, (float)a->h, a->spacing, (float)a->nbdt,
+                       (float)weight, a->ncol, pa, a->force, a->dpart);
+  else
+    hipLaunchKernelGGL(k_field_mfma, dim3(grid), dim3(64), 0, st, a->parts[a->cur], a->npad,
+                       fine->csol, a->n, (const float4 *)a->og, a->occupied, a->bounds,
+                       a->counters, a->nk, (float)a->x0, (float)a->h, a->spacing, (float)a->nbdt,
+                       (float)weight, a->ncol, pa, a->force, a->dpart);
   vh_prof_end(c, tok);
   VH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_reduce_cols, dim3(4), dim3(1024), 0, st, a->dpart, grid, 4, a->dscal + 8);
