@@ -191,6 +191,23 @@ int vh_verlet(vh_accel *a, double dt, double mass, double rcmax, double diag[5])
  * particles closer than `radius` to `center` (multiply by the weight). */
 int vh_enclosed_count(vh_accel *a, const double center[3], double radius, int64_t *count);
 
+/* --- the projectile kept on the device ------------------------------------
+ * No counterpart in the reference, which integrates the projectile on the
+ * host between the force kernel and the cloud's push (src/simulation.jl:524,
+ * src/accelerator.jl:746-756, src/projectile.jl:298-304), and so
+ * synchronises every step.  With the projectile on the device a step needs
+ * no host synchronisation.  state = position[3], previous[3], velocity[3]. */
+int vh_projectile_set(vh_accel *a, const double state[9], double mass, double charge,
+                      double scale, int soft_mode, double ball_radius, double jel_nions,
+                      double jel_radius);
+int vh_projectile_get(vh_accel *a, double state[9]);
+/* forces!(...; projectile) as vh_forces, with the device projectile's
+ * position; no output, no synchronisation. */
+int vh_forces_resident(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight);
+/* projectile_forces!(cloud, acc, proj, jel) from the last vh_forces_resident's
+ * reduction, then step!(proj, force, dt), in FP64 on the device. */
+int vh_projectile_step(vh_accel *a, double dt);
+
 /* interaction_energy(cloud, fine, csol_f, coarse, csol_c, sm; enclosed)
  * (src/energy.jl:19-43): sum over the particles of weight * phi(r), in FP64,
  * with phi the smoothed potential of the fine csol inside

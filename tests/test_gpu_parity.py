@@ -114,6 +114,34 @@ def test_coarse_deposit(setup, orc):
     assert dmc.rho.tobytes() == rho.tobytes()
 
 
+def test_fresh_objects_start_clean(setup, orc):
+    """Objects created after others were used and released start from zero:
+    the allocator hands back memory holding the earlier owner's grids, and
+    the first coarse deposit of a new mesh sums into its accumulator.  Three
+    generations of context, meshes and accelerator, each run through the
+    chain, each first deposit checked against the oracle."""
+    s = setup
+    rec = s["rec"]
+    want = None
+    for _ in range(3):
+        ctx = va.Context(0)
+        dmf, dmc = va.DeviceMesh(ctx, s["fine"]), va.DeviceMesh(ctx, s["coarse"])
+        acc = va.DeviceAccelerator(ctx, s["fine"].axis, s["sm"], NPART)
+        try:
+            acc.upload(rec)
+            acc.deposit_smoothed(dmf, s["w"], count=False)
+            acc.deposit_cic(dmc, s["w"])
+            rho = dmc.rho
+            if want is None:
+                want = s["ost"].deposit_coarse(acc.download())
+            assert relerr(rho, want) < REL
+            va.poisson_nested(dmf, dmc)
+            dmf.spline_coefficients()
+            acc.compute_forces(dmf, dmc, s["w"])
+        finally:
+            acc.close(); dmf.close(); dmc.close(); ctx.close()
+
+
 def test_poisson_chain(setup, orc):
     """nested poisson!, spline_coefficients! and effective_potential! on the
     oracle's own densities (narrowed to Float32)."""
@@ -334,3 +362,31 @@ def test_step_with_energy_budget(config1, orc):
         assert b.ions == 3 * p.nions ** 2 / (5 * va.Jellium.of(p.nions).radius)
     finally:
         sim.close()
+
+
+def test_resident_projectile_matches_host(config1):
+    """The projectile integrated on the device (so that a step waits for
+    nothing on the host) follows the host integration of the same step:
+    the cloud identical to the bit after four steps, the projectile's state to
+    FP64 rounding (the jellium term's r2^1.5 is the device's pow)."""
+    p, fine, coarse, sm = config1
+    n = p.nparticles
+    w = p.nelectrons / n
+    pos, mom = va.sample_thomas_fermi(va.read_radial_profile(), n, w, seed=-5)
+    sims = []
+    try:
+        for resident in (True, False):
+            proj = va.xenon_projectile(x0=-20.0, impact=10.0)
+            sims.append(va.Simulation(p, None, projectile=proj, tables=(fine, coarse, sm),
+                                      positions=pos, momenta=mom, resident_projectile=resident))
+        for _ in range(4):
+            for s in sims:
+                s.step()
+        a, b = sims[0].projectile, sims[1].projectile
+        assert np.allclose(a.position, b.position, rtol=1e-13, atol=1e-12)
+        assert np.allclose(a.previous, b.previous, rtol=1e-13, atol=1e-12)
+        assert np.allclose(a.velocity, b.velocity, rtol=1e-10, atol=1e-14)
+        assert sims[0].acc.download().tobytes() == sims[1].acc.download().tobytes()
+    finally:
+        for s in sims:
+            s.close()

@@ -61,6 +61,11 @@ _SIGNATURES = {
     "vh_forces": [_vp, _vp, _vp, C.c_double, _vp, C.c_int, C.c_double, _vp, C.POINTER(C.c_int32)],
     "vh_verlet": [_vp, C.c_double, C.c_double, C.c_double, _vp],
     "vh_enclosed_count": [_vp, _f64p, C.c_double, C.POINTER(C.c_int64)],
+    "vh_projectile_set": [_vp, _f64p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double,
+                          C.c_double, C.c_double],
+    "vh_projectile_get": [_vp, _f64p],
+    "vh_forces_resident": [_vp, _vp, _vp, C.c_double],
+    "vh_projectile_step": [_vp, C.c_double],
     "vh_interaction_energy": [_vp, _vp, _vp, C.c_double, C.c_double, C.POINTER(C.c_double)],
     "vh_profile_enable": [_vp, C.c_int],
     "vh_profile_read": [_vp, _f64p, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")],

@@ -221,9 +221,20 @@ extern "C" int vh_mesh_create(vh_context *ctx, int n, int nk, const double *coll
       return VH_EHIP;
     }
   }
+  // every buffer starts at zero: device memory comes back from the allocator
+  // holding whatever an earlier owner left, and the CIC accumulator in
+  // particular is summed into (the deposit clears it again after each use)
   MK(vh_fill_f32(ctx, m->rho, (int64_t)g, 0.f));
   MK(vh_fill_f32(ctx, m->phi, (int64_t)g, 0.f));
   MK(vh_fill_f32(ctx, m->csol, (int64_t)g, 0.f));
+  MK(vh_fill_f32(ctx, m->extra, (int64_t)g, 0.f));
+  MK(vh_fill_f32(ctx, m->work, (int64_t)g, 0.f));
+  MK(vh_fill_f32(ctx, m->inner, (int64_t)gi, 0.f));
+  MK(vh_fill_f32(ctx, m->work1, (int64_t)gi, 0.f));
+  MK(vh_fill_f32(ctx, m->work2, (int64_t)gi, 0.f));
+  MK(vh_fill_u64(ctx, (uint64_t *)m->acc64, (int64_t)g, 0ull));
+  MK(vh_fill_u64(ctx, (uint64_t *)m->partials, (int64_t)10 * n * n, 0ull));
+  MK(vh_fill_u64(ctx, (uint64_t *)m->scal, 32, 0ull));
 #undef MK
   VH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   *out = m;
@@ -271,7 +282,8 @@ extern "C" int vh_accel_destroy(vh_accel *a) {
   void *ptrs[] = {a->ovl, a->grad, a->nodes, a->og, a->ovl64, a->knots64, a->parts[0], a->parts[1], a->force,
                   a->keys[0], a->keys[1], a->vals[0], a->vals[1], a->counts, a->scan_tmp,
                   a->occupied, a->bounds, a->cellmap, a->fitlist, a->nonfit, a->nonfit_prefix,
-                  a->outlist, a->counters, a->flags, a->tiles, a->dpart, a->dscal, a->cellscratch};
+                  a->outlist, a->counters, a->flags, a->tiles, a->dpart, a->dscal, a->cellscratch,
+                  a->dproj};
   for (void *p : ptrs) vh_free(p);
   delete a;
   return VH_OK;
@@ -377,6 +389,8 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
   MK(vh_fill_i32(ctx, a->counters, 64, 0));
   MK(vh_fill_f32(ctx, a->force, (int64_t)np * 3, 0.f));
   MK(vh_fill_i32(ctx, a->parts[0], (int64_t)np * PS_COUNT, 0));
+  MK(vh_fill_i32(ctx, a->parts[1], (int64_t)np * PS_COUNT, 0));
+  MK(vh_fill_u64(ctx, (uint64_t *)a->dscal, 64, 0ull));
 #undef MK
   VH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   a->cur = 0;
@@ -764,12 +778,81 @@ extern "C" int vh_forces(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double wei
   VH_CHECK_HIP(hipSetDevice(a->ctx->device));
   {
     vh_region r(a->ctx, VH_ST_FIELD);
-    VH_TRY(vh_forces_impl(a, fine, coarse, weight, proj, soft_mode, ball_radius));
+    VH_TRY(vh_forces_impl(a, fine, coarse, weight, proj, soft_mode, ball_radius, nullptr));
   }
   if (red) VH_CHECK_HIP(hipMemcpyAsync(red, a->dscal + 8, 32, hipMemcpyDeviceToHost, a->ctx->stream));
   if (nout) VH_CHECK_HIP(hipMemcpyAsync(nout, a->counters + CNT_OUTCOUNT, 4, hipMemcpyDeviceToHost, a->ctx->stream));
   if (red || nout) VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
   return VH_OK;
+}
+
+extern "C" int vh_projectile_set(vh_accel *a, const double state[9], double mass, double charge,
+                                 double scale, int soft_mode, double ball_radius, double jel_nions,
+                                 double jel_radius) {
+  VH_REQUIRE(a && state, "vh_projectile_set: null argument");
+  VH_REQUIRE(soft_mode == VH_SOFT_DEVICE || soft_mode == VH_SOFT_BALL,
+             "vh_projectile_set: unknown softening mode %d", soft_mode);
+  VH_REQUIRE(mass > 0 && scale > 0 && jel_radius > 0 && std::isfinite(charge) &&
+                 std::isfinite(jel_nions), "vh_projectile_set: bad projectile or jellium constants");
+  for (int i = 0; i < 9; ++i) VH_REQUIRE(std::isfinite(state[i]), "vh_projectile_set: state must be finite");
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  if (!a->dproj) VH_TRY(vh_alloc((void **)&a->dproj, sizeof(double) * 16));
+  double h[16] = {0};
+  for (int i = 0; i < 9; ++i) h[i] = state[i];
+  h[9] = mass;
+  h[10] = charge;
+  h[11] = jel_nions;
+  h[12] = jel_radius;
+  VH_CHECK_HIP(hipMemcpyAsync(a->dproj, h, sizeof h, hipMemcpyHostToDevice, a->ctx->stream));
+  VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
+  a->proj_charge = charge;
+  a->proj_scale = scale;
+  a->proj_soft_mode = soft_mode;
+  a->proj_ball_radius = ball_radius;
+  a->has_dproj = 1;
+  return VH_OK;
+}
+
+extern "C" int vh_projectile_get(vh_accel *a, double state[9]) {
+  VH_REQUIRE(a && state, "vh_projectile_get: null argument");
+  if (!a->has_dproj) {
+    vh_set_error("vh_projectile_get: no projectile on the device");
+    return VH_ESTATE;
+  }
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  VH_CHECK_HIP(hipMemcpyAsync(state, a->dproj, sizeof(double) * 9, hipMemcpyDeviceToHost, a->ctx->stream));
+  VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
+  return VH_OK;
+}
+
+extern "C" int vh_forces_resident(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight) {
+  VH_REQUIRE(a && fine && coarse, "vh_forces_resident: null argument");
+  VH_REQUIRE(fine->n == a->n, "vh_forces_resident: fine csol has n=%d, accelerator n=%d", fine->n, a->n);
+  VH_REQUIRE(fine->ctx == a->ctx && coarse->ctx == a->ctx,
+             "vh_forces_resident: objects on different contexts");
+  VH_REQUIRE(std::isfinite(weight), "vh_forces_resident: weight must be finite");
+  if (!a->has_dproj || !a->sorted_once) {
+    vh_set_error("vh_forces_resident: %s", a->has_dproj ? "the cloud has not been sorted"
+                                                         : "no projectile on the device");
+    return VH_ESTATE;
+  }
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  // the position entries are read on the device; charge and scale from here
+  const double proj[5] = {0.0, 0.0, 0.0, a->proj_charge, a->proj_scale};
+  vh_region r(a->ctx, VH_ST_FIELD);
+  return vh_forces_impl(a, fine, coarse, weight, proj, a->proj_soft_mode, a->proj_ball_radius,
+                        a->dproj);
+}
+
+extern "C" int vh_projectile_step(vh_accel *a, double dt) {
+  VH_REQUIRE(a != nullptr, "vh_projectile_step: null accelerator");
+  VH_REQUIRE(dt > 0 && std::isfinite(dt), "vh_projectile_step: dt must be positive");
+  if (!a->has_dproj) {
+    vh_set_error("vh_projectile_step: no projectile on the device");
+    return VH_ESTATE;
+  }
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  return vh_projectile_step_impl(a, dt);
 }
 
 extern "C" int vh_interaction_energy(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight,

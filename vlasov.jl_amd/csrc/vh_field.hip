@@ -74,6 +74,7 @@ struct vh_proj_args {
   float sigma;        // E(scale(softening))
   int ball;           // VH_SOFT_BALL
   float radius;       // ball radius (ball mode)
+  const double *dev;  // device-resident projectile state (position at 0..2), or null
 };
 
 __global__ void __launch_bounds__(64)
@@ -85,6 +86,11 @@ __global__ void __launch_bounds__(64)
             double *__restrict__ part) {
   const float *D = (const float *)P;
   const int t = threadIdx.x;
+  if (pa.dev) {
+    pa.px = (float)pa.dev[0];
+    pa.py = (float)pa.dev[1];
+    pa.pz = (float)pa.dev[2];
+  }
   const float half = spacing * 0.5f;
   const float nan = __int_as_float(0x7fc00000);
   const int nocc = cnt[CNT_NOCC];
@@ -344,6 +350,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
                  float nbdt, float w, int ncol, vh_proj_args pa, float *__restrict__ force,
                  double *__restrict__ part) {
   typedef ft_f4 f4;
+  if (pa.dev) {
+    pa.px = (float)pa.dev[0];
+    pa.py = (float)pa.dev[1];
+    pa.pz = (float)pa.dev[2];
+  }
   __shared__ float fld[3][64];
   __shared__ float tab[64 * FT_STRIDE];
   const float *D = (const float *)P;
@@ -521,6 +532,7 @@ struct vh_fallback_args {
   int has_proj;
   double w;           // weight
   double escaped;     // enclosed-charge count for particles outside both grids
+  const double *dev;  // device-resident projectile state (position at 0..2), or null
 };
 
 __global__ void k_fallback(const int32_t *__restrict__ P, int64_t npad, const int32_t *__restrict__ outlist,
@@ -529,6 +541,11 @@ __global__ void k_fallback(const int32_t *__restrict__ P, int64_t npad, const in
                            int nc, vh_fallback_args fa, float *__restrict__ force) {
   const int nout = cnt[CNT_OUTCOUNT];
   const float *D = (const float *)P;
+  if (fa.dev) {
+    fa.px = fa.dev[0];
+    fa.py = fa.dev[1];
+    fa.pz = fa.dev[2];
+  }
   for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nout;
        u += (int64_t)gridDim.x * blockDim.x) {
     const int s = outlist[u];
@@ -734,10 +751,11 @@ int vh_energy_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, d
 }
 
 int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, const double *proj,
-                   int soft_mode, double ball_radius) {
+                   int soft_mode, double ball_radius, const double *dproj) {
   vh_context *c = a->ctx;
   hipStream_t st = c->stream;
   vh_proj_args pa;
+  pa.dev = dproj;
   pa.px = pa.py = pa.pz = 0.f;
   pa.coef = 0.f;
   pa.sigma = 1.f;
@@ -781,9 +799,40 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
   fa.has_proj = proj != nullptr;
   fa.w = weight;
   fa.escaped = 0.0;
+  fa.dev = dproj;
   hipLaunchKernelGGL(k_fallback, dim3(1024), dim3(256), 0, st, a->parts[a->cur], a->npad,
                      a->outlist, a->counters, a->x0, a->h, coarse->csol, coarse->knots64, coarse->nk,
                      coarse->n, fa, a->force);
+  VH_LAUNCH_CHECK();
+  return VH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// the projectile on the device: projectile_forces! (src/projectile.jl:203-216)
+// from the field kernel's reduction plus the jellium term, then its Verlet
+// step (src/projectile.jl:298-304), in FP64 with the reference's operation
+// order.  State: [0..2] position, [3..5] previous, [6..8] velocity,
+// [9] mass, [10] charge, [11] jellium ions, [12] jellium radius.
+// ---------------------------------------------------------------------------
+__global__ void k_projectile_step(double *__restrict__ st, const double *__restrict__ red, double dt) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double p[3] = {st[0], st[1], st[2]}, prev[3] = {st[3], st[4], st[5]};
+  const double q = st[10], R = st[12];
+  const double r2 = (p[0] * p[0] + p[1] * p[1]) + p[2] * p[2];
+  const double modf = r2 > R * R ? st[11] * q / pow(r2, 1.5) : q / 64.0;  // WIGNER_SEITZ_NA^3
+  const double c = dt * dt / st[9];
+  for (int d = 0; d < 3; ++d) {
+    const double f = modf * p[d] + red[d];
+    const double nw = (2.0 * p[d] - prev[d]) + c * f;
+    st[6 + d] = (nw - prev[d]) / (2.0 * dt);
+    st[3 + d] = p[d];
+    st[d] = nw;
+  }
+}
+
+int vh_projectile_step_impl(vh_accel *a, double dt) {
+  hipLaunchKernelGGL(k_projectile_step, dim3(1), dim3(64), 0, a->ctx->stream, a->dproj, a->dscal + 8,
+                     dt);
   VH_LAUNCH_CHECK();
   return VH_OK;
 }

@@ -151,6 +151,11 @@ struct vh_accel {
   uint32_t *cellscratch;       // 3 x (maxocc + 1) scan scratch of the cell classification
   int sorted_once;             // host flag: the lists of a sort exist
   int exact_field;             // VH_OPT_EXACT_FIELD: the reference's FMA order
+  // the projectile kept on the device (vh_projectile_set)
+  double *dproj;               // 16 doubles, see k_projectile_step
+  int has_dproj;
+  double proj_charge, proj_scale, proj_ball_radius;
+  int proj_soft_mode;
 };
 
 // ---------------------------------------------------------------------------
@@ -210,6 +215,7 @@ int vh_spline_coefficients_impl(vh_mesh *m, const float *src, float *dst);
 int vh_effective_potential_impl(vh_mesh *m, double nions, double radius);
 int vh_total_charge_impl(vh_mesh *m, const float *rho, double *dq);
 int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, const double *proj,
-                   int soft_mode, double ball_radius);
+                   int soft_mode, double ball_radius, const double *dproj);
+int vh_projectile_step_impl(vh_accel *a, double dt);
 int vh_verlet_impl(vh_accel *a, double dt, double mass, double rcmax);
 int vh_energy_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, double enclosed);

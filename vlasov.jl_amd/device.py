@@ -209,10 +209,14 @@ class DeviceAccelerator:
         _lib.call("vh_columns", self.handle, cols, C.byref(nout))
         return cols.reshape(self.npart, 3), int(nout.value)
 
-    def deposit_smoothed(self, mesh: DeviceMesh, charge: float) -> int:
-        """deposit_smoothed!(rho, acc, mesh, sm, positions; charge) -> nout."""
+    def deposit_smoothed(self, mesh: DeviceMesh, charge: float, count: bool = True):
+        """deposit_smoothed!(rho, acc, mesh, sm, positions; charge) -> nout
+        (None without `count`: then nothing waits for the device)."""
         if mesh.n != self.n:
             raise ValueError("rho must cover the whole collocation grid")
+        if not count:
+            _lib.call("vh_deposit_smoothed", self.handle, mesh.handle, float(charge), None)
+            return None
         nout = C.c_int32()
         _lib.call("vh_deposit_smoothed", self.handle, mesh.handle, float(charge), C.byref(nout))
         return int(nout.value)
@@ -222,9 +226,11 @@ class DeviceAccelerator:
         _lib.call("vh_deposit_cic", self.handle, mesh.handle, float(charge))
 
     def compute_forces(self, fine: DeviceMesh, coarse: DeviceMesh, weight: float,
-                       projectile: Projectile | None = None, mode: str = "device"):
+                       projectile: Projectile | None = None, mode: str = "device",
+                       outputs: bool = True):
         """forces!(cloud, acc, ...; projectile, packed=true): returns
-        (outcount, red[4]) with red the projectile reduction."""
+        (outcount, red[4]) with red the projectile reduction; (None, None)
+        without `outputs` (then nothing waits for the device)."""
         if mode not in SOFTENING_MODES:
             raise ValueError(f"unknown softening mode {mode!r}")
         if fine.n != self.n:
@@ -238,14 +244,46 @@ class DeviceAccelerator:
             radius = projectile.softening.radius if isinstance(projectile.softening, BallSoftening) else 0.0
         _lib.call("vh_forces", self.handle, fine.handle, coarse.handle, float(weight),
                   None if pp is None else pp.ctypes.data, SOFTENING_MODES[mode], float(radius),
-                  red.ctypes.data, C.byref(nout))
-        return int(nout.value), red
+                  red.ctypes.data if outputs else None, C.byref(nout) if outputs else None)
+        return (int(nout.value), red) if outputs else (None, None)
 
-    def verlet(self, dt: float, mass: float, rcmax: float = float("inf")):
-        """step!(cloud, dt, acc; rcmax) -> (kinetic, escaped, angular)."""
+    def verlet(self, dt: float, mass: float, rcmax: float = float("inf"), diagnostics: bool = True):
+        """step!(cloud, dt, acc; rcmax) -> (kinetic, escaped, angular); None
+        without `diagnostics` (then nothing waits for the device)."""
+        if not diagnostics:
+            _lib.call("vh_verlet", self.handle, float(dt), float(mass), float(rcmax), None)
+            return None
         d = np.zeros(5)
         _lib.call("vh_verlet", self.handle, float(dt), float(mass), float(rcmax), d.ctypes.data)
         return {"kinetic": d[0], "escaped": d[1], "angular": (d[2], d[3], d[4])}
+
+    # --- the projectile kept on the device -----------------------------------
+    def set_projectile(self, proj: Projectile, mode: str, jel: Jellium):
+        """Places the projectile's state on the device; from then on
+        forces_resident and projectile_step integrate it there."""
+        if mode not in SOFTENING_MODES:
+            raise ValueError(f"unknown softening mode {mode!r}")
+        st = np.array([*proj.position, *proj.previous, *proj.velocity], dtype=np.float64)
+        radius = proj.softening.radius if isinstance(proj.softening, BallSoftening) else 0.0
+        _lib.call("vh_projectile_set", self.handle, st, float(proj.mass), float(proj.charge),
+                  float(proj.scale), SOFTENING_MODES[mode], float(radius), float(jel.nions),
+                  float(jel.radius))
+
+    def projectile_state(self) -> np.ndarray:
+        """(position, previous, velocity) of the device projectile, 9 values."""
+        st = np.zeros(9)
+        _lib.call("vh_projectile_get", self.handle, st)
+        return st
+
+    def forces_resident(self, fine: DeviceMesh, coarse: DeviceMesh, weight: float):
+        """forces! with the device projectile; nothing waits for the device."""
+        if fine.n != self.n:
+            raise ValueError(f"accelerator sized for n={self.n}")
+        _lib.call("vh_forces_resident", self.handle, fine.handle, coarse.handle, float(weight))
+
+    def projectile_step(self, dt: float):
+        """projectile_forces! and step!(proj) on the device."""
+        _lib.call("vh_projectile_step", self.handle, float(dt))
 
     def interaction_energy(self, fine: DeviceMesh, coarse: DeviceMesh, weight: float,
                            enclosed: float = 0.0) -> float:

@@ -74,9 +74,12 @@ class Simulation:
 
     def __init__(self, params: SimulationParameters, profile, *, projectile: Projectile | None = None,
                  seed: int = -1, softening_mode: str = "device", consistent_startup: bool = False,
-                 device: int = 0, tables=None, positions=None, momenta=None):
+                 device: int = 0, tables=None, positions=None, momenta=None,
+                 resident_projectile: bool = True):
         self.params = params
-        self.projectile = projectile
+        self._projectile = projectile
+        self._proj_on_device = False
+        self._proj_stale = False
         self.softening_mode = softening_mode
         if tables is None:
             fine, coarse = build_meshes(params)
@@ -98,7 +101,24 @@ class Simulation:
         self.outcount = 0
         self.red = np.zeros(4)
         self.step_count = 0
+        self.diagnostics = None
         self._prime_leapfrog(positions, momenta, consistent_startup)
+        if projectile is not None and resident_projectile:
+            # from here the projectile is integrated on the device, and a step
+            # waits for nothing on the host
+            self.acc.set_projectile(projectile, softening_mode, self.jellium)
+            self._proj_on_device = True
+
+    @property
+    def projectile(self) -> Projectile | None:
+        """The projectile, brought up to date from the device when it lives
+        there (one synchronisation when it has moved since the last read)."""
+        p = self._projectile
+        if p is not None and self._proj_on_device and self._proj_stale:
+            st = self.acc.projectile_state()
+            p.position, p.previous, p.velocity = tuple(st[0:3]), tuple(st[3:6]), tuple(st[6:9])
+            self._proj_stale = False
+        return p
 
     # --- construction ------------------------------------------------------------
     def _prime_leapfrog(self, positions, momenta, consistent):
@@ -120,23 +140,36 @@ class Simulation:
         self.acc.upload(rec)
 
     # --- the step ------------------------------------------------------------------
-    def update_forces(self, advance: bool = True, energy: bool = False):
+    def update_forces(self, advance: bool = True, energy: bool = False, counts: bool = False):
         """_update_forces_resident! (src/simulation.jl:472-531).  With `energy`,
         returns the Hartree energy (half the interaction energy on the
         coefficients of the Hartree potential alone, before the mean field is
-        added to them); otherwise None."""
+        added to them); otherwise None.  With `counts`, records nout, outcount
+        and the projectile reduction (which waits for the device); otherwise,
+        with the projectile on the device, nothing waits."""
         w = self.weight
-        self.nout = self.acc.deposit_smoothed(self.dmf, w)
+        nout = self.acc.deposit_smoothed(self.dmf, w, count=counts)
+        if counts:
+            self.nout = nout
         self.acc.deposit_cic(self.dmc, w)
         poisson_nested(self.dmf, self.dmc)        # leaves csol_c = S^-1 phi_c
         self.dmf.spline_coefficients()
         hartree = self.acc.interaction_energy(self.dmf, self.dmc, w) / 2 if energy else None
         self.dmf.effective_potential(self.jellium)
         self.dmc.effective_potential(self.jellium)
-        self.outcount, self.red = self.acc.compute_forces(self.dmf, self.dmc, w, self.projectile,
-                                                          self.softening_mode)
-        if advance and self.projectile is not None:
-            self.advance_projectile()
+        if self._proj_on_device:
+            self.acc.forces_resident(self.dmf, self.dmc, w)
+            if advance:
+                self.acc.projectile_step(self.params.dt)
+                self._proj_stale = True
+        else:
+            host_outputs = counts or self._projectile is not None
+            outcount, red = self.acc.compute_forces(self.dmf, self.dmc, w, self._projectile,
+                                                    self.softening_mode, outputs=host_outputs)
+            if host_outputs:
+                self.outcount, self.red = outcount, red
+            if advance and self._projectile is not None:
+                self.advance_projectile()
         return hartree
 
     def projectile_forces(self):
@@ -157,7 +190,8 @@ class Simulation:
         step's total potential at the pushed positions (the host path's
         order, src/simulation.jl:309-351).  Without, returns None."""
         hartree = self.update_forces(advance=True, energy=energy)
-        self.diagnostics = self.acc.verlet(self.params.dt, self.mass, self.params.rcmax)
+        self.diagnostics = self.acc.verlet(self.params.dt, self.mass, self.params.rcmax,
+                                           diagnostics=energy)
         self.step_count += 1
         if not energy:
             return None
