@@ -29,6 +29,11 @@ ROOT = Path(__file__).resolve().parent
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FIELD_FLOP_PER_PARTICLE = 4900.0      # 2300 FMA + 300 MUL of contract_spline_10
+# HBM bytes per k_field_mfma launch at 8e6 particles on 130^3, from two rocprofv3
+# counter passes over `bench.py --steps 20 --warmup 5` (FETCH_SIZE 54 532 KiB x 2.000
+# + WRITE_SIZE 102 888 KiB x 1.000, the factors that make a 1 GiB device copy read
+# and write 1 GiB); recorded with the command in profiles/hbm_traffic_xenon_8M_130.md
+FIELD_TRAFFIC_BYTES_8M = (54532 * 2.0 + 102888 * 1.0) * 1024
 METRIC = "ms/step & particle-steps/s @8M,130^3; capture-q err vs 80M plain run"
 
 
@@ -160,7 +165,7 @@ def main():
     ap.add_argument("--drift-steps", type=int, default=700,
                     help="steps of the isolated-cluster energy-conservation run (0: skip)")
     ap.add_argument("--cpu-particles", type=int, default=4_000_000)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB, five times "
@@ -201,10 +206,12 @@ def main():
                                f"pseudo-particles, nfine={args.nfine} (130^3 fine + coarse), "
                                f"dt=0.5, steps {args.warmup + 1}-{args.warmup + args.steps} of 700",
                    "particles": args.particles, "grid": 2 * args.nfine + 2, "softening": args.mode},
-        "roofline": {"bound": "mfma", "engine": "VALU FP32 fma (packed FP32 disabled)",
-                     "kernel": "k_field",
+        "roofline": {"bound": "mfma",
+                     "engine": "FP32 MFMA 16x16x4 (table contraction) + VALU FP32 (second stage)",
+                     "kernel": "k_field_mfma",
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": FIELD_TRAFFIC_BYTES_8M if (args.particles, args.nfine) == (8_000_000, 64) else None,
                      "per_launch_flop": inside * FIELD_FLOP_PER_PARTICLE,
                      "avg_launch_ms": field_ms},
         "stages_ms": stages,
