@@ -283,7 +283,7 @@ extern "C" int vh_accel_destroy(vh_accel *a) {
                   a->keys[0], a->keys[1], a->vals[0], a->vals[1], a->counts, a->scan_tmp,
                   a->occupied, a->bounds, a->cellmap, a->fitlist, a->nonfit, a->nonfit_prefix,
                   a->outlist, a->counters, a->flags, a->tiles, a->dpart, a->dscal, a->cellscratch,
-                  a->dproj};
+                  a->dproj, a->units, a->unitstart};
   for (void *p : ptrs) vh_free(p);
   delete a;
   return VH_OK;
@@ -372,7 +372,10 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
   MK(vh_alloc((void **)&a->scan_tmp, (size_t)a->scan_tmp_len * 4));
   int64_t maxocc = a->ncell < npart ? a->ncell : npart;
   a->maxocc = maxocc;
-  MK(vh_alloc((void **)&a->cellscratch, sizeof(uint32_t) * 3 * (maxocc + 2)));
+  a->maxunits = maxocc + (npart + VH_UNIT - 1) / VH_UNIT;
+  MK(vh_alloc((void **)&a->cellscratch, sizeof(uint32_t) * 4 * (maxocc + 2)));
+  MK(vh_alloc((void **)&a->units, sizeof(vh_unit) * (size_t)a->maxunits));
+  MK(vh_alloc((void **)&a->unitstart, sizeof(int32_t) * (maxocc + 2)));
   MK(vh_alloc((void **)&a->occupied, sizeof(int32_t) * (maxocc + 1)));
   MK(vh_alloc((void **)&a->bounds, sizeof(int32_t) * (maxocc + 2)));
   MK(vh_alloc((void **)&a->cellmap, sizeof(int32_t) * a->ncell));
@@ -382,7 +385,7 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
   MK(vh_alloc((void **)&a->outlist, np * 4));
   MK(vh_alloc((void **)&a->counters, sizeof(int32_t) * 64));
   MK(vh_alloc((void **)&a->flags, (np + 1) * 4));
-  MK(vh_alloc((void **)&a->tiles, sizeof(float) * 512 * (size_t)maxocc));
+  MK(vh_alloc((void **)&a->tiles, sizeof(float) * 512 * (size_t)a->maxunits));
   a->dpart_len = 1 << 16;
   MK(vh_alloc((void **)&a->dpart, sizeof(double) * 8 * a->dpart_len));
   MK(vh_alloc((void **)&a->dscal, sizeof(double) * 64));

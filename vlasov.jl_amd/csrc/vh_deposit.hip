@@ -2,21 +2,20 @@
 // cloud-in-cell deposit on the stretched coarse grid.
 //
 // Fine (src/kernels.jl:601-726, src/accelerator.jl:531-565).  One wave per
-// occupied cell whose stencil fits; 32 particles at a time have their 24
-// tabulated node values staged in LDS; lane (jj, kk) owns the x-column of 8
-// points and accumulates w = gy[jj] gz[kk] into 8 registers with the
-// reference's fma order.  The reference then commits each cell with float
-// atomics.  Here each cell writes its 512 sums to a tile, and a second kernel
-// builds every grid point from the (at most 64) tiles that cover it, in a
-// fixed cell order: no float atomics, bit-reproducible, and the total charge
-// (FP64) is reduced in the same pass.
+// work unit (at most VH_UNIT particles of one occupied cell whose stencil
+// fits): the unit's 8^3 sums are rank-4 updates on the matrix cores, written
+// to a tile of its own.  The reference commits each cell with float atomics;
+// here a second kernel builds every grid point from the tiles that cover it,
+// units of a cell in slot order and cells in a fixed order: no float
+// atomics, bit-reproducible, and the total charge (FP64) is reduced in the
+// same pass.
 //
 // Coarse (src/kernels.jl:1207-1273, src/devicemesh.jl:93-106).  One wave per
-// occupied fine cell; each lane owns one point of a 4^3 window of coarse
-// collocation points anchored at the cell's lower corner and sums its share
-// of the cell's particles in order.  Windows are committed, and particles
-// that leave their window are deposited, through 64-bit fixed-point atomics,
-// which are exact and therefore order-independent.
+// work unit of an occupied fine cell: the unit's particles are summed into a
+// 4^3 window of coarse collocation points on the matrix cores; windows are
+// committed, and particles that leave their window are deposited, through
+// 64-bit fixed-point atomics, which are exact and therefore
+// order-independent.
 #include "vh_internal.h"
 
 #define DEP_STAGE 32
@@ -37,19 +36,22 @@ typedef float vh_f4d __attribute__((ext_vector_type(4)));
 // a time are staged in LDS; missing particles of a group contribute zeros.
 __global__ void __launch_bounds__(64)
     k_dep_tiles(const int32_t *__restrict__ P, int64_t npad, const float *__restrict__ nodes,
-                const int32_t *__restrict__ fitlist, const int32_t *__restrict__ bounds,
-                const int32_t *__restrict__ cnt, float spacing, float nbdt, int ncol,
-                float *__restrict__ tiles) {
+                const vh_unit *__restrict__ units, const int32_t *__restrict__ cnt, int nk, int n,
+                float spacing, float nbdt, int ncol, float *__restrict__ tiles) {
   __shared__ float vals[24 * DEP_STAGE];
   const float *D = (const float *)P;
   const int l = threadIdx.x;
   const int row = l & 15, kq = l >> 4;       // A: row kk, particle kq of the group
   const int ii = l & 7, jh = (l >> 3) & 1;   // B: column l & 15 -> (ii, jj = 2 t + jh)
   const float half = spacing * 0.5f;
-  const int nfit = cnt[CNT_NFIT];
-  for (int f = blockIdx.x; f < nfit; f += gridDim.x) {
-    const int g = fitlist[f];
-    const int lo = bounds[g], hi = bounds[g + 1];
+  const int nunits = cnt[CNT_NUNITS];
+  for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const vh_unit un = units[u];
+    // units of cells whose 10^3 stencil leaves the grid deposit nothing
+    const int kx = un.key % nk + 1, ky = (un.key / nk) % nk + 1, kz = un.key / (nk * nk) + 1;
+    const int bx = 2 * kx - 5, by = 2 * ky - 5, bz = 2 * kz - 5;
+    if (!(bx >= 1 && by >= 1 && bz >= 1 && bx + 9 <= n && by + 9 <= n && bz + 9 <= n)) continue;
+    const int lo = un.lo, hi = un.hi;
     vh_f4d acc[4];
 #pragma unroll
     for (int tt = 0; tt < 4; ++tt) acc[tt] = (vh_f4d){0.f, 0.f, 0.f, 0.f};
@@ -85,7 +87,7 @@ __global__ void __launch_bounds__(64)
     }
     // lane l holds rows 4 (l >> 4) + r, column l & 15 of each tile; rows >= 8 are padding
     if ((l >> 4) < 2) {
-      float *out = tiles + (int64_t)g * 512;
+      float *out = tiles + (int64_t)u * 512;
 #pragma unroll
       for (int tt = 0; tt < 4; ++tt)
 #pragma unroll
@@ -98,12 +100,34 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// A cell cut into several units: its tiles summed in slot order into the
+// tile of its first unit.  One block per fitting cell, a float2 per thread.
+__global__ void __launch_bounds__(256)
+    k_dep_merge(float *__restrict__ tiles, const int32_t *__restrict__ fitlist,
+                const int32_t *__restrict__ unitstart, const int32_t *__restrict__ cnt) {
+  const int nfit = cnt[CNT_NFIT];
+  for (int f = blockIdx.x; f < nfit; f += gridDim.x) {
+    const int g = fitlist[f];
+    const int u0 = unitstart[g], u1 = unitstart[g + 1];
+    if (u1 - u0 < 2) continue;
+    float2 *t0 = (float2 *)(tiles + (int64_t)u0 * 512) + threadIdx.x;
+    float2 v = *t0;
+    for (int u = u0 + 1; u < u1; ++u) {
+      const float2 w = ((const float2 *)(tiles + (int64_t)u * 512))[threadIdx.x];
+      v.x += w.x;
+      v.y += w.y;
+    }
+    *t0 = v;
+  }
+}
+
 // every grid point from the tiles that cover it; thread = 2x2x2 block of
 // points (2m-1, 2m) per axis, whose covering cells are m-2 .. m+1 (0-based).
 // Also the FP64 charge sum(rho m0x m0y m0z) per block.
 __global__ void __launch_bounds__(256)
-    k_dep_combine(const float *__restrict__ tiles, const int32_t *__restrict__ cellmap, int nk,
-                  int n, const double *__restrict__ m0, float *__restrict__ rho,
+    k_dep_combine(const float *__restrict__ tiles, const int32_t *__restrict__ cellmap,
+                  const int32_t *__restrict__ unitstart, int nk, int n,
+                  const double *__restrict__ m0, float *__restrict__ rho,
                   double *__restrict__ part) {
   __shared__ double lds[8];
   const int nb = nk + 1;
@@ -126,7 +150,8 @@ __global__ void __launch_bounds__(256)
           const int g = cellmap[cx + nk * (cy + nk * cz)];
           if (g < 0) continue;
           const int lx = 2 * (mx - cx) + 2;
-          const float *tl = tiles + (int64_t)g * 512;
+          // the cell's sums, merged into the tile of its first unit
+          const float *tl = tiles + (int64_t)unitstart[g] * 512;
 #pragma unroll
           for (int dz = 0; dz < 2; ++dz)
 #pragma unroll
@@ -181,17 +206,20 @@ __global__ void k_dep_scale(float *__restrict__ rho, int64_t len, const double *
 int vh_deposit_fine_impl(vh_accel *a, vh_mesh *fine, double charge) {
   vh_context *c = a->ctx;
   hipStream_t st = c->stream;
-  // one wave per fitting cell up to 65536 cells: the dispatcher balances
-  // cells of very different weights
-  const int grid = (int)(a->maxocc < 65536 ? a->maxocc : 65536);
+  // one wave per work unit up to 65536 units, then a stride over units
+  const int grid = (int)(a->maxunits < 65536 ? a->maxunits : 65536);
   int tok = vh_prof_begin(c, VH_ST_DEPOSIT_KERNEL);
   hipLaunchKernelGGL(k_dep_tiles, dim3(grid), dim3(64), 0, st, a->parts[a->cur], a->npad, a->nodes,
-                     a->fitlist, a->bounds, a->counters, a->spacing, (float)a->nbdt, a->ncol,
+                     a->units, a->counters, a->nk, a->n, a->spacing, (float)a->nbdt, a->ncol,
                      a->tiles);
   vh_prof_end(c, tok);
   VH_LAUNCH_CHECK();
+  const int mgrid = (int)(a->maxocc < 16384 ? a->maxocc : 16384);
+  hipLaunchKernelGGL(k_dep_merge, dim3(mgrid), dim3(256), 0, st, a->tiles, a->fitlist, a->unitstart,
+                     a->counters);
+  VH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_dep_combine, dim3(COMBINE_BLOCKS), dim3(256), 0, st, a->tiles, a->cellmap,
-                     a->nk, a->n, fine->dual, fine->rho, a->dpart);
+                     a->unitstart, a->nk, a->n, fine->dual, fine->rho, a->dpart);
   VH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, a->dpart, COMBINE_BLOCKS, 0, 1,
                      a->dscal + 16);
@@ -231,26 +259,55 @@ __device__ __forceinline__ void fx_add(unsigned long long *acc, int64_t idx, flo
   if (v != 0.f) atomicAdd(acc + idx, fx_of(v, scale));
 }
 
-#define CIC_THREADS 256
+// One wave per work unit (at most VH_UNIT particles of one occupied fine
+// cell), CIC_WAVES waves per block sharing the coarse axis's collocation
+// points and locate table in LDS.  The unit's particles fall into a 4^3 window of coarse points
+// anchored at the coarse interval of the cell's lower corner; the window's
+// sums are a rank-P update
+//   W[x + 4 y][z] = sum_p (X_p[x] Y_p[y]) Z_p[z],
+// X_p = the particle's two linear weights at its window offsets along x (zero
+// elsewhere), run on v_mfma_f32_16x16x4_f32: A = the 16 products X Y of four
+// particles, B = their z weights, each particle in its own four-column group
+// (B[k][4 g + z] = Z_k[z] when g = k), so that a product takes four particles
+// and the four groups are four partial sums.  Every 64 particles the FP32
+// sums are converted to 64-bit fixed point and added in registers; at the
+// end of the unit the four groups are added (integers: exact).  The window
+// of a cell's only unit is committed with global 64-bit atomics; a cell cut
+// into several units writes one window per unit, and k_cic_flush adds them
+// and commits once per cell, which keeps the atomics on the hot points of
+// the cluster's core few.  Particles whose window offset falls outside the
+// window go straight to the global accumulator.  Integer sums do not depend
+// on order, so the result is reproducible.
+#define CIC_STRIDE 21   // 16 products + 4 z weights, odd row stride
+#define CIC_WAVES 4     // waves per block, each on its own units; the block shares the tables
 
-// One workgroup per occupied fine cell.  Each thread takes particles of the
-// cell in turn and adds its eight corner weights, as 64-bit fixed point, into
-// a 4^3 LDS window of coarse points anchored at the cell's lower corner;
-// particles whose corners leave the window go straight to the global
-// accumulator.  Integer sums are exact, so the result does not depend on the
-// order the atomics land in.
-__global__ void __launch_bounds__(CIC_THREADS)
-    k_cic_tiles(const int32_t *__restrict__ P, int64_t npad, const int32_t *__restrict__ occupied,
-                const int32_t *__restrict__ bounds, const int32_t *__restrict__ cnt, int nkf,
-                float x0f, float hf, const float *__restrict__ gt, int n, float x0t, float iwt,
-                const int32_t *__restrict__ cells, int nbin, double scale,
-                unsigned long long *__restrict__ acc) {
-  __shared__ unsigned long long win[64];
+// dynamic LDS of k_cic_tiles: per wave 64 staging rows, then the collocation
+// points and the locate table of the coarse axis
+static inline size_t cic_lds_bytes(int n, int nbin) {
+  return sizeof(float) * ((size_t)CIC_WAVES * 64 * CIC_STRIDE + n + nbin);
+}
+
+__global__ void __launch_bounds__(64 * CIC_WAVES)
+    k_cic_tiles(const int32_t *__restrict__ P, int64_t npad, const vh_unit *__restrict__ units,
+                const int32_t *__restrict__ unitstart, const int32_t *__restrict__ cnt, int nkf,
+                float x0f, float hf, const float *__restrict__ gt_g, int n, float x0t, float iwt,
+                const int32_t *__restrict__ cells_g, int nbin, double scale,
+                unsigned long long *__restrict__ acc, long long *__restrict__ wins) {
+  extern __shared__ float lds_cic[];
+  float *gt = lds_cic + CIC_WAVES * 64 * CIC_STRIDE;
+  int32_t *cells = (int32_t *)(gt + n);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) gt[i] = gt_g[i];
+  for (int i = threadIdx.x; i < nbin; i += blockDim.x) cells[i] = cells_g[i];
+  __syncthreads();
+  const int wv = threadIdx.x >> 6;
+  float *st = lds_cic + wv * 64 * CIC_STRIDE;
   const float *D = (const float *)P;
-  const int t = threadIdx.x;
-  const int nocc = cnt[CNT_NOCC];
-  for (int g = blockIdx.x; g < nocc; g += gridDim.x) {
-    const int key = occupied[g];
+  const int l = threadIdx.x & 63;
+  const int kq = l >> 4, xy = l & 15, grp = (l & 15) >> 2, zc = l & 3;
+  const int nunits = cnt[CNT_NUNITS];
+  for (int u = blockIdx.x * CIC_WAVES + wv; u < nunits; u += gridDim.x * CIC_WAVES) {
+    const vh_unit un = units[u];
+    const int key = un.key;
     const int kc[3] = {key % nkf + 1, (key / nkf) % nkf + 1, key / (nkf * nkf) + 1};
     // window anchor: coarse interval of the cell's lower corner, per axis
     int anc[3];
@@ -261,58 +318,125 @@ __global__ void __launch_bounds__(CIC_THREADS)
       anc[d] = locate_cell(gt, n, x0t, iwt, cells, nbin, xl, w);  // 0: every particle escapes
     }
     const bool anchored = anc[0] > 0 && anc[1] > 0 && anc[2] > 0;
-    if (t < 64) win[t] = 0ull;
-    __syncthreads();
-    const int lo = bounds[g], hi = bounds[g + 1];
-    for (int s = lo + t; s < hi; s += CIC_THREADS) {
-      int ix[3];
-      float ax[3];
-      bool inside = true;
+    long long sum[4] = {0, 0, 0, 0};
+    for (int c0 = un.lo; c0 < un.hi; c0 += 64) {
+      const int s = c0 + l;
+      float X[4] = {0.f, 0.f, 0.f, 0.f}, Y[4] = {0.f, 0.f, 0.f, 0.f}, Z[4] = {0.f, 0.f, 0.f, 0.f};
+      if (s < un.hi) {
+        int ix[3];
+        float ax[3];
+        bool inside = true;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const float p = x0f + (float)(P[(PS_KX + d) * npad + s] - 1) * hf + D[(PS_DX + d) * npad + s];
-        ix[d] = locate_cell(gt, n, x0t, iwt, cells, nbin, p, ax[d]);
-        inside = inside && ix[d] > 0;
+        for (int d = 0; d < 3; ++d) {
+          const float p = x0f + (float)(P[(PS_KX + d) * npad + s] - 1) * hf + D[(PS_DX + d) * npad + s];
+          ix[d] = locate_cell(gt, n, x0t, iwt, cells, nbin, p, ax[d]);
+          inside = inside && ix[d] > 0;
+        }
+        if (inside) {
+          const int q0 = ix[0] - anc[0], q1 = ix[1] - anc[1], q2 = ix[2] - anc[2];
+          const bool fits = anchored && q0 >= 0 && q0 <= 2 && q1 >= 0 && q1 <= 2 && q2 >= 0 && q2 <= 2;
+          if (fits) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              X[e] = e == q0 ? ax[0] : (e == q0 + 1 ? 1.f - ax[0] : 0.f);
+              Y[e] = e == q1 ? ax[1] : (e == q1 + 1 ? 1.f - ax[1] : 0.f);
+              Z[e] = e == q2 ? ax[2] : (e == q2 + 1 ? 1.f - ax[2] : 0.f);
+            }
+          } else {
+            const float bx = 1.f - ax[0], by = 1.f - ax[1], bz = 1.f - ax[2];
+            const int64_t i0 = ix[0] - 1, j0 = ix[1] - 1, k0 = ix[2] - 1;
+            const int64_t nn = n;
+            fx_add(acc, i0 + nn * (j0 + nn * k0), ax[0] * ax[1] * ax[2], scale);
+            fx_add(acc, i0 + nn * (j0 + nn * (k0 + 1)), ax[0] * ax[1] * bz, scale);
+            fx_add(acc, i0 + nn * ((j0 + 1) + nn * k0), ax[0] * by * ax[2], scale);
+            fx_add(acc, i0 + nn * ((j0 + 1) + nn * (k0 + 1)), ax[0] * by * bz, scale);
+            fx_add(acc, (i0 + 1) + nn * (j0 + nn * k0), bx * ax[1] * ax[2], scale);
+            fx_add(acc, (i0 + 1) + nn * (j0 + nn * (k0 + 1)), bx * ax[1] * bz, scale);
+            fx_add(acc, (i0 + 1) + nn * ((j0 + 1) + nn * k0), bx * by * ax[2], scale);
+            fx_add(acc, (i0 + 1) + nn * ((j0 + 1) + nn * (k0 + 1)), bx * by * bz, scale);
+          }
+        }
       }
-      if (!inside) continue;
-      const float bx = 1.f - ax[0], by = 1.f - ax[1], bz = 1.f - ax[2];
-      const float c[8] = {ax[0] * ax[1] * ax[2], ax[0] * ax[1] * bz, ax[0] * by * ax[2],
-                          ax[0] * by * bz,       bx * ax[1] * ax[2], bx * ax[1] * bz,
-                          bx * by * ax[2],       bx * by * bz};
-      const int q0 = ix[0] - anc[0], q1 = ix[1] - anc[1], q2 = ix[2] - anc[2];
-      const bool fits = anchored && q0 >= 0 && q0 <= 2 && q1 >= 0 && q1 <= 2 && q2 >= 0 && q2 <= 2;
-      if (fits) {
-        const int base = q0 + 4 * (q1 + 4 * q2);
-        // corner (di, dj, dk) -> window point base + di + 4 dj + 16 dk
-        atomicAdd(&win[base], fx_of(c[0], scale));
-        atomicAdd(&win[base + 16], fx_of(c[1], scale));
-        atomicAdd(&win[base + 4], fx_of(c[2], scale));
-        atomicAdd(&win[base + 20], fx_of(c[3], scale));
-        atomicAdd(&win[base + 1], fx_of(c[4], scale));
-        atomicAdd(&win[base + 17], fx_of(c[5], scale));
-        atomicAdd(&win[base + 5], fx_of(c[6], scale));
-        atomicAdd(&win[base + 21], fx_of(c[7], scale));
-      } else {
-        const int64_t i0 = ix[0] - 1, j0 = ix[1] - 1, k0 = ix[2] - 1;
-        const int64_t nn = n;
-        fx_add(acc, i0 + nn * (j0 + nn * k0), c[0], scale);
-        fx_add(acc, i0 + nn * (j0 + nn * (k0 + 1)), c[1], scale);
-        fx_add(acc, i0 + nn * ((j0 + 1) + nn * k0), c[2], scale);
-        fx_add(acc, i0 + nn * ((j0 + 1) + nn * (k0 + 1)), c[3], scale);
-        fx_add(acc, (i0 + 1) + nn * (j0 + nn * k0), c[4], scale);
-        fx_add(acc, (i0 + 1) + nn * (j0 + nn * (k0 + 1)), c[5], scale);
-        fx_add(acc, (i0 + 1) + nn * ((j0 + 1) + nn * k0), c[6], scale);
-        fx_add(acc, (i0 + 1) + nn * ((j0 + 1) + nn * (k0 + 1)), c[7], scale);
+      // the particle's row: X[x] Y[y] at x + 4 y, then Z; the wave's own rows
+      // only, so a wave-level order suffices between the writes and the reads
+      float *row = st + CIC_STRIDE * l;
+#pragma unroll
+      for (int y = 0; y < 4; ++y)
+#pragma unroll
+        for (int x = 0; x < 4; ++x) row[x + 4 * y] = X[x] * Y[y];
+#pragma unroll
+      for (int z = 0; z < 4; ++z) row[16 + z] = Z[z];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      const int np = min(64, un.hi - c0);
+      vh_f4d d = {0.f, 0.f, 0.f, 0.f};
+      for (int m = 0; 4 * m < np; ++m) {
+        const float *pr = st + CIC_STRIDE * (4 * m + kq);
+        const float av = pr[xy];
+        const float zv = pr[16 + zc];
+        const float bv = grp == kq ? zv : 0.f;
+        d = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, d, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) sum[v] += (long long)rint((double)d[v] * scale);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+    // lane l holds W[4 (l >> 4) + v][4 grp + zc]: rows x = v, y = l >> 4; add the
+    // four groups (lanes l ^ 4, l ^ 8, l ^ 12 share y and z)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      sum[v] += __shfl_xor(sum[v], 4, 64);
+      sum[v] += __shfl_xor(sum[v], 8, 64);
+    }
+    if (unitstart[un.g + 1] - unitstart[un.g] > 1) {
+      // window point x + 4 y + 16 z of this unit, for k_cic_flush
+      if (grp == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) wins[(int64_t)u * 64 + v + 4 * kq + 16 * zc] = sum[v];
+      }
+    } else if (grp == 0 && anchored) {
+      const int64_t j = anc[1] - 1 + kq, k = anc[2] - 1 + zc;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t i = anc[0] - 1 + v;
+        if (sum[v] != 0 && i < n && j < n && k < n)
+          atomicAdd(acc + i + (int64_t)n * (j + (int64_t)n * k), (unsigned long long)sum[v]);
       }
     }
-    __syncthreads();
-    if (t < 64 && anchored) {
-      const int tx = t & 3, ty = (t >> 2) & 3, tz = t >> 4;
-      const int64_t i = anc[0] - 1 + tx, j = anc[1] - 1 + ty, k = anc[2] - 1 + tz;
-      const unsigned long long v = win[t];
-      if (v != 0ull && i < n && j < n && k < n) atomicAdd(acc + i + (int64_t)n * (j + (int64_t)n * k), v);
+  }
+}
+
+// The windows of a cell cut into several units: added (exact integers) and
+// committed once.  One wave per occupied cell, a window point per lane.
+__global__ void __launch_bounds__(64)
+    k_cic_flush(const int32_t *__restrict__ occupied, const int32_t *__restrict__ unitstart,
+                const int32_t *__restrict__ cnt, int nkf, float x0f, float hf,
+                const float *__restrict__ gt, int n, float x0t, float iwt,
+                const int32_t *__restrict__ cells, int nbin, const long long *__restrict__ wins,
+                unsigned long long *__restrict__ acc) {
+  const int l = threadIdx.x;
+  const int nocc = cnt[CNT_NOCC];
+  for (int g = blockIdx.x; g < nocc; g += gridDim.x) {
+    const int u0 = unitstart[g], u1 = unitstart[g + 1];
+    if (u1 - u0 < 2) continue;
+    const int key = occupied[g];
+    const int kc[3] = {key % nkf + 1, (key / nkf) % nkf + 1, key / (nkf * nkf) + 1};
+    int anc[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      float w;
+      const float xl = x0f + ((float)(kc[d] - 1) - 0.5f) * hf;
+      anc[d] = locate_cell(gt, n, x0t, iwt, cells, nbin, xl, w);
     }
-    __syncthreads();
+    if (!(anc[0] > 0 && anc[1] > 0 && anc[2] > 0)) continue;  // nothing was windowed
+    long long sum = 0;
+    for (int u = u0; u < u1; ++u) sum += wins[(int64_t)u * 64 + l];
+    const int64_t i = anc[0] - 1 + (l & 3), j = anc[1] - 1 + ((l >> 2) & 3), k = anc[2] - 1 + (l >> 4);
+    if (sum != 0 && i < n && j < n && k < n)
+      atomicAdd(acc + i + (int64_t)n * (j + (int64_t)n * k), (unsigned long long)sum);
   }
 }
 
@@ -334,11 +458,22 @@ int vh_deposit_cic_impl(vh_accel *a, vh_mesh *m, double charge) {
   vh_context *c = a->ctx;
   hipStream_t st = c->stream;
   const double scale = ldexp(1.0, a->cic_shift);
-  const int grid = (int)(a->maxocc < 65535 ? a->maxocc : 65535);
-  hipLaunchKernelGGL(k_cic_tiles, dim3(grid), dim3(CIC_THREADS), 0, st, a->parts[a->cur], a->npad,
-                     a->occupied, a->bounds, a->counters, a->nk, (float)a->x0, (float)a->h,
+  // the fine deposit's tiles (2 KiB per unit, consumed by then) hold the
+  // 512-byte windows of units of multi-unit cells
+  long long *wins = (long long *)a->tiles;
+  const size_t lds = cic_lds_bytes(m->n, m->nbin);
+  VH_REQUIRE(lds <= 65536, "vh_deposit_cic: coarse locate table too large (%d bins)", m->nbin);
+  const int grid = (int)((a->maxunits + CIC_WAVES - 1) / CIC_WAVES < 16384
+                             ? (a->maxunits + CIC_WAVES - 1) / CIC_WAVES : 16384);
+  hipLaunchKernelGGL(k_cic_tiles, dim3(grid), dim3(64 * CIC_WAVES), lds, st, a->parts[a->cur], a->npad,
+                     a->units, a->unitstart, a->counters, a->nk, (float)a->x0, (float)a->h,
                      m->colloc, m->n, m->loc_x0, m->loc_iw, m->loc_cells, m->nbin, scale,
-                     (unsigned long long *)m->acc64);
+                     (unsigned long long *)m->acc64, wins);
+  VH_LAUNCH_CHECK();
+  const int fgrid = (int)(a->maxocc < 65536 ? a->maxocc : 65536);
+  hipLaunchKernelGGL(k_cic_flush, dim3(fgrid), dim3(64), 0, st, a->occupied, a->unitstart,
+                     a->counters, a->nk, (float)a->x0, (float)a->h, m->colloc, m->n, m->loc_x0,
+                     m->loc_iw, m->loc_cells, m->nbin, wins, (unsigned long long *)m->acc64);
   VH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_cic_convert, dim3(2048), dim3(256), 0, st, (unsigned long long *)m->acc64,
                      m->rho, m->duallen, m->n, 1.0 / scale, (float)charge);

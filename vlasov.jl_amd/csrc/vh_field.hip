@@ -1,7 +1,8 @@
 // Smoothed field at the pseudo-particles with the projectile fused in, the
 // out-of-stencil fallback, and the packed position Verlet.
 //
-// Field (src/kernels.jl:339-470).  One wave per occupied cell, two variants:
+// Field (src/kernels.jl:339-470).  One wave per work unit (at most VH_UNIT
+// particles of one occupied cell), two variants:
 //   k_field_mfma (default): the first stage of the 10^3 contraction on the
 //     matrix cores, 16 particles per product (see the kernel);
 //   k_field (exact order): each lane takes one particle, keeps 40 of its
@@ -80,8 +81,7 @@ struct vh_proj_args {
 __global__ void __launch_bounds__(64)
     k_field(const int32_t *__restrict__ P, int64_t npad, const float *__restrict__ csol, int n,
             const float *__restrict__ ovl, const float *__restrict__ grad,
-            const int32_t *__restrict__ occupied, const int32_t *__restrict__ bounds,
-            const int32_t *__restrict__ cnt, int nk, float x0, float h, float spacing, float nbdt,
+            const vh_unit *__restrict__ units, const int32_t *__restrict__ cnt, int nk, float x0, float h, float spacing, float nbdt,
             float w, int ncol, vh_proj_args pa, float *__restrict__ force,
             double *__restrict__ part) {
   const float *D = (const float *)P;
@@ -93,17 +93,18 @@ __global__ void __launch_bounds__(64)
   }
   const float half = spacing * 0.5f;
   const float nan = __int_as_float(0x7fc00000);
-  const int nocc = cnt[CNT_NOCC];
+  const int nunits = cnt[CNT_NUNITS];
   const int64_t nn = n;
   double sax = 0.0, say = 0.0, saz = 0.0, sae = 0.0;
-  for (int g = blockIdx.x; g < nocc; g += gridDim.x) {
-    const int key = occupied[g];
+  for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const vh_unit un = units[u];
+    const int key = un.key;
     const int kx = key % nk + 1, ky = (key / nk) % nk + 1, kz = key / (nk * nk) + 1;
     const int bx = 2 * kx - 5, by = 2 * ky - 5, bz = 2 * kz - 5;  // 1-based stencil corner
     const bool ok = bx >= 1 && by >= 1 && bz >= 1 && bx + 9 <= n && by + 9 <= n && bz + 9 <= n;
     // the cell's 10^3 block of csol, read at wave-uniform addresses
     const float *tb = csol + (ok ? (bx - 1) + nn * ((by - 1) + nn * (bz - 1)) : 0);
-    const int lo = bounds[g], hi = bounds[g + 1];
+    const int lo = un.lo, hi = un.hi;
     for (int s = lo + t; s < hi; s += 64) {
       const float dx0 = D[PS_DX * npad + s], dy0 = D[PS_DY * npad + s], dz0 = D[PS_DZ * npad + s];
       float fxp = nan, fyp = nan, fzp = nan;
@@ -344,8 +345,7 @@ __device__ __forceinline__ void ft_contract(const float A[8][3], const float *ta
 
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     k_field_mfma(const int32_t *__restrict__ P, int64_t npad, const float *__restrict__ csol, int n,
-                 const float4 *__restrict__ og,
-                 const int32_t *__restrict__ occupied, const int32_t *__restrict__ bounds,
+                 const float4 *__restrict__ og, const vh_unit *__restrict__ units,
                  const int32_t *__restrict__ cnt, int nk, float x0, float h, float spacing,
                  float nbdt, float w, int ncol, vh_proj_args pa, float *__restrict__ force,
                  double *__restrict__ part) {
@@ -362,15 +362,16 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
   const int j16 = l & 15, grp = l >> 4;
   const float half = spacing * 0.5f;
   const float nan = __int_as_float(0x7fc00000);
-  const int nocc = cnt[CNT_NOCC];
+  const int nunits = cnt[CNT_NUNITS];
   const int64_t nn = n;
   double sax = 0.0, say = 0.0, saz = 0.0, sae = 0.0;
-  for (int g = blockIdx.x; g < nocc; g += gridDim.x) {
-    const int key = occupied[g];
+  for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const vh_unit un = units[u];
+    const int key = un.key;
     const int kx = key % nk + 1, ky = (key / nk) % nk + 1, kz = key / (nk * nk) + 1;
     const int bx = 2 * kx - 5, by = 2 * ky - 5, bz = 2 * kz - 5;
     const bool ok = bx >= 1 && by >= 1 && bz >= 1 && bx + 9 <= n && by + 9 <= n && bz + 9 <= n;
-    const int lo = bounds[g], hi = bounds[g + 1];
+    const int lo = un.lo, hi = un.hi;
     // A operands of the cell: row j16 of tile t, k = ii = 4 s + grp
     float A[8][3];
     if (ok) {
@@ -769,22 +770,22 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
     pa.sigma = (float)proj[4];
     pa.radius = (float)(ball_radius > 0 ? ball_radius : proj[4]);
   }
-  // One block per occupied cell up to 65536 cells (the dispatcher then
-  // balances cells of very different weights); beyond, blocks stride over
-  // cells.  Four FP64 partials per block fit the 8 x dpart_len buffer.
-  int grid = (int)(a->maxocc < 65536 ? a->maxocc : 65536);
+  // One block per work unit up to 65536 units (the dispatcher then balances
+  // units of different weights); beyond, blocks stride over units.  Four
+  // FP64 partials per block fit the 8 x dpart_len buffer.
+  int grid = (int)(a->maxunits < 65536 ? a->maxunits : 65536);
   if (4 * (int64_t)grid > 8 * a->dpart_len) grid = (int)(2 * a->dpart_len);
   int tok = vh_prof_begin(c, VH_ST_FIELD_KERNEL);
   if (a->exact_field)
     hipLaunchKernelGGL(k_field, dim3(grid), dim3(64), 0, st, a->parts[a->cur], a->npad,
-                       fine->csol, a->n, a->ovl, a->grad, a->occupied, a->bounds, a->counters,
+                       fine->csol, a->n, a->ovl, a->grad, a->units, a->counters,
                        a->nk, (float)a->x0, (float)a->h, a->spacing, (float)a->nbdt,
                        (float)weight, a->ncol, pa, a->force, a->dpart);
   else
     hipLaunchKernelGGL(k_field_mfma, dim3(grid), dim3(64), 0, st, a->parts[a->cur], a->npad,
-                       fine->csol, a->n, (const float4 *)a->og, a->occupied, a->bounds,
-                       a->counters, a->nk, (float)a->x0, (float)a->h, a->spacing, (float)a->nbdt,
-                       (float)weight, a->ncol, pa, a->force, a->dpart);
+                       fine->csol, a->n, (const float4 *)a->og, a->units, a->counters, a->nk,
+                       (float)a->x0, (float)a->h, a->spacing, (float)a->nbdt, (float)weight,
+                       a->ncol, pa, a->force, a->dpart);
   vh_prof_end(c, tok);
   VH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_reduce_cols, dim3(4), dim3(1024), 0, st, a->dpart, grid, 4, a->dscal + 8);

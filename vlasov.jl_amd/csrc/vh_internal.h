@@ -106,7 +106,19 @@ enum {
   CNT_NFIT,       // occupied cells whose stencil fits
   CNT_NNONFIT,    // occupied cells that do not fit
   CNT_SORTED,     // 1 once a sort has run
+  CNT_NUNITS,     // work units (see VH_UNIT)
   CNT_COUNT
+};
+
+// Work units of the per-cell kernels: an occupied cell's particles in runs of
+// at most VH_UNIT slots.  A cell of the cluster's core holds thousands of
+// particles; as units they spread over many waves instead of one.
+#define VH_UNIT 256
+struct vh_unit {
+  int32_t key;   // 0-based cell key
+  int32_t lo;    // first slot
+  int32_t hi;    // one past the last slot
+  int32_t g;     // occupied index of the cell
 };
 
 struct vh_accel {
@@ -134,6 +146,9 @@ struct vh_accel {
   int64_t scan_tmp_len;
   int32_t *occupied;           // 0-based keys, up to ncell
   int32_t *bounds;             // n_occ + 1
+  vh_unit *units;              // CNT_NUNITS work units, in slot order
+  int32_t *unitstart;          // n_occ + 1: first unit of each occupied cell
+  int64_t maxunits;            // maxocc + ceil(npart / VH_UNIT)
   int32_t *cellmap;            // ncell -> fitting occupied index or -1
   int32_t *fitlist;            // occupied index of fitting cells
   int32_t *nonfit;             // occupied index of non-fitting cells
@@ -141,14 +156,14 @@ struct vh_accel {
   int32_t *outlist;            // 0-based slots of out-of-stencil particles
   int32_t *counters;           // CNT_*
   uint32_t *flags;             // npad scratch (run starts)
-  float *tiles;                // 512 per fitting cell (fine deposit)
+  float *tiles;                // 512 per work unit (fine deposit)
   double *dpart;               // per-block FP64 partials of the reductions
   int64_t dpart_len;
   double *dscal;               // [0..4] verlet diag, [8..11] field reduction, [16] q, [17] enclosed
   int cic_shift;               // fixed-point scale 2^shift of the coarse deposit
   float lo, hi;                // E(knots[2] + h/2), E(knots[end-1] - h/2): deposit bounds
   int64_t maxocc;              // min(ncell, npart)
-  uint32_t *cellscratch;       // 3 x (maxocc + 1) scan scratch of the cell classification
+  uint32_t *cellscratch;       // 4 x (maxocc + 2) scan scratch of the cell classification
   int sorted_once;             // host flag: the lists of a sort exist
   int exact_field;             // VH_OPT_EXACT_FIELD: the reference's FMA order
   // the projectile kept on the device (vh_projectile_set)

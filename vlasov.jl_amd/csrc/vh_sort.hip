@@ -270,11 +270,13 @@ __global__ void k_run_write(const uint32_t *__restrict__ keys, const uint32_t *_
 
 // per occupied cell: does the 10^3 stencil fit (src/kernels.jl:1047-1052)?
 // Fitting cells enter the deposit map; the particle counts of the others are
-// compacted into the out-of-stencil list.  Entries past n_occ are zero so
-// that the exclusive scans over cap + 1 entries end with the totals.
+// compacted into the out-of-stencil list; every cell is cut into work units
+// of at most VH_UNIT slots.  Entries past n_occ are zero so that the
+// exclusive scans over cap + 1 entries end with the totals.
 __global__ void k_cell_classify(const int32_t *__restrict__ occupied, const int32_t *__restrict__ bounds,
                                 const int32_t *__restrict__ cnt, int nk, int n, int32_t *cellmap,
-                                uint32_t *fitidx, uint32_t *nfidx, uint32_t *nfpre, int64_t len) {
+                                uint32_t *fitidx, uint32_t *nfidx, uint32_t *nfpre, uint32_t *nunit,
+                                int64_t len) {
   int nocc = cnt[CNT_NOCC];
   for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < len;
        g += (int64_t)gridDim.x * blockDim.x) {
@@ -282,23 +284,29 @@ __global__ void k_cell_classify(const int32_t *__restrict__ occupied, const int3
       fitidx[g] = 0;
       nfidx[g] = 0;
       nfpre[g] = 0;
+      nunit[g] = 0;
       continue;
     }
     int c0 = occupied[g];
     int kx = c0 % nk + 1, ky = (c0 / nk) % nk + 1, kz = c0 / (nk * nk) + 1;
     int bx = 2 * kx - 5, by = 2 * ky - 5, bz = 2 * kz - 5;
     bool ok = bx >= 1 && by >= 1 && bz >= 1 && bx + 9 <= n && by + 9 <= n && bz + 9 <= n;
+    const uint32_t count = (uint32_t)(bounds[g + 1] - bounds[g]);
     fitidx[g] = ok ? 1u : 0u;
     nfidx[g] = ok ? 0u : 1u;
-    nfpre[g] = ok ? 0u : (uint32_t)(bounds[g + 1] - bounds[g]);
+    nfpre[g] = ok ? 0u : count;
+    nunit[g] = (count + VH_UNIT - 1) / VH_UNIT;
     cellmap[c0] = ok ? (int32_t)g : -1;
   }
 }
 
-// after the scans: scatter the fitting / non-fitting lists, write the totals
+// after the scans: scatter the fitting / non-fitting lists and the work
+// units, write the totals
 __global__ void k_cell_lists(const uint32_t *__restrict__ fitidx, const uint32_t *__restrict__ nfidx,
-                             const uint32_t *__restrict__ nfpre, int32_t *fitlist, int32_t *nonfit,
-                             int32_t *nonfit_prefix, int32_t *cnt, int64_t len) {
+                             const uint32_t *__restrict__ nfpre, const uint32_t *__restrict__ ustart,
+                             const int32_t *__restrict__ occupied, const int32_t *__restrict__ bounds,
+                             int32_t *fitlist, int32_t *nonfit, int32_t *nonfit_prefix,
+                             vh_unit *units, int32_t *unitstart, int32_t *cnt, int64_t len) {
   int nocc = cnt[CNT_NOCC];
   for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < len;
        g += (int64_t)gridDim.x * blockDim.x) {
@@ -308,10 +316,23 @@ __global__ void k_cell_lists(const uint32_t *__restrict__ fitidx, const uint32_t
         nonfit[nfidx[g]] = (int32_t)g;
         nonfit_prefix[nfidx[g]] = (int32_t)nfpre[g];
       }
+      const int32_t key = occupied[g], lo = bounds[g], hi = bounds[g + 1];
+      uint32_t u = ustart[g];
+      unitstart[g] = (int32_t)u;
+      for (int32_t s = lo; s < hi; s += VH_UNIT, ++u) {
+        vh_unit un;
+        un.key = key;
+        un.lo = s;
+        un.hi = hi - s > VH_UNIT ? s + VH_UNIT : hi;
+        un.g = (int32_t)g;
+        units[u] = un;
+      }
     } else if (g == nocc) {
       cnt[CNT_NFIT] = (int32_t)fitidx[g];
       cnt[CNT_NNONFIT] = (int32_t)nfidx[g];
       cnt[CNT_OUTCOUNT] = (int32_t)nfpre[g];
+      cnt[CNT_NUNITS] = (int32_t)ustart[g];
+      unitstart[g] = (int32_t)ustart[g];
       nonfit_prefix[nfidx[g]] = (int32_t)nfpre[g];
       cnt[CNT_SORTED] = 1;
     }
@@ -375,16 +396,19 @@ int vh_sort_impl(vh_accel *a) {
   VH_LAUNCH_CHECK();
   // --- classification of the occupied cells ---
   int64_t len = a->maxocc + 1;
-  uint32_t *fitidx = a->cellscratch, *nfidx = fitidx + len, *nfpre = nfidx + len;
+  uint32_t *fitidx = a->cellscratch, *nfidx = fitidx + len, *nfpre = nfidx + len,
+           *nunit = nfpre + len;
   VH_TRY(vh_fill_i32(c, a->cellmap, a->ncell, -1));
   hipLaunchKernelGGL(k_cell_classify, dim3(1024), dim3(256), 0, st, a->occupied, a->bounds,
-                     a->counters, a->nk, a->n, a->cellmap, fitidx, nfidx, nfpre, len);
+                     a->counters, a->nk, a->n, a->cellmap, fitidx, nfidx, nfpre, nunit, len);
   VH_LAUNCH_CHECK();
   VH_TRY(vh_scan_exclusive(c, fitidx, len, a->scan_tmp, a->scan_tmp_len, nullptr));
   VH_TRY(vh_scan_exclusive(c, nfidx, len, a->scan_tmp, a->scan_tmp_len, nullptr));
   VH_TRY(vh_scan_exclusive(c, nfpre, len, a->scan_tmp, a->scan_tmp_len, nullptr));
-  hipLaunchKernelGGL(k_cell_lists, dim3(1024), dim3(256), 0, st, fitidx, nfidx, nfpre, a->fitlist,
-                     a->nonfit, a->nonfit_prefix, a->counters, len);
+  VH_TRY(vh_scan_exclusive(c, nunit, len, a->scan_tmp, a->scan_tmp_len, nullptr));
+  hipLaunchKernelGGL(k_cell_lists, dim3(1024), dim3(256), 0, st, fitidx, nfidx, nfpre, nunit,
+                     a->occupied, a->bounds, a->fitlist, a->nonfit, a->nonfit_prefix, a->units,
+                     a->unitstart, a->counters, len);
   VH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_outlist, dim3(1024), dim3(256), 0, st, a->nonfit, a->nonfit_prefix,
                      a->bounds, a->counters, a->outlist);
