@@ -360,8 +360,8 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
   }
   MK(upload_converted(ctx, &a->knots64, knots, (size_t)nk));
   size_t np = (size_t)a->npad;
-  MK(vh_alloc((void **)&a->parts[0], np * 4 * PS_COUNT));
-  MK(vh_alloc((void **)&a->parts[1], np * 4 * PS_COUNT));
+  MK(vh_alloc((void **)&a->parts[0], np * 4 * VH_REC_WORDS));
+  MK(vh_alloc((void **)&a->parts[1], np * 4 * VH_REC_WORDS));
   MK(vh_alloc((void **)&a->force, np * 4 * 3));
   MK(vh_alloc((void **)&a->keys[0], np * 4));
   MK(vh_alloc((void **)&a->keys[1], np * 4));
@@ -391,8 +391,8 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
   MK(vh_alloc((void **)&a->dscal, sizeof(double) * 64));
   MK(vh_fill_i32(ctx, a->counters, 64, 0));
   MK(vh_fill_f32(ctx, a->force, (int64_t)np * 3, 0.f));
-  MK(vh_fill_i32(ctx, a->parts[0], (int64_t)np * PS_COUNT, 0));
-  MK(vh_fill_i32(ctx, a->parts[1], (int64_t)np * PS_COUNT, 0));
+  MK(vh_fill_i32(ctx, a->parts[0], (int64_t)np * VH_REC_WORDS, 0));
+  MK(vh_fill_i32(ctx, a->parts[1], (int64_t)np * VH_REC_WORDS, 0));
   MK(vh_fill_u64(ctx, (uint64_t *)a->dscal, 64, 0ull));
 #undef MK
   VH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
@@ -414,24 +414,35 @@ extern "C" int vh_accel_set_option(vh_accel *a, int option, int value) {
   return VH_EINVAL;
 }
 
-// AoS record <-> SoA streams
-__global__ void k_aos_to_soa(const int32_t *__restrict__ rec, int32_t *__restrict__ soa, int64_t n,
+// 48-byte PackedParticle records <-> the buffer's current and previous arrays
+__global__ void k_records_in(const int32_t *__restrict__ rec, int32_t *__restrict__ P, int64_t n,
                              int64_t npad) {
+  int32_t *prev = vh_prev_half(P, npad);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const int32_t *r = rec + 12 * i;
+    const int2 *r = reinterpret_cast<const int2 *>(rec + 12 * i);
+    int2 *c = reinterpret_cast<int2 *>(P + 6 * i), *q = reinterpret_cast<int2 *>(prev + 6 * i);
 #pragma unroll
-    for (int s = 0; s < 12; ++s) soa[s * npad + i] = r[s];
+    for (int w = 0; w < 3; ++w) {
+      c[w] = r[w];
+      q[w] = r[3 + w];
+    }
   }
 }
 
-__global__ void k_soa_to_aos(const int32_t *__restrict__ soa, int32_t *__restrict__ rec, int64_t n,
-                             int64_t npad) {
+__global__ void k_records_out(const int32_t *__restrict__ P, int32_t *__restrict__ rec, int64_t n,
+                              int64_t npad) {
+  const int32_t *prev = vh_prev_half(P, npad);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    int32_t *r = rec + 12 * i;
+    int2 *r = reinterpret_cast<int2 *>(rec + 12 * i);
+    const int2 *c = reinterpret_cast<const int2 *>(P + 6 * i),
+               *q = reinterpret_cast<const int2 *>(prev + 6 * i);
 #pragma unroll
-    for (int s = 0; s < 12; ++s) r[s] = soa[s * npad + i];
+    for (int w = 0; w < 3; ++w) {
+      r[w] = c[w];
+      r[3 + w] = q[w];
+    }
   }
 }
 
@@ -457,7 +468,7 @@ extern "C" int vh_particles_upload(vh_accel *a, const void *records) {
   int32_t *stage = a->parts[1 - a->cur];
   VH_CHECK_HIP(hipMemcpyAsync(stage, records, (size_t)a->npart * 48, hipMemcpyHostToDevice,
                               a->ctx->stream));
-  hipLaunchKernelGGL(k_aos_to_soa, dim3(2048), dim3(256), 0, a->ctx->stream, stage,
+  hipLaunchKernelGGL(k_records_in, dim3(2048), dim3(256), 0, a->ctx->stream, stage,
                      a->parts[a->cur], a->npart, a->npad);
   VH_LAUNCH_CHECK();
   VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
@@ -468,7 +479,7 @@ extern "C" int vh_particles_download(vh_accel *a, void *records) {
   VH_REQUIRE(a && records, "vh_particles_download: null argument");
   VH_CHECK_HIP(hipSetDevice(a->ctx->device));
   int32_t *stage = a->parts[1 - a->cur];
-  hipLaunchKernelGGL(k_soa_to_aos, dim3(2048), dim3(256), 0, a->ctx->stream, a->parts[a->cur],
+  hipLaunchKernelGGL(k_records_out, dim3(2048), dim3(256), 0, a->ctx->stream, a->parts[a->cur],
                      stage, a->npart, a->npad);
   VH_LAUNCH_CHECK();
   VH_CHECK_HIP(hipMemcpyAsync(records, stage, (size_t)a->npart * 48, hipMemcpyDeviceToHost,
@@ -633,16 +644,16 @@ __global__ void k_columns(const int32_t *__restrict__ P, int64_t npad, int64_t n
   int local = 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const float *D = (const float *)P;
+    const vh_pos r = vh_load_pos(P, i);
     float half = spacing * 0.5f;
     bool inside = true;
     for (int d = 0; d < 3; ++d) {
-      float p = x0 + (float)(P[d * npad + i] - 1) * h + D[(PS_DX + d) * npad + i];
+      float p = x0 + (float)(r.k[d] - 1) * h + r.d[d];
       inside = inside && (lo <= p) && (p <= hi);
     }
     if (inside) {
       for (int d = 0; d < 3; ++d) {
-        float t = (D[(PS_DX + d) * npad + i] + half) / spacing * nbdt + 0.5f;
+        float t = (r.d[d] + half) / spacing * nbdt + 0.5f;
         int c = (int)floorf(t) + 1;
         c = c < 1 ? 1 : (c > ncol ? ncol : c);
         cols[3 * i + d] = c;
@@ -893,12 +904,12 @@ __global__ void k_enclosed(const int32_t *__restrict__ P, int64_t npad, int64_t 
                            double h, double cx, double cy, double cz, double r2,
                            unsigned long long *out) {
   unsigned long long local = 0;
-  const float *D = (const float *)P;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    double px = x0 + (double)(P[PS_KX * npad + i] - 1) * h + (double)D[PS_DX * npad + i];
-    double py = x0 + (double)(P[PS_KY * npad + i] - 1) * h + (double)D[PS_DY * npad + i];
-    double pz = x0 + (double)(P[PS_KZ * npad + i] - 1) * h + (double)D[PS_DZ * npad + i];
+    const vh_pos r = vh_load_pos(P, i);
+    double px = x0 + (double)(r.k[0] - 1) * h + (double)r.d[0];
+    double py = x0 + (double)(r.k[1] - 1) * h + (double)r.d[1];
+    double pz = x0 + (double)(r.k[2] - 1) * h + (double)r.d[2];
     double dx = px - cx, dy = py - cy, dz = pz - cz;
     if ((dx * dx + dy * dy) + dz * dz < r2) ++local;
   }

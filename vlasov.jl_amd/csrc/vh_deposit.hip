@@ -39,7 +39,6 @@ __global__ void __launch_bounds__(64)
                 const vh_unit *__restrict__ units, const int32_t *__restrict__ cnt, int nk, int n,
                 float spacing, float nbdt, int ncol, float *__restrict__ tiles) {
   __shared__ float vals[24 * DEP_STAGE];
-  const float *D = (const float *)P;
   const int l = threadIdx.x;
   const int row = l & 15, kq = l >> 4;       // A: row kk, particle kq of the group
   const int ii = l & 7, jh = (l >> 3) & 1;   // B: column l & 15 -> (ii, jj = 2 t + jh)
@@ -60,9 +59,11 @@ __global__ void __launch_bounds__(64)
       __syncthreads();
       if (l < chunk) {
         const int64_t s = cur + l;
-        const int c1 = table_col(D[PS_DX * npad + s], half, spacing, nbdt, ncol) - 1;
-        const int c2 = table_col(D[PS_DY * npad + s], half, spacing, nbdt, ncol) - 1;
-        const int c3 = table_col(D[PS_DZ * npad + s], half, spacing, nbdt, ncol) - 1;
+        float dx, dy, dz;
+        vh_load_delta(P, s, dx, dy, dz);
+        const int c1 = table_col(dx, half, spacing, nbdt, ncol) - 1;
+        const int c2 = table_col(dy, half, spacing, nbdt, ncol) - 1;
+        const int c3 = table_col(dz, half, spacing, nbdt, ncol) - 1;
         float *v = vals + 24 * l;
 #pragma unroll
         for (int a = 0; a < 8; ++a) {
@@ -301,7 +302,6 @@ __global__ void __launch_bounds__(64 * CIC_WAVES)
   __syncthreads();
   const int wv = threadIdx.x >> 6;
   float *st = lds_cic + wv * 64 * CIC_STRIDE;
-  const float *D = (const float *)P;
   const int l = threadIdx.x & 63;
   const int kq = l >> 4, xy = l & 15, grp = (l & 15) >> 2, zc = l & 3;
   const int nunits = cnt[CNT_NUNITS];
@@ -326,9 +326,10 @@ __global__ void __launch_bounds__(64 * CIC_WAVES)
         int ix[3];
         float ax[3];
         bool inside = true;
+        const vh_pos r = vh_load_pos(P, s);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          const float p = x0f + (float)(P[(PS_KX + d) * npad + s] - 1) * hf + D[(PS_DX + d) * npad + s];
+          const float p = x0f + (float)(r.k[d] - 1) * hf + r.d[d];
           ix[d] = locate_cell(gt, n, x0t, iwt, cells, nbin, p, ax[d]);
           inside = inside && ix[d] > 0;
         }

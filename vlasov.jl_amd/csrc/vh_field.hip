@@ -81,10 +81,10 @@ struct vh_proj_args {
 __global__ void __launch_bounds__(64)
     k_field(const int32_t *__restrict__ P, int64_t npad, const float *__restrict__ csol, int n,
             const float *__restrict__ ovl, const float *__restrict__ grad,
-            const vh_unit *__restrict__ units, const int32_t *__restrict__ cnt, int nk, float x0, float h, float spacing, float nbdt,
-            float w, int ncol, vh_proj_args pa, float *__restrict__ force,
+            const vh_unit *__restrict__ units, const int32_t *__restrict__ cnt, int nk, float x0,
+            float h, float spacing, float nbdt, float w, int ncol, vh_proj_args pa,
+            float *__restrict__ force,
             double *__restrict__ part) {
-  const float *D = (const float *)P;
   const int t = threadIdx.x;
   if (pa.dev) {
     pa.px = (float)pa.dev[0];
@@ -106,7 +106,8 @@ __global__ void __launch_bounds__(64)
     const float *tb = csol + (ok ? (bx - 1) + nn * ((by - 1) + nn * (bz - 1)) : 0);
     const int lo = un.lo, hi = un.hi;
     for (int s = lo + t; s < hi; s += 64) {
-      const float dx0 = D[PS_DX * npad + s], dy0 = D[PS_DY * npad + s], dz0 = D[PS_DZ * npad + s];
+      const vh_pos rp = vh_load_pos(P, s);
+      const float dx0 = rp.d[0], dy0 = rp.d[1], dz0 = rp.d[2];
       float fxp = nan, fyp = nan, fzp = nan;
       if (ok) {
         int cx = (int)floorf((dx0 + half) / spacing * nbdt + 0.5f) + 1;
@@ -149,9 +150,9 @@ __global__ void __launch_bounds__(64)
       if (pa.coef != 0.f) {
         float px, py, pz;
         if (pa.ball) {  // the particle's own position
-          px = x0 + (float)(P[PS_KX * npad + s] - 1) * h + dx0;
-          py = x0 + (float)(P[PS_KY * npad + s] - 1) * h + dy0;
-          pz = x0 + (float)(P[PS_KZ * npad + s] - 1) * h + dz0;
+          px = x0 + (float)(rp.k[0] - 1) * h + dx0;
+          py = x0 + (float)(rp.k[1] - 1) * h + dy0;
+          pz = x0 + (float)(rp.k[2] - 1) * h + dz0;
         } else {        // rebuilt from the cell, as the reference's kernel does
           px = x0 + (float)(kx - 1) * h + dx0;
           py = x0 + (float)(ky - 1) * h + dy0;
@@ -357,7 +358,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
   }
   __shared__ float fld[3][64];
   __shared__ float tab[64 * FT_STRIDE];
-  const float *D = (const float *)P;
   const int l = threadIdx.x;
   const int j16 = l & 15, grp = l >> 4;
   const float half = spacing * 0.5f;
@@ -396,7 +396,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
       const int s = base + l;
       const bool mine = l < nlive;
       const int sr = mine ? s : lo;
-      const float dx0 = D[PS_DX * npad + sr], dy0 = D[PS_DY * npad + sr], dz0 = D[PS_DZ * npad + sr];
+      const vh_pos rp = vh_load_pos(P, sr);
+      const float dx0 = rp.d[0], dy0 = rp.d[1], dz0 = rp.d[2];
       if (ok) {
         int cx = (int)floorf((dx0 + half) / spacing * nbdt + 0.5f) + 1;
         int cy = (int)floorf((dy0 + half) / spacing * nbdt + 0.5f) + 1;
@@ -424,9 +425,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
       if (mine && pa.coef != 0.f) {
         float px, py, pz;
         if (pa.ball) {  // the particle's own position
-          px = x0 + (float)(P[PS_KX * npad + s] - 1) * h + dx0;
-          py = x0 + (float)(P[PS_KY * npad + s] - 1) * h + dy0;
-          pz = x0 + (float)(P[PS_KZ * npad + s] - 1) * h + dz0;
+          px = x0 + (float)(rp.k[0] - 1) * h + dx0;
+          py = x0 + (float)(rp.k[1] - 1) * h + dy0;
+          pz = x0 + (float)(rp.k[2] - 1) * h + dz0;
         } else {        // rebuilt from the cell, as the reference's kernel does
           px = x0 + (float)(kx - 1) * h + dx0;
           py = x0 + (float)(ky - 1) * h + dy0;
@@ -541,7 +542,6 @@ __global__ void k_fallback(const int32_t *__restrict__ P, int64_t npad, const in
                            const float *__restrict__ csolc, const double *__restrict__ kc, int nkc,
                            int nc, vh_fallback_args fa, float *__restrict__ force) {
   const int nout = cnt[CNT_OUTCOUNT];
-  const float *D = (const float *)P;
   if (fa.dev) {
     fa.px = fa.dev[0];
     fa.py = fa.dev[1];
@@ -551,8 +551,8 @@ __global__ void k_fallback(const int32_t *__restrict__ P, int64_t npad, const in
        u += (int64_t)gridDim.x * blockDim.x) {
     const int s = outlist[u];
     double p[3];
-    for (int d = 0; d < 3; ++d)
-      p[d] = x0 + (double)(P[(PS_KX + d) * npad + s] - 1) * h + (double)D[(PS_DX + d) * npad + s];
+    const vh_pos rp = vh_load_pos(P, s);
+    for (int d = 0; d < 3; ++d) p[d] = x0 + (double)(rp.k[d] - 1) * h + (double)rp.d[d];
     int cell[3];
     bool out = false;
     for (int d = 0; d < 3; ++d) {
@@ -660,15 +660,15 @@ __global__ void __launch_bounds__(256)
              const double *__restrict__ ovl, const float *__restrict__ csolc,
              const double *__restrict__ kc, vh_energy_args ea, double *__restrict__ part) {
   __shared__ double lds[8];
-  const float *D = (const float *)P;
   const int64_t nf = ea.nf, nc = ea.nc;
   double total = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npart;
        i += (int64_t)gridDim.x * blockDim.x) {
     double p[3];
     bool inside = true;
+    const vh_pos rp = vh_load_pos(P, i);
     for (int d = 0; d < 3; ++d) {
-      p[d] = ea.x0 + (double)(P[(PS_KX + d) * npad + i] - 1) * ea.h + (double)D[(PS_DX + d) * npad + i];
+      p[d] = ea.x0 + (double)(rp.k[d] - 1) * ea.h + (double)rp.d[d];
       inside = inside && ea.lo < p[d] && p[d] < ea.hi;
     }
     double phi = 0.0;
@@ -856,14 +856,15 @@ __global__ void __launch_bounds__(256)
              float a, float h, float x0, float pfac, float r2max, float inv2M,
              double *__restrict__ part) {
   __shared__ double lds[8];
-  float *D = (float *)P;
+  int32_t *Q = vh_prev_half(P, npad);
   double ek = 0.0, eo = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const int k1 = P[PS_KX * npad + i], k2 = P[PS_KY * npad + i], k3 = P[PS_KZ * npad + i];
-    const float d1 = D[PS_DX * npad + i], d2 = D[PS_DY * npad + i], d3 = D[PS_DZ * npad + i];
-    const int o1 = P[PS_PKX * npad + i], o2 = P[PS_PKY * npad + i], o3 = P[PS_PKZ * npad + i];
-    const float e1 = D[PS_PDX * npad + i], e2 = D[PS_PDY * npad + i], e3 = D[PS_PDZ * npad + i];
+    const vh_pos cur = vh_load_pos(P, i), old = vh_load_pos(Q, i);
+    const int k1 = cur.k[0], k2 = cur.k[1], k3 = cur.k[2];
+    const float d1 = cur.d[0], d2 = cur.d[1], d3 = cur.d[2];
+    const int o1 = old.k[0], o2 = old.k[1], o3 = old.k[2];
+    const float e1 = old.d[0], e2 = old.d[1], e3 = old.d[2];
     int n1, n2, n3;
     float m1, m2, m3, g1, g2, g3;
     verlet1(k1, d1, o1, e1, a, force[i], h, n1, m1, g1);
@@ -879,18 +880,15 @@ __global__ void __launch_bounds__(256)
     lx += qy * pz - qz * py;
     ly += qz * px - qx * pz;
     lz += qx * py - qy * px;
-    P[PS_KX * npad + i] = n1;
-    P[PS_KY * npad + i] = n2;
-    P[PS_KZ * npad + i] = n3;
-    D[PS_DX * npad + i] = m1;
-    D[PS_DY * npad + i] = m2;
-    D[PS_DZ * npad + i] = m3;
-    P[PS_PKX * npad + i] = k1;
-    P[PS_PKY * npad + i] = k2;
-    P[PS_PKZ * npad + i] = k3;
-    D[PS_PDX * npad + i] = d1;
-    D[PS_PDY * npad + i] = d2;
-    D[PS_PDZ * npad + i] = d3;
+    vh_pos nw;
+    nw.k[0] = n1;
+    nw.k[1] = n2;
+    nw.k[2] = n3;
+    nw.d[0] = m1;
+    nw.d[1] = m2;
+    nw.d[2] = m3;
+    vh_store_pos(P, i, nw);
+    vh_store_pos(Q, i, cur);
   }
   double vals[5] = {ek, eo, lx, ly, lz};
   for (int v = 0; v < 5; ++v) {

@@ -95,8 +95,54 @@ struct vh_mesh {
   int64_t *acc64;    // n^3 fixed-point accumulator of the CIC deposit
 };
 
-// Particle streams, structure of arrays: 12 int32/float streams of npad words.
-enum { PS_KX = 0, PS_KY, PS_KZ, PS_DX, PS_DY, PS_DZ, PS_PKX, PS_PKY, PS_PKZ, PS_PDX, PS_PDY, PS_PDZ, PS_COUNT };
+// Particle buffer: two arrays of 24-byte position records, npad each: the
+// current positions (kx, ky, kz, dx, dy, dz; int32 and float words) of slot s
+// at words [6 s, 6 s + 6), the previous positions at 6 npad + 6 s.  The
+// per-cell kernels read the current array alone, coalesced; the sort's gather
+// moves a particle that changed cell as two 24-byte pieces (two or three
+// memory sectors) rather than twelve scattered words.
+#define VH_REC_WORDS 12   // int32 words per particle in a buffer (both arrays)
+
+struct vh_pos {
+  int k[3];
+  float d[3];
+};
+
+__device__ __forceinline__ const int32_t *vh_prev_half(const int32_t *P, int64_t npad) {
+  return P + 6 * npad;
+}
+__device__ __forceinline__ int32_t *vh_prev_half(int32_t *P, int64_t npad) { return P + 6 * npad; }
+
+// one record: three 8-byte loads (records are 8-byte aligned)
+__device__ __forceinline__ vh_pos vh_load_pos(const int32_t *__restrict__ half, int64_t s) {
+  const int2 *p = reinterpret_cast<const int2 *>(half + 6 * s);
+  const int2 a = p[0], b = p[1], c = p[2];
+  vh_pos r;
+  r.k[0] = a.x;
+  r.k[1] = a.y;
+  r.k[2] = b.x;
+  r.d[0] = __int_as_float(b.y);
+  r.d[1] = __int_as_float(c.x);
+  r.d[2] = __int_as_float(c.y);
+  return r;
+}
+
+// the offsets alone (words 3..5)
+__device__ __forceinline__ void vh_load_delta(const int32_t *__restrict__ half, int64_t s, float &dx,
+                                              float &dy, float &dz) {
+  const int2 *p = reinterpret_cast<const int2 *>(half + 6 * s);
+  const int2 b = p[1], c = p[2];
+  dx = __int_as_float(b.y);
+  dy = __int_as_float(c.x);
+  dz = __int_as_float(c.y);
+}
+
+__device__ __forceinline__ void vh_store_pos(int32_t *__restrict__ half, int64_t s, const vh_pos &r) {
+  int2 *p = reinterpret_cast<int2 *>(half + 6 * s);
+  p[0] = make_int2(r.k[0], r.k[1]);
+  p[1] = make_int2(r.k[2], __float_as_int(r.d[0]));
+  p[2] = make_int2(__float_as_int(r.d[1]), __float_as_int(r.d[2]));
+}
 
 // device-side counters of the accelerator
 enum {
@@ -134,7 +180,7 @@ struct vh_accel {
   double *knots64;             // fine knots in FP64
   double spacing64;            // smoothing spacing in FP64
   double klo, khi;             // knots[3], knots[end-2]: smoothed-potential region
-  int32_t *parts[2];           // 12 streams each
+  int32_t *parts[2];           // current and previous record arrays each (vh_load_pos)
   int cur;
   float *force;                // 3 streams (fx, fy, fz), slot order
   // sort

@@ -6,9 +6,9 @@
 // stable one: an LSD radix sort of (cell key, index) pairs in passes of at
 // most 11 bits, each pass = per-tile digit histogram (LDS) -> exclusive scan
 // over (digit, tile) -> stable scatter with in-wave ranks from ballots.  The
-// 48-byte records (12 SoA streams) are then gathered once through the
-// permutation.  Keys, occupied cells and bounds follow src/kernels.jl:902-908
-// and src/accelerator.jl:397-411.
+// records are then gathered once through the permutation (each particle's
+// two 24-byte records).  Keys, occupied cells and bounds follow
+// src/kernels.jl:902-908 and src/accelerator.jl:397-411.
 #include "vh_internal.h"
 
 #define SORT_THREADS 256
@@ -47,8 +47,13 @@ __device__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t &excl, uint32_t *ld
   return tot;
 }
 
-__global__ void k_scan_reduce(const uint32_t *__restrict__ data, int64_t n, uint32_t *sums) {
+// Several arrays at once: array a (blockIdx.y, or blockIdx.x for the sums)
+// is data + a * n with block sums at sums + a * (nb + 1).
+__global__ void k_scan_reduce(const uint32_t *__restrict__ data, int64_t n, uint32_t *sums,
+                              int64_t nb) {
   __shared__ uint32_t lds[16];
+  data += blockIdx.y * n;
+  sums += blockIdx.y * (nb + 1);
   int64_t base = (int64_t)blockIdx.x * SCAN_THREADS * SCAN_ITEMS;
   uint32_t s = 0;
   for (int it = 0; it < SCAN_ITEMS; ++it) {
@@ -60,9 +65,10 @@ __global__ void k_scan_reduce(const uint32_t *__restrict__ data, int64_t n, uint
   if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
-// single block: exclusive scan of nb sums in place, total written to sums[nb]
+// one block per array: exclusive scan of its nb sums in place, total at sums[nb]
 __global__ void k_scan_sums(uint32_t *sums, int64_t nb) {
   __shared__ uint32_t lds[16];
+  sums += blockIdx.x * (nb + 1);
   uint32_t carry = 0;
   for (int64_t base = 0; base < nb; base += blockDim.x) {
     int64_t i = base + threadIdx.x;
@@ -75,8 +81,10 @@ __global__ void k_scan_sums(uint32_t *sums, int64_t nb) {
   if (threadIdx.x == 0) sums[nb] = carry;
 }
 
-__global__ void k_scan_down(uint32_t *data, int64_t n, const uint32_t *__restrict__ sums) {
+__global__ void k_scan_down(uint32_t *data, int64_t n, const uint32_t *__restrict__ sums, int64_t nb) {
   __shared__ uint32_t lds[16];
+  data += blockIdx.y * n;
+  sums += blockIdx.y * (nb + 1);
   int64_t base = (int64_t)blockIdx.x * SCAN_THREADS * SCAN_ITEMS;
   // each thread owns SCAN_ITEMS consecutive elements
   int64_t i0 = base + (int64_t)threadIdx.x * SCAN_ITEMS;
@@ -95,26 +103,35 @@ __global__ void k_scan_down(uint32_t *data, int64_t n, const uint32_t *__restric
   }
 }
 
-int vh_scan_exclusive(vh_context *c, uint32_t *data, int64_t n, uint32_t *tmp, int64_t tmp_len,
-                      uint32_t *total_out) {
+// exclusive scans of `count` consecutive arrays of n entries each, in place
+static int scan_arrays(vh_context *c, uint32_t *data, int64_t n, int count, uint32_t *tmp,
+                       int64_t tmp_len) {
   int64_t per = SCAN_THREADS * SCAN_ITEMS;
   int64_t nb = (n + per - 1) / per;
-  VH_REQUIRE(nb + 1 <= tmp_len, "scan: scratch too small (%lld blocks)", (long long)nb);
-  if (nb == 0) {
-    if (total_out) {
-      uint32_t z = 0;
-      VH_CHECK_HIP(hipMemcpyAsync(total_out, &z, 4, hipMemcpyHostToDevice, c->stream));
-    }
-    return VH_OK;
-  }
-  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, c->stream, data, n, tmp);
+  VH_REQUIRE(count * (nb + 1) <= tmp_len, "scan: scratch too small (%lld blocks)", (long long)nb);
+  if (nb == 0) return VH_OK;
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb, count), dim3(SCAN_THREADS), 0, c->stream,
+                     data, n, tmp, nb);
   VH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, c->stream, tmp, nb);
+  hipLaunchKernelGGL(k_scan_sums, dim3(count), dim3(1024), 0, c->stream, tmp, nb);
   VH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, c->stream, data, n, tmp);
+  hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb, count), dim3(SCAN_THREADS), 0, c->stream,
+                     data, n, tmp, nb);
   VH_LAUNCH_CHECK();
+  return VH_OK;
+}
+
+int vh_scan_exclusive(vh_context *c, uint32_t *data, int64_t n, uint32_t *tmp, int64_t tmp_len,
+                      uint32_t *total_out) {
+  VH_TRY(scan_arrays(c, data, n, 1, tmp, tmp_len));
   if (total_out) {
-    VH_CHECK_HIP(hipMemcpyAsync(total_out, tmp + nb, 4, hipMemcpyDeviceToDevice, c->stream));
+    int64_t per = SCAN_THREADS * SCAN_ITEMS;
+    int64_t nb = (n + per - 1) / per;
+    if (nb == 0) {
+      VH_TRY(vh_fill_i32(c, (int32_t *)total_out, 1, 0));
+    } else {
+      VH_CHECK_HIP(hipMemcpyAsync(total_out, tmp + nb, 4, hipMemcpyDeviceToDevice, c->stream));
+    }
   }
   return VH_OK;
 }
@@ -133,7 +150,11 @@ __device__ __forceinline__ uint32_t cell_key0(int kx, int ky, int kz, int nk) {
 __device__ __forceinline__ uint32_t load_key(bool first, const uint32_t *__restrict__ keys,
                                              const int32_t *__restrict__ P, int64_t npad, int64_t i,
                                              int nk) {
-  if (first) return cell_key0(P[PS_KX * npad + i], P[PS_KY * npad + i], P[PS_KZ * npad + i], nk);
+  if (first) {
+    const int2 *p = reinterpret_cast<const int2 *>(P + 6 * i);
+    const int2 a = p[0], b = p[1];
+    return cell_key0(a.x, a.y, b.x, nk);
+  }
   return keys[i];
 }
 
@@ -226,15 +247,13 @@ __global__ void k_permute(const int32_t *__restrict__ src, int32_t *__restrict__
   for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
        s += (int64_t)gridDim.x * blockDim.x) {
     int64_t i = perm[s];
-    int32_t v[PS_COUNT];
-#pragma unroll
-    for (int q = 0; q < PS_COUNT; ++q) v[q] = src[q * npad + i];
-#pragma unroll
-    for (int q = 0; q < PS_COUNT; ++q) dst[q * npad + s] = v[q];
+    const vh_pos cur = vh_load_pos(src, i), old = vh_load_pos(vh_prev_half(src, npad), i);
+    vh_store_pos(dst, s, cur);
+    vh_store_pos(vh_prev_half(dst, npad), s, old);
     bool inside = true;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      float p = x0 + (float)(v[PS_KX + d] - 1) * h + __int_as_float(v[PS_DX + d]);
+      float p = x0 + (float)(cur.k[d] - 1) * h + cur.d[d];
       inside = inside && (lo <= p) && (p <= hi);
     }
     local += inside ? 0 : 1;
@@ -402,10 +421,7 @@ int vh_sort_impl(vh_accel *a) {
   hipLaunchKernelGGL(k_cell_classify, dim3(1024), dim3(256), 0, st, a->occupied, a->bounds,
                      a->counters, a->nk, a->n, a->cellmap, fitidx, nfidx, nfpre, nunit, len);
   VH_LAUNCH_CHECK();
-  VH_TRY(vh_scan_exclusive(c, fitidx, len, a->scan_tmp, a->scan_tmp_len, nullptr));
-  VH_TRY(vh_scan_exclusive(c, nfidx, len, a->scan_tmp, a->scan_tmp_len, nullptr));
-  VH_TRY(vh_scan_exclusive(c, nfpre, len, a->scan_tmp, a->scan_tmp_len, nullptr));
-  VH_TRY(vh_scan_exclusive(c, nunit, len, a->scan_tmp, a->scan_tmp_len, nullptr));
+  VH_TRY(scan_arrays(c, fitidx, len, 4, a->scan_tmp, a->scan_tmp_len));   // the four arrays at once
   hipLaunchKernelGGL(k_cell_lists, dim3(1024), dim3(256), 0, st, fitidx, nfidx, nfpre, nunit,
                      a->occupied, a->bounds, a->fitlist, a->nonfit, a->nonfit_prefix, a->units,
                      a->unitstart, a->counters, len);
