@@ -18,99 +18,141 @@ typedef float vh_f4 __attribute__((ext_vector_type(4)));
 
 // The product C = A S (C n x m, row-major out[i' m + j]; S[k][j] = src[j n + k])
 // by v_mfma_f32_16x16x4_f32 (lane l: A[l%16][k], B[k][l%16], k from l/16).
-// A block owns 32 output columns (two column tiles) and W row tiles of 16,
-// one wave each (W = all MT row tiles up to 9, else half of them, so at most
-// 9 waves and 170 VGPRs each):
+// A block holds W row tiles of 16, one wave each (W = all MT row tiles up to
+// 9, else half of them), and walks panels of NC = 16 NCT output columns,
+// panel p, p + gridDim.x, ... (a few blocks per compute unit):
 //   * each wave keeps the A operands of its row tile for the whole K range in
-//     registers, with K permuted so that lane group g = l/16 owns the
-//     contiguous slice k = g KQ + s, s < KQ (KQ odd, >= n/4; k >= n padded
-//     with zero A).  A is stored once, at mesh creation, in that operand
-//     order (vh_gemm_pack): step s of row tile r is 64 consecutive floats,
-//     one per lane, so the wave's reads are coalesced;
-//   * the 32 source columns are contiguous in memory: the block reads them
-//     with coalesced 8-byte loads into LDS, column stride ldb = 2 (mod 32),
-//     so that the 32 lanes of a half-wave (16 columns x 2 lane groups, odd
-//     KQ) hit 32 different banks;
-//   * two independent accumulation chains (one per column tile).
-template <int KQ, int NCT>
+//     registers, loaded once per block, with K permuted so that lane group
+//     g = l/16 owns the contiguous slice k = g KQ + s, s < KQ (KQ odd,
+//     >= n/4; k >= n padded with zero A).  A is stored once, at mesh
+//     creation, in that operand order (vh_gemm_pack): step s of row tile r
+//     is 64 consecutive floats, one per lane, so the wave's reads are
+//     coalesced;
+//   * a panel's source columns are contiguous in memory: the block reads
+//     them with coalesced 8-byte loads into one of two LDS buffers, column
+//     stride ldb >= 4 KQ, ldb = 2 (mod 32), so that the 32 lanes of a
+//     half-wave (16 columns x 2 lane groups, odd KQ) hit 32 different banks;
+//     rows k >= n are zero, so every operand address is a fixed offset;
+//   * software pipeline: a panel's loads are issued a whole panel ahead of
+//     its products and land in the other LDS buffer while the current panel
+//     is multiplied, so that HBM latency, matrix-core work and stores of
+//     successive panels overlap.
+template <int KQ, int NCT, int EPI>
 __global__ void __launch_bounds__(576)
     k_rot_gemm(const float *__restrict__ A, const float *__restrict__ src, float *__restrict__ out,
-               int n, int64_t m, int ldb, int epi, const float *__restrict__ aux, int nfull) {
-  extern __shared__ float Bs[];
-  constexpr int NC = 16 * NCT;  // output columns of the block
+               int n, int64_t m, int ldb, const float *__restrict__ aux, int nfull) {
+  extern __shared__ float Bs[];   // two panels of NC x ldb
+  constexpr int NC = 16 * NCT;    // output columns of a panel
+  constexpr int PF = 4 * NCT;     // source pairs per thread and panel (NC/2 n <= PF blockDim)
   const int nthr = blockDim.x;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int lc = l & 15, lk = l >> 4;
   const int mt = (n + 15) >> 4;
   const int r = blockIdx.y * (nthr >> 6) + w;
-  const int64_t j0 = (int64_t)blockIdx.x * NC;
+  const int64_t npanel = (m + NC - 1) / NC;
+  // XCD-aware walk (gridDim.x is a multiple of 8; workgroups go to the eight
+  // XCDs in turn): the blocks of one XCD take consecutive panels, so that the
+  // 64-byte pieces they write of each output row meet in that XCD's L2
+  const int xcd = blockIdx.x & 7, per = gridDim.x >> 3;
+  const int64_t chunk = (npanel + 7) / 8;
+  int64_t p = xcd * chunk + (blockIdx.x >> 3);
+  const int64_t pend = (xcd + 1) * chunk < npanel ? (xcd + 1) * chunk : npanel;
+  if (p >= pend) return;  // the whole block
   // A operands of row tile r (waves past the last tile read tile mt - 1 and
-  // leave after the barrier)
+  // only move panels)
   float a[KQ];
   const float *ar = A + ((int64_t)(r < mt ? r : mt - 1) * KQ) * 64 + l;
 #pragma unroll
   for (int s = 0; s < KQ; ++s) a[s] = ar[64 * s];
-  // the NC x n panel (n even: pairs never straddle a column), eight pairs
-  // per thread per round, all eight loads issued before the LDS writes;
-  // out-of-range pairs read pair 0 and are replaced by zeros (the select
-  // tests the address's condition, so the load stays unconditional)
   const float2 *src2 = (const float2 *)src;
-  const int64_t tot2 = m * n / 2, base2 = j0 * n / 2;
+  const int64_t tot2 = m * n / 2;
   const int per2 = NC / 2 * n;
-  for (int e0 = 0; e0 < per2; e0 += 8 * nthr) {
-    float2 pv[8];
+  const int pbuf = NC * ldb;
+  // Every load is issued, at an address clamped into the source; pairs
+  // outside the panel or the source are zeroed when the panel is written to
+  // LDS.  (Using the loaded value right away, or a load under a condition,
+  // makes the compiler wait for the load at once, which serialises the
+  // pipeline.)
+  auto fetch = [&](float2 (&pv)[PF], int64_t q) {
+    const int64_t base2 = q * NC * (int64_t)n / 2;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int e = e0 + threadIdx.x + u * nthr;
-      const int64_t g = base2 + e;
-      const bool in = e < per2 && g < tot2;
-      const float2 v = src2[in ? g : 0];
-      pv[u] = in ? v : make_float2(0.f, 0.f);
+    for (int u = 0; u < PF; ++u) {
+      const int64_t g = base2 + threadIdx.x + u * nthr;
+      pv[u] = src2[g < tot2 ? g : tot2 - 1];
     }
+  };
+  auto stash = [&](const float2 (&pv)[PF], int64_t q, float *B) {
+    const int64_t base2 = q * NC * (int64_t)n / 2;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int e = e0 + threadIdx.x + u * nthr;
+    for (int u = 0; u < PF; ++u) {
+      const int e = threadIdx.x + u * nthr;
       if (e < per2) {
-        const int f = 2 * e, cc = f / n, k = f - cc * n;
-        *(float2 *)(Bs + cc * ldb + k) = pv[u];
+        const int f = 2 * e, cc = f / n, k = f - cc * n;   // n even: pairs never straddle a column
+        *(float2 *)(B + cc * ldb + k) = base2 + e < tot2 ? pv[u] : make_float2(0.f, 0.f);
       }
     }
+  };
+  // rows k in [n, ldb) of both buffers stay zero (the panels fill k < n)
+  for (int e = threadIdx.x; e < 2 * NC * (ldb - n); e += nthr) {
+    const int row = e / (ldb - n), k = n + e % (ldb - n);
+    Bs[row * ldb + k] = 0.f;
   }
+  float2 R[PF];
+  fetch(R, p);
+  stash(R, p, Bs);
   __syncthreads();
-  if (r >= mt) return;
-  vh_f4 acc[NCT];
+  if (p + per < pend) fetch(R, p + per);
+  int buf = 0;
+  for (; p < pend; p += per, buf ^= 1) {
+    // the products of this panel while the next one's loads are in flight
+    const float *B = Bs + buf * pbuf;
+    vh_f4 acc[NCT];
 #pragma unroll
-  for (int ct = 0; ct < NCT; ++ct) acc[ct] = (vh_f4){0.f, 0.f, 0.f, 0.f};
+    for (int ct = 0; ct < NCT; ++ct) acc[ct] = (vh_f4){0.f, 0.f, 0.f, 0.f};
+    if (r < mt) {
 #pragma unroll
-  for (int s = 0; s < KQ; ++s) {
-    int k = lk * KQ + s;
-    k = k < n ? k : n - 1;  // A is zero there
+      for (int s = 0; s < KQ; ++s) {
+        const int k = lk * KQ + s;  // k >= n: zero A, zero LDS padding
 #pragma unroll
-    for (int ct = 0; ct < NCT; ++ct)
-      acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], Bs[(16 * ct + lc) * ldb + k], acc[ct], 0, 0, 0);
-  }
-#pragma unroll
-  for (int ct = 0; ct < NCT; ++ct) {
-    const int64_t j = j0 + ct * 16 + lc;
-    if (j >= m) continue;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int orow = r * 16 + 4 * lk + v;
-      if (orow >= n) continue;
-      const int64_t idx = (int64_t)orow * m + j;
-      const float val = acc[ct][v];
-      if (epi == EPI_STORE) {
-        out[idx] = val;
-      } else if (epi == EPI_HADAMARD) {
-        out[idx] = val * aux[idx];
-      } else if (epi == EPI_ACCUMULATE) {
-        out[idx] = out[idx] + val;
-      } else {  // EPI_FILL_INTERIOR: idx is (i, j, k) of the interior, i fastest
-        const int64_t nn = n;
-        const int64_t i = idx % nn, jj = (idx / nn) % nn, kz = idx / (nn * nn);
-        out[(i + 1) + nfull * ((jj + 1) + (int64_t)nfull * (kz + 1))] = val;
+        for (int ct = 0; ct < NCT; ++ct)
+          acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], B[(16 * ct + lc) * ldb + k], acc[ct], 0, 0, 0);
       }
     }
+    // the next panel into the other buffer before this one's stores, so that
+    // waiting for its loads does not wait for the stores; the scheduling
+    // barrier keeps that wait after the products
+    __builtin_amdgcn_sched_barrier(0);
+    const bool more = p + per < pend;
+    if (more) stash(R, p + per, Bs + (buf ^ 1) * pbuf);
+    // the loads of the panel after next, issued before this panel's stores
+    if (p + 2 * per < pend) fetch(R, p + 2 * per);
+    if (r < mt) {
+      const int64_t j0 = p * NC;
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) {
+        const int64_t j = j0 + ct * 16 + lc;
+        if (j >= m) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int orow = r * 16 + 4 * lk + v;
+          if (orow >= n) continue;
+          const int64_t idx = (int64_t)orow * m + j;
+          const float val = acc[ct][v];
+          if (EPI == EPI_STORE) {
+            out[idx] = val;
+          } else if (EPI == EPI_HADAMARD) {
+            out[idx] = val * aux[idx];
+          } else if (EPI == EPI_ACCUMULATE) {
+            out[idx] = out[idx] + val;
+          } else {  // EPI_FILL_INTERIOR: idx is (i, j, k) of the interior, i fastest
+            const int64_t nn = n;
+            const int64_t i = idx % nn, jj = (idx / nn) % nn, kz = idx / (nn * nn);
+            out[(i + 1) + nfull * ((jj + 1) + (int64_t)nfull * (kz + 1))] = val;
+          }
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -141,9 +183,11 @@ int vh_gemm_pack(vh_context *c, float **dst, const double *colmajor, int n) {
   return VH_OK;
 }
 
-// launch shape: column tiles per block, and waves (row tiles) per block
-#define GEMM_NCT 2
+// launch shape: column tiles per panel, waves (row tiles) per block, blocks
+// per compute unit
+#define GEMM_NCT 1
 #define GEMM_WAVES(mt) ((mt) <= 9 ? (mt) : ((mt) + 1) / 2)
+#define GEMM_BPC 1
 
 int vh_rotate_gemm(vh_context *c, const float *A, const float *src, float *out, int n, int64_t m,
                    int epi, const float *aux, int nfull) {
@@ -152,20 +196,34 @@ int vh_rotate_gemm(vh_context *c, const float *A, const float *src, float *out, 
   VH_REQUIRE(n % 2 == 0 && ((uintptr_t)src & 7) == 0,
              "rotating product: n (%d) must be even and the source 8-byte aligned", n);
   const int kq = gemm_kq(n);
-  const int ldb = n + ((2 - n % 32) + 32) % 32;  // smallest >= n with ldb = 2 (mod 32)
+  // smallest >= 4 kq (the K range the lanes read) with ldb = 2 (mod 32)
+  const int ldb = 4 * kq + ((2 - (4 * kq) % 32) + 32) % 32;
   const int wv = GEMM_WAVES(mt);
   constexpr int nc = 16 * GEMM_NCT;
-  dim3 grid((unsigned)((m + nc - 1) / nc), (unsigned)((mt + wv - 1) / wv)), block(64 * wv);
-  const size_t lds = sizeof(float) * nc * (size_t)ldb;
-  switch (kq) {
+  VH_REQUIRE(nc / 2 * n <= 4 * GEMM_NCT * 64 * wv, "rotating product: panel of n = %d too large", n);
+  const int gy = (mt + wv - 1) / wv;
+  const int64_t npanel = (m + nc - 1) / nc;
+  int64_t gx = ((int64_t)c->num_cus * GEMM_BPC + gy - 1) / gy;
+  if (gx > npanel) gx = npanel;
+  gx = (gx + 7) / 8 * 8;
+  dim3 grid((unsigned)gx, (unsigned)gy), block(64 * wv);
+  const size_t lds = sizeof(float) * 2 * nc * (size_t)ldb;
+  VH_REQUIRE(epi >= EPI_STORE && epi <= EPI_ACCUMULATE, "rotating product: epilogue %d", epi);
+#define LAUNCH(T, E)                                                                               \
+  hipLaunchKernelGGL((k_rot_gemm<T, GEMM_NCT, E>), grid, block, lds, c->stream, A, src, out, n, m, \
+                     ldb, aux, nfull)
 #define CASE(T)                                                                                    \
   case T:                                                                                          \
-    hipLaunchKernelGGL((k_rot_gemm<T, GEMM_NCT>), grid, block, lds, c->stream, A, src, out, n, m,  \
-                       ldb, epi, aux, nfull);                                                      \
+    if (epi == EPI_STORE) LAUNCH(T, EPI_STORE);                                                    \
+    else if (epi == EPI_HADAMARD) LAUNCH(T, EPI_HADAMARD);                                         \
+    else if (epi == EPI_ACCUMULATE) LAUNCH(T, EPI_ACCUMULATE);                                     \
+    else LAUNCH(T, EPI_FILL_INTERIOR);                                                             \
     break;
+  switch (kq) {
     CASE(5) CASE(9) CASE(15) CASE(17) CASE(25) CASE(33) CASE(41) CASE(49) CASE(57) CASE(65) CASE(69)
-#undef CASE
   }
+#undef CASE
+#undef LAUNCH
   VH_LAUNCH_CHECK();
   return VH_OK;
 }
