@@ -283,7 +283,7 @@ extern "C" int vh_accel_destroy(vh_accel *a) {
                   a->keys[0], a->keys[1], a->vals[0], a->vals[1], a->counts, a->scan_tmp,
                   a->occupied, a->bounds, a->cellmap, a->fitlist, a->nonfit, a->nonfit_prefix,
                   a->outlist, a->counters, a->flags, a->tiles, a->dpart, a->dscal, a->cellscratch,
-                  a->dproj, a->units, a->unitstart};
+                  a->dproj, a->units, a->unitstart, a->zero};
   for (void *p : ptrs) vh_free(p);
   delete a;
   return VH_OK;
@@ -389,6 +389,8 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
   a->dpart_len = 1 << 16;
   MK(vh_alloc((void **)&a->dpart, sizeof(double) * 8 * a->dpart_len));
   MK(vh_alloc((void **)&a->dscal, sizeof(double) * 64));
+  MK(vh_alloc((void **)&a->zero, sizeof(float) * 64));
+  MK(vh_fill_f32(ctx, a->zero, 64, 0.f));
   MK(vh_fill_i32(ctx, a->counters, 64, 0));
   MK(vh_fill_f32(ctx, a->force, (int64_t)np * 3, 0.f));
   MK(vh_fill_i32(ctx, a->parts[0], (int64_t)np * VH_REC_WORDS, 0));

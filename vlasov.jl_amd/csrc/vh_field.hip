@@ -346,10 +346,10 @@ __device__ __forceinline__ void ft_contract(const float A[8][3], const float *ta
 
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     k_field_mfma(const int32_t *__restrict__ P, int64_t npad, const float *__restrict__ csol, int n,
-                 const float4 *__restrict__ og, const vh_unit *__restrict__ units,
-                 const int32_t *__restrict__ cnt, int nk, float x0, float h, float spacing,
-                 float nbdt, float w, int ncol, vh_proj_args pa, float *__restrict__ force,
-                 double *__restrict__ part) {
+                 const float4 *__restrict__ og, const float *__restrict__ zero,
+                 const vh_unit *__restrict__ units, const int32_t *__restrict__ cnt, int nk,
+                 float x0, float h, float spacing, float nbdt, float w, int ncol,
+                 vh_proj_args pa, float *__restrict__ force, double *__restrict__ part) {
   typedef ft_f4 f4;
   if (pa.dev) {
     pa.px = (float)pa.dev[0];
@@ -383,11 +383,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
         for (int s = 0; s < 3; ++s) {
           const int ii = 4 * s + grp;
           const bool in = q < 30 && kk < 10 && ii < 10;
-          const float v = tb[in ? ii + nn * (jj + nn * kk) : 0];  // unconditional read
-          A[t][s] = in ? v : 0.f;
+          // padding reads a zero word: the loaded value is used as it is, so
+          // the loads stay in flight until the first product
+          A[t][s] = *(in ? tb + ii + nn * (jj + nn * kk) : zero);
         }
       }
     }
+    // the first chunk's records; each later chunk's are loaded during the
+    // previous chunk's contraction
+    vh_pos rnext = vh_load_pos(P, lo + l < hi ? lo + l : lo);
     for (int base = lo; base < hi; base += 64) {
       const int nlive = min(64, hi - base);
       // phase A, one lane per particle: offsets, the six table columns of the
@@ -395,44 +399,41 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
       // contraction, so its latency overlaps the table loads)
       const int s = base + l;
       const bool mine = l < nlive;
-      const int sr = mine ? s : lo;
-      const vh_pos rp = vh_load_pos(P, sr);
+      const vh_pos rp = rnext;
       const float dx0 = rp.d[0], dy0 = rp.d[1], dz0 = rp.d[2];
+      // everything that reads the record comes before the first branch (a
+      // load still in flight at a branch is waited for with every later one)
+      int cx = (int)floorf((dx0 + half) / spacing * nbdt + 0.5f) + 1;
+      int cy = (int)floorf((dy0 + half) / spacing * nbdt + 0.5f) + 1;
+      int cz = (int)floorf((dz0 + half) / spacing * nbdt + 0.5f) + 1;
+      cx = min(max(cx, 1), ncol) - 1;
+      cy = min(max(cy, 1), ncol) - 1;
+      cz = min(max(cz, 1), ncol) - 1;
+      float px, py, pz;
+      if (pa.ball) {  // the particle's own position
+        px = x0 + (float)(rp.k[0] - 1) * h + dx0;
+        py = x0 + (float)(rp.k[1] - 1) * h + dy0;
+        pz = x0 + (float)(rp.k[2] - 1) * h + dz0;
+      } else {        // rebuilt from the cell, as the reference's kernel does
+        px = x0 + (float)(kx - 1) * h + dx0;
+        py = x0 + (float)(ky - 1) * h + dy0;
+        pz = x0 + (float)(kz - 1) * h + dz0;
+      }
+      float4 gv[3][5];
       if (ok) {
-        int cx = (int)floorf((dx0 + half) / spacing * nbdt + 0.5f) + 1;
-        int cy = (int)floorf((dy0 + half) / spacing * nbdt + 0.5f) + 1;
-        int cz = (int)floorf((dz0 + half) / spacing * nbdt + 0.5f) + 1;
-        cx = min(max(cx, 1), ncol) - 1;
-        cy = min(max(cy, 1), ncol) - 1;
-        cz = min(max(cz, 1), ncol) - 1;
-        // each axis: overlap and gradient side by side, five 16-byte loads
+        // each axis: overlap and gradient side by side, five 16-byte loads;
+        // all fifteen are issued before any is written to LDS, and the
+        // projectile term below runs while they are in flight
         const int col[3] = {cx, cy, cz};
-        float *row = tab + l * FT_STRIDE;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           const float4 *src = og + 5 * col[d];
 #pragma unroll
-          for (int a = 0; a < 5; ++a) {
-            const float4 v = src[a];
-            row[20 * d + 4 * a] = v.x;
-            row[20 * d + 4 * a + 1] = v.y;
-            row[20 * d + 4 * a + 2] = v.z;
-            row[20 * d + 4 * a + 3] = v.w;
-          }
+          for (int a = 0; a < 5; ++a) gv[d][a] = src[a];
         }
       }
       float fx2 = 0.f, fy2 = 0.f, fz2 = 0.f;
       if (mine && pa.coef != 0.f) {
-        float px, py, pz;
-        if (pa.ball) {  // the particle's own position
-          px = x0 + (float)(rp.k[0] - 1) * h + dx0;
-          py = x0 + (float)(rp.k[1] - 1) * h + dy0;
-          pz = x0 + (float)(rp.k[2] - 1) * h + dz0;
-        } else {        // rebuilt from the cell, as the reference's kernel does
-          px = x0 + (float)(kx - 1) * h + dx0;
-          py = x0 + (float)(ky - 1) * h + dy0;
-          pz = x0 + (float)(kz - 1) * h + dz0;
-        }
         const float dx = pa.px - px, dy = pa.py - py, dz = pa.pz - pz;
         const float d2 = dx * dx + dy * dy + dz * dz;
         const float mm = pa.coef * (pa.ball ? ball_force_kernel_f(d2, pa.radius)
@@ -452,7 +453,23 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
                                         : erff(r / 1.4142135624f / pa.sigma) / r;
         }
       }
+      if (ok) {
+        float *row = tab + l * FT_STRIDE;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+          for (int a = 0; a < 5; ++a) {
+            row[20 * d + 4 * a] = gv[d][a].x;
+            row[20 * d + 4 * a + 1] = gv[d][a].y;
+            row[20 * d + 4 * a + 2] = gv[d][a].z;
+            row[20 * d + 4 * a + 3] = gv[d][a].w;
+          }
+      }
       __syncthreads();
+      {
+        const int sn = base + 64 + l;
+        rnext = vh_load_pos(P, sn < hi ? sn : lo);
+      }
       // phase B: the contraction, one sub-group of 16 particles at a time
       // (two at a time, ft_contract<2>, measured no faster at twice the
       // registers)
@@ -783,9 +800,9 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
                        (float)weight, a->ncol, pa, a->force, a->dpart);
   else
     hipLaunchKernelGGL(k_field_mfma, dim3(grid), dim3(64), 0, st, a->parts[a->cur], a->npad,
-                       fine->csol, a->n, (const float4 *)a->og, a->units, a->counters, a->nk,
-                       (float)a->x0, (float)a->h, a->spacing, (float)a->nbdt, (float)weight,
-                       a->ncol, pa, a->force, a->dpart);
+                       fine->csol, a->n, (const float4 *)a->og, a->zero, a->units, a->counters,
+                       a->nk, (float)a->x0, (float)a->h, a->spacing, (float)a->nbdt,
+                       (float)weight, a->ncol, pa, a->force, a->dpart);
   vh_prof_end(c, tok);
   VH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_reduce_cols, dim3(4), dim3(1024), 0, st, a->dpart, grid, 4, a->dscal + 8);

@@ -176,6 +176,7 @@ struct vh_accel {
   int nbdt, ncol;
   float *ovl, *grad, *nodes;   // [col*10 + a], [col*8 + a]
   float *og;                   // [col*20 + a]: overlap a < 10, gradient 10 + a (16-byte columns)
+  float *zero;                 // 64 zero words (the target of padding loads)
   double *ovl64;               // overlap in FP64 (energy budget)
   double *knots64;             // fine knots in FP64
   double spacing64;            // smoothing spacing in FP64
