@@ -43,6 +43,30 @@ __device__ __forceinline__ float gaussian_force_kernel_f(float r2, float s) {
   return (erff(x) - (float)0.79788456080286535588 * (r / s) * expf(-u2 / 2.0f)) / (r2 * r);
 }
 
+// The Gaussian law and, with `pot`, the pair potential erf(r / (sqrt2 s)) / r
+// of src/kernels.jl:425-443 for the matrix-core kernel: one erf serves both,
+// and the quotients by s are products by 1/s (agreement to FP32 rounding).
+__device__ __forceinline__ void gaussian_law_f(float r2, float r, float s, int pot, float &kern,
+                                               float &pe) {
+  const float is = 1.0f / s;
+  const float u2 = r2 * (is * is);
+  const float x = r * (is * 0.70710678118654752f);
+  const bool near = u2 <= 0.25f;
+  const float e = (pot || !near) ? erff(x) : 0.f;
+  if (near) {
+    const float c[7] = {(float)(1.0 / 3.0), (float)(-1.0 / 10.0), (float)(1.0 / 56.0),
+                        (float)(-1.0 / 432.0), (float)(1.0 / 4224.0), (float)(-1.0 / 49920.0),
+                        (float)(1.0 / 685440.0)};
+    float p = 0.f;
+    for (int k = 6; k >= 0; --k) p = c[k] + u2 * p;
+    kern = (float)0.79788456080286535588 * p * (is * is * is);
+  } else {
+    kern = (e - (float)0.79788456080286535588 * (r * is) * expf(-0.5f * u2)) / (r2 * r);
+  }
+  pe = 0.f;
+  if (pot) pe = r < 1.0e-4f * s ? (float)0.79788456080286535588 * is : e / r;
+}
+
 __device__ __forceinline__ float ball_force_kernel_f(float r2, float R) {
   return r2 > R * R ? 1.0f / (r2 * sqrtf(r2)) : 1.0f / (R * R * R);
 }
@@ -76,6 +100,7 @@ struct vh_proj_args {
   int ball;           // VH_SOFT_BALL
   float radius;       // ball radius (ball mode)
   const double *dev;  // device-resident projectile state (position at 0..2), or null
+  int pot;            // the pair-potential sum is wanted (matrix-core kernel)
 };
 
 __global__ void __launch_bounds__(64)
@@ -436,22 +461,23 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
       if (mine && pa.coef != 0.f) {
         const float dx = pa.px - px, dy = pa.py - py, dz = pa.pz - pz;
         const float d2 = dx * dx + dy * dy + dz * dz;
-        const float mm = pa.coef * (pa.ball ? ball_force_kernel_f(d2, pa.radius)
-                                            : gaussian_force_kernel_f(d2, pa.sigma));
+        const float r = sqrtf(d2);
+        float kern, pot = 0.f;
+        if (pa.ball) {
+          const float R = pa.radius;
+          kern = ball_force_kernel_f(d2, R);
+          if (pa.pot) pot = r < R ? -(3.0f - (r / R) * (r / R)) / (2.0f * R) : -1.0f / r;
+        } else {
+          gaussian_law_f(d2, r, pa.sigma, pa.pot, kern, pot);
+        }
+        const float mm = pa.coef * kern;
         fx2 = mm * dx;
         fy2 = mm * dy;
         fz2 = mm * dz;
-        const float r = sqrtf(d2);
         sax += fx2;
         say += fy2;
         saz += fz2;
-        if (pa.ball) {
-          const float R = pa.radius;
-          sae += r < R ? -(3.0f - (r / R) * (r / R)) / (2.0f * R) : -1.0f / r;
-        } else {
-          sae += r < 1.0e-4f * pa.sigma ? 0.7978845608f / pa.sigma
-                                        : erff(r / 1.4142135624f / pa.sigma) / r;
-        }
+        sae += pot;
       }
       if (ok) {
         float *row = tab + l * FT_STRIDE;
@@ -779,6 +805,7 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
   pa.sigma = 1.f;
   pa.ball = soft_mode == VH_SOFT_BALL;
   pa.radius = 1.f;
+  pa.pot = dproj == nullptr;  // the projectile on the device reads only the force sums
   if (proj) {
     pa.px = (float)proj[0];
     pa.py = (float)proj[1];
