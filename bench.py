@@ -95,24 +95,39 @@ def run_crossing(va, npart: int, nfine: int, mode: str, warmup: int, steps: int,
     """The crossing at `npart` particles: `warmup` untimed steps, then `steps`
     timed ones.  The energy budget is taken on the first warm-up step and on
     the last timed step (two interaction-energy sweeps, inside the timed
-    region for the second)."""
+    region for the second).  With `profile`, the field kernel is timed with
+    HIP events on the library's stream over the timed steps (two events per
+    step), and every stage over the last 20 warm-up steps (outside the timed
+    region: a region's two events cost about 4 us of device time)."""
     t0 = time.perf_counter()
     p = va.xenon_parameters(npart, nfine)
     proj = va.xenon_projectile()
     sim = va.Simulation(p, va.read_radial_profile(), projectile=proj, softening_mode=mode)
     log(f"built in {time.perf_counter() - t0:.1f} s: {npart} particles, {sim.fine.n}^3 grids")
+    nst = len(va._lib.STAGES)
+    field_bit = 1 << va._lib.STAGES.index("field_kernel")
+
+    def read_profile():
+        ms = np.zeros(nst)
+        cnt = np.zeros(nst, dtype=np.int64)
+        va._lib.call("vh_profile_read", sim.ctx.handle, ms, cnt)
+        va._lib.call("vh_profile_enable", sim.ctx.handle, 0)
+        return {s: (ms[i] / cnt[i] if cnt[i] else 0.0) for i, s in enumerate(va._lib.STAGES)}
+
     e_first = None
+    stage_from = max(0, warmup - 20)
     for i in range(warmup):
+        if profile and i == stage_from:
+            sim.ctx.sync()
+            va._lib.call("vh_profile_enable", sim.ctx.handle, va._lib.VH_PROF_ALL)
         b = sim.step(energy=(i == 0))
         if b is not None:
             e_first = b
     sim.ctx.sync()
+    stages = read_profile() if profile else {}
     log(f"warm-up {warmup} steps done")
-    nst = len(va._lib.STAGES)
-    ms = np.zeros(nst)
-    cnt = np.zeros(nst, dtype=np.int64)
     if profile:
-        va._lib.call("vh_profile_enable", sim.ctx.handle, 1)
+        va._lib.call("vh_profile_enable", sim.ctx.handle, field_bit)
     sim.ctx.sync()
     t1 = time.perf_counter()
     e_last = None
@@ -124,14 +139,11 @@ def run_crossing(va, npart: int, nfine: int, mode: str, warmup: int, steps: int,
             log(f"step {warmup + i + 1}: x_ion = {sim.projectile.position[0]:.1f}")
     sim.ctx.sync()
     elapsed = time.perf_counter() - t1
-    if profile:
-        va._lib.call("vh_profile_read", sim.ctx.handle, ms, cnt)
-        va._lib.call("vh_profile_enable", sim.ctx.handle, 0)
-    stages = {s: (ms[i] / cnt[i] if cnt[i] else 0.0) for i, s in enumerate(va._lib.STAGES)}
+    field_ms = read_profile()["field_kernel"] if profile else 0.0
     q = sim.enclosed_charge(8.0)
     nocc, outcount = sim.acc.sort()
-    out = dict(elapsed=elapsed, stages=stages, q=q, nocc=nocc, outcount=outcount,
-               step=sim.step_count, e_first=e_first, e_last=e_last,
+    out = dict(elapsed=elapsed, stages=stages, field_ms=field_ms, q=q, nocc=nocc,
+               outcount=outcount, step=sim.step_count, e_first=e_first, e_last=e_last,
                eloss_ev=sim.projectile.energy_loss() * va.physics.HARTREE_TO_EV)
     sim.close()
     return out
@@ -183,8 +195,9 @@ def main():
     r = run_crossing(va, args.particles, args.nfine, args.mode, args.warmup, args.steps, True)
     elapsed, stages, outcount = r["elapsed"], r["stages"], r["outcount"]
     ms_step = 1000.0 * elapsed / args.steps
-    # roofline of the dominant kernel: the smoothed-field contraction (FP32)
-    field_ms = stages["field_kernel"]
+    # roofline of the dominant kernel: the smoothed-field contraction (FP32),
+    # timed with HIP events over the timed steps
+    field_ms = r["field_ms"]
     inside = args.particles - outcount
     achieved = inside * FIELD_FLOP_PER_PARTICLE / (field_ms * 1e-3) / 1e12 if field_ms else 0.0
     e0, e1 = r["e_first"], r["e_last"]
@@ -215,6 +228,7 @@ def main():
                      "per_launch_flop": inside * FIELD_FLOP_PER_PARTICLE,
                      "avg_launch_ms": field_ms},
         "stages_ms": stages,
+        "stages_ms_steps": f"{max(0, args.warmup - 20) + 1}-{args.warmup} (warm-up, every stage timed)",
         "capture_q8": r["q"],
         "step": r["step"],
         "outcount": outcount,

@@ -208,6 +208,17 @@ int vh_forces_resident(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weigh
  * reduction, then step!(proj, force, dt), in FP64 on the device. */
 int vh_projectile_step(vh_accel *a, double dt);
 
+/* One step of the resident loop without host synchronisation, the sequence
+ * of _update_forces_resident! and step! (src/simulation.jl:472-541; the
+ * driver's nine calls): vh_deposit_smoothed, vh_deposit_cic,
+ * vh_poisson_nested, vh_spline_coefficients (fine), vh_effective_potential
+ * (both levels, jellium of jel_nions ions and radius jel_radius), the forces
+ * with the device projectile (vh_forces_resident + vh_projectile_step) or,
+ * without vh_projectile_set, without projectile, and vh_verlet.  One call
+ * from the host per step. */
+int vh_step(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, double dt, double mass,
+            double rcmax, double jel_nions, double jel_radius);
+
 /* interaction_energy(cloud, fine, csol_f, coarse, csol_c, sm; enclosed)
  * (src/energy.jl:19-43): sum over the particles of weight * phi(r), in FP64,
  * with phi the smoothed potential of the fine csol inside
@@ -238,7 +249,10 @@ enum {
   VH_ST_DEPOSIT_KERNEL, /* the per-cell deposit tile kernel alone */
   VH_ST_COUNT
 };
-int vh_profile_enable(vh_context *ctx, int on);
+/* mask: bit s times stage s (0 stops timing; VH_PROF_ALL every stage).
+ * Each timed region costs two event records on the stream. */
+#define VH_PROF_ALL (-1)
+int vh_profile_enable(vh_context *ctx, int mask);
 /* Synchronises, adds the elapsed time of every recorded region to ms[stage]
  * and the number of regions to count[stage] (arrays of VH_ST_COUNT), and
  * clears the record. */

@@ -390,3 +390,25 @@ def test_resident_projectile_matches_host(config1):
     finally:
         for s in sims:
             s.close()
+
+
+def test_native_step_matches_separate_calls(config1):
+    """vh_step (one native call per step) against the same step made of the
+    separate entry points, without projectile: identical clouds after three
+    steps (the same kernels in the same order)."""
+    p, fine, coarse, sm = config1
+    n = p.nparticles
+    w = p.nelectrons / n
+    pos, mom = va.sample_thomas_fermi(va.read_radial_profile(), n, w, seed=-7)
+    a = va.Simulation(p, None, tables=(fine, coarse, sm), positions=pos, momenta=mom)
+    b = va.Simulation(p, None, tables=(fine, coarse, sm), positions=pos, momenta=mom)
+    try:
+        for _ in range(3):
+            a.step()
+            b.update_forces(advance=True)
+            b.acc.verlet(p.dt, b.mass, p.rcmax, diagnostics=False)
+        assert a.acc.download().tobytes() == b.acc.download().tobytes()
+        assert a.acc.forces().tobytes() == b.acc.forces().tobytes()
+    finally:
+        a.close()
+        b.close()

@@ -13,6 +13,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "libvlasov_hip.so"
 
 VH_GRID_RHO, VH_GRID_PHI, VH_GRID_CSOL = 0, 1, 2
+VH_PROF_ALL = -1
 VH_SOFT_DEVICE, VH_SOFT_BALL = 0, 1
 VH_OPT_EXACT_FIELD = 0
 STAGES = ("sort", "deposit_fine", "deposit_cic", "poisson", "spline", "meanfield", "field",
@@ -66,6 +67,8 @@ _SIGNATURES = {
     "vh_projectile_get": [_vp, _f64p],
     "vh_forces_resident": [_vp, _vp, _vp, C.c_double],
     "vh_projectile_step": [_vp, C.c_double],
+    "vh_step": [_vp, _vp, _vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                C.c_double],
     "vh_interaction_energy": [_vp, _vp, _vp, C.c_double, C.c_double, C.POINTER(C.c_double)],
     "vh_profile_enable": [_vp, C.c_int],
     "vh_profile_read": [_vp, _f64p, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")],

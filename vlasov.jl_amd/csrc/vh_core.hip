@@ -531,7 +531,7 @@ static hipEvent_t take_event(vh_context *c) {
 }
 
 int vh_prof_begin(vh_context *c, int stage) {
-  if (!c->prof_on) return -1;
+  if (!((unsigned)c->prof_on >> stage & 1u)) return -1;
   vh_prof_rec r;
   r.stage = stage;
   r.start = take_event(c);
@@ -549,7 +549,7 @@ void vh_prof_end(vh_context *c, int token) {
 
 extern "C" int vh_profile_enable(vh_context *ctx, int on) {
   VH_REQUIRE(ctx != nullptr, "vh_profile_enable: null context");
-  ctx->prof_on = on ? 1 : 0;
+  ctx->prof_on = on;  // a mask of stages
   return VH_OK;
 }
 
@@ -869,6 +869,28 @@ extern "C" int vh_projectile_step(vh_accel *a, double dt) {
   }
   VH_CHECK_HIP(hipSetDevice(a->ctx->device));
   return vh_projectile_step_impl(a, dt);
+}
+
+extern "C" int vh_step(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, double dt,
+                       double mass, double rcmax, double jel_nions, double jel_radius) {
+  VH_REQUIRE(a && fine && coarse, "vh_step: null argument");
+  VH_REQUIRE(std::isfinite(weight) && dt > 0 && mass > 0 && jel_radius > 0,
+             "vh_step: weight finite, dt, mass and jellium radius positive");
+  // _update_forces_resident! (src/simulation.jl:472-531), then step!
+  VH_TRY(vh_deposit_smoothed(a, fine, weight, nullptr));
+  VH_TRY(vh_deposit_cic(a, coarse, weight));
+  VH_TRY(vh_poisson_nested(fine, coarse));
+  VH_TRY(vh_spline_coefficients(fine));
+  VH_TRY(vh_effective_potential(fine, jel_nions, jel_radius));
+  VH_TRY(vh_effective_potential(coarse, jel_nions, jel_radius));
+  if (a->has_dproj) {
+    VH_TRY(vh_forces_resident(a, fine, coarse, weight));
+    VH_TRY(vh_projectile_step(a, dt));
+  } else {
+    VH_TRY(vh_forces(a, fine, coarse, weight, nullptr, VH_SOFT_DEVICE, 0.0, nullptr, nullptr));
+  }
+  VH_TRY(vh_verlet(a, dt, mass, rcmax, nullptr));
+  return VH_OK;
 }
 
 extern "C" int vh_interaction_energy(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight,

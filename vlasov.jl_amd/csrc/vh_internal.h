@@ -59,7 +59,7 @@ struct vh_context {
   hipStream_t stream;
   int num_cus;
   // timing of regions with events on `stream`
-  int prof_on;
+  int prof_on;                          // mask of the stages timed
   std::vector<vh_prof_rec> prof_open;   // recorded, not yet read
   std::vector<hipEvent_t> prof_pool;    // free events
 };

@@ -257,6 +257,14 @@ class DeviceAccelerator:
         _lib.call("vh_verlet", self.handle, float(dt), float(mass), float(rcmax), d.ctypes.data)
         return {"kinetic": d[0], "escaped": d[1], "angular": (d[2], d[3], d[4])}
 
+    def step(self, fine: DeviceMesh, coarse: DeviceMesh, weight: float, dt: float, mass: float,
+             rcmax: float, jel: Jellium):
+        """One resident step in one native call (vh_step): deposits, nested
+        Poisson, coefficients, mean field, forces with the device projectile
+        (or none), its step, push.  Nothing waits for the device."""
+        _lib.call("vh_step", self.handle, fine.handle, coarse.handle, float(weight), float(dt),
+                  float(mass), float(rcmax), float(jel.nions), float(jel.radius))
+
     # --- the projectile kept on the device -----------------------------------
     def set_projectile(self, proj: Projectile, mode: str, jel: Jellium):
         """Places the projectile's state on the device; from then on

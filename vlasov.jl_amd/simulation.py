@@ -189,6 +189,14 @@ class Simulation:
         push, the kinetic energy of the push, and the total interaction on the
         step's total potential at the pushed positions (the host path's
         order, src/simulation.jl:309-351).  Without, returns None."""
+        if not energy and (self._proj_on_device or self._projectile is None):
+            # nothing comes back to the host: the whole step in one native call
+            self.acc.step(self.dmf, self.dmc, self.weight, self.params.dt, self.mass,
+                          self.params.rcmax, self.jellium)
+            self._proj_stale = self._proj_on_device
+            self.diagnostics = None
+            self.step_count += 1
+            return None
         hartree = self.update_forces(advance=True, energy=energy)
         self.diagnostics = self.acc.verlet(self.params.dt, self.mass, self.params.rcmax,
                                            diagnostics=energy)
