@@ -130,8 +130,13 @@ int vh_accel_destroy(vh_accel *a);
  *                       first stage on the matrix cores, summed in another
  *                       order than contract_spline_10 (FP32-rounding
  *                       agreement); 1: the reference's exact FMA order
- *                       (src/fields.jl:305-326), bit-identical to it. */
-enum { VH_OPT_EXACT_FIELD = 0 };
+ *                       (src/fields.jl:305-326), bit-identical to it.
+ *   VH_OPT_STEP_GRAPH   0 (default): vh_step launches its kernels one by
+ *                       one; 1: vh_step records them once into a HIP graph
+ *                       per particle buffer and replays it (recorded again
+ *                       when an argument changes; eager while stage timing
+ *                       is on). */
+enum { VH_OPT_EXACT_FIELD = 0, VH_OPT_STEP_GRAPH = 1 };
 int vh_accel_set_option(vh_accel *a, int option, int value);
 
 /* upload!/download! of the particle buffer (src/accelerator.jl:48, 64-68):

@@ -412,3 +412,33 @@ def test_native_step_matches_separate_calls(config1):
     finally:
         a.close()
         b.close()
+
+
+def test_step_graph_replays_match_eager(config1):
+    """vh_step recorded into HIP graphs (one per particle buffer) and
+    replayed: every replay gives the cloud of the eager step to the bit, with
+    host reads, an energy step (the separate entry points) and a projectile
+    read between the replays."""
+    p, fine, coarse, sm = config1
+    n = p.nparticles
+    w = p.nelectrons / n
+    pos, mom = va.sample_thomas_fermi(va.read_radial_profile(), n, w, seed=-9)
+    sims = []
+    try:
+        for graph in (True, False):
+            proj = va.xenon_projectile(x0=-20.0, impact=10.0)
+            s = va.Simulation(p, None, projectile=proj, tables=(fine, coarse, sm),
+                              positions=pos, momenta=mom)
+            s.acc.set_step_graph(graph)
+            sims.append(s)
+        for i in range(7):
+            energy = i == 3
+            for s in sims:
+                s.step(energy=energy)
+            a, b = (s.acc.download() for s in sims)
+            assert a.tobytes() == b.tobytes(), f"step {i}"
+            if i % 2:
+                assert sims[0].projectile.position == sims[1].projectile.position
+    finally:
+        for s in sims:
+            s.close()

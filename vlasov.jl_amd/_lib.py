@@ -16,6 +16,7 @@ VH_GRID_RHO, VH_GRID_PHI, VH_GRID_CSOL = 0, 1, 2
 VH_PROF_ALL = -1
 VH_SOFT_DEVICE, VH_SOFT_BALL = 0, 1
 VH_OPT_EXACT_FIELD = 0
+VH_OPT_STEP_GRAPH = 1
 STAGES = ("sort", "deposit_fine", "deposit_cic", "poisson", "spline", "meanfield", "field",
           "verlet", "field_kernel", "deposit_kernel")
 

@@ -275,10 +275,13 @@ extern "C" int vh_mesh_grid_upload(vh_mesh *m, int which, const float *host) {
 // ---------------------------------------------------------------------------
 // accelerator
 // ---------------------------------------------------------------------------
+static void drop_graphs(vh_accel *a);
+
 extern "C" int vh_accel_destroy(vh_accel *a) {
   if (!a) return VH_OK;
   hipSetDevice(a->ctx->device);
   hipStreamSynchronize(a->ctx->stream);
+  drop_graphs(a);
   void *ptrs[] = {a->ovl, a->grad, a->nodes, a->og, a->ovl64, a->knots64, a->parts[0], a->parts[1], a->force,
                   a->keys[0], a->keys[1], a->vals[0], a->vals[1], a->counts, a->scan_tmp,
                   a->occupied, a->bounds, a->cellmap, a->fitlist, a->nonfit, a->nonfit_prefix,
@@ -409,6 +412,10 @@ extern "C" int vh_accel_set_option(vh_accel *a, int option, int value) {
     case VH_OPT_EXACT_FIELD:
       VH_REQUIRE(value == 0 || value == 1, "vh_accel_set_option: exact field takes 0 or 1");
       a->exact_field = value;
+      return VH_OK;
+    case VH_OPT_STEP_GRAPH:
+      VH_REQUIRE(value == 0 || value == 1, "vh_accel_set_option: step graph takes 0 or 1");
+      a->step_graph = value;
       return VH_OK;
     default:
       VH_REQUIRE(false, "vh_accel_set_option: unknown option %d", option);
@@ -871,11 +878,68 @@ extern "C" int vh_projectile_step(vh_accel *a, double dt) {
   return vh_projectile_step_impl(a, dt);
 }
 
+static void drop_graphs(vh_accel *a) {
+  for (int b = 0; b < 2; ++b)
+    if (a->graph[b]) {
+      hipGraphExecDestroy(a->graph[b]);
+      a->graph[b] = nullptr;
+    }
+}
+
+static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, double dt,
+                      double mass, double rcmax, double jel_nions, double jel_radius);
+
 extern "C" int vh_step(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, double dt,
                        double mass, double rcmax, double jel_nions, double jel_radius) {
   VH_REQUIRE(a && fine && coarse, "vh_step: null argument");
   VH_REQUIRE(std::isfinite(weight) && dt > 0 && mass > 0 && jel_radius > 0,
              "vh_step: weight finite, dt, mass and jellium radius positive");
+  if (!a->step_graph || a->ctx->prof_on)
+    return step_eager(a, fine, coarse, weight, dt, mass, rcmax, jel_nions, jel_radius);
+  // the launch sequence depends only on these and on which buffer is current
+  const double key[VH_GRAPH_KEY] = {weight, dt, mass, rcmax, jel_nions, jel_radius,
+                                    (double)(uintptr_t)fine, (double)(uintptr_t)coarse,
+                                    (double)a->has_dproj, (double)a->exact_field, a->proj_charge,
+                                    a->proj_scale, (double)a->proj_soft_mode, a->proj_ball_radius};
+  if (memcmp(key, a->graph_key, sizeof key) != 0) {
+    drop_graphs(a);
+    memcpy(a->graph_key, key, sizeof key);
+  }
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  hipStream_t st = a->ctx->stream;
+  const int b = a->cur;
+  if (!a->graph[b]) {
+    VH_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int r = step_eager(a, fine, coarse, weight, dt, mass, rcmax, jel_nions, jel_radius);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (r != VH_OK) {
+      if (g) hipGraphDestroy(g);
+      return r;
+    }
+    if (e != hipSuccess) {
+      vh_set_error("vh_step: graph capture: %s", hipGetErrorString(e));
+      return VH_EHIP;
+    }
+    const hipError_t ei = hipGraphInstantiate(&a->graph[b], g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      a->graph[b] = nullptr;
+      vh_set_error("vh_step: graph instantiation: %s", hipGetErrorString(ei));
+      return VH_EHIP;
+    }
+    // recording left the host state of a step behind (current buffer
+    // swapped); the device work runs now
+    VH_CHECK_HIP(hipGraphLaunch(a->graph[b], st));
+    return VH_OK;
+  }
+  VH_CHECK_HIP(hipGraphLaunch(a->graph[b], st));
+  a->cur = 1 - b;  // what the recorded sort did on the host
+  return VH_OK;
+}
+
+static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, double dt,
+                      double mass, double rcmax, double jel_nions, double jel_radius) {
   // _update_forces_resident! (src/simulation.jl:472-531), then step!
   VH_TRY(vh_deposit_smoothed(a, fine, weight, nullptr));
   VH_TRY(vh_deposit_cic(a, coarse, weight));

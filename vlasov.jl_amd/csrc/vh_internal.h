@@ -167,6 +167,8 @@ struct vh_unit {
   int32_t g;     // occupied index of the cell
 };
 
+#define VH_GRAPH_KEY 14
+
 struct vh_accel {
   vh_context *ctx;
   int64_t npart, npad;
@@ -213,6 +215,9 @@ struct vh_accel {
   uint32_t *cellscratch;       // 4 x (maxocc + 2) scan scratch of the cell classification
   int sorted_once;             // host flag: the lists of a sort exist
   int exact_field;             // VH_OPT_EXACT_FIELD: the reference's FMA order
+  int step_graph;              // VH_OPT_STEP_GRAPH
+  hipGraphExec_t graph[2];     // vh_step recorded with parts[cur] = parts[0], parts[1]
+  double graph_key[14];        // the arguments they were recorded with (VH_GRAPH_KEY)
   // the projectile kept on the device (vh_projectile_set)
   double *dproj;               // 16 doubles, see k_projectile_step
   int has_dproj;

@@ -163,6 +163,11 @@ class DeviceAccelerator:
         first stage on the matrix cores."""
         _lib.call("vh_accel_set_option", self.handle, _lib.VH_OPT_EXACT_FIELD, 1 if exact else 0)
 
+    def set_step_graph(self, on: bool = True):
+        """vh_step recorded once into a HIP graph per particle buffer and
+        replayed (VH_OPT_STEP_GRAPH)."""
+        _lib.call("vh_accel_set_option", self.handle, _lib.VH_OPT_STEP_GRAPH, 1 if on else 0)
+
     # --- buffers -------------------------------------------------------------
     def upload(self, records: np.ndarray):
         r = np.ascontiguousarray(records)
