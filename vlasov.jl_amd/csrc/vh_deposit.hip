@@ -18,7 +18,8 @@
 // order-independent.
 #include "vh_internal.h"
 
-#define DEP_STAGE 32
+#define DEP_STAGE 64   // particles staged at a time (one per lane)
+#define DEP_ROW 25     // LDS row of a particle: gx, gy, gz (8 each), odd stride
 
 // table column of src/kernels.jl:770-772, FP32, 1-based, clamped
 __device__ __forceinline__ int table_col(float d, float half, float spacing, float nbdt, int ncol) {
@@ -29,19 +30,22 @@ __device__ __forceinline__ int table_col(float d, float half, float spacing, flo
 
 typedef float vh_f4d __attribute__((ext_vector_type(4)));
 
-// The cell's 8^3 sums as a rank-P update on the matrix cores:
+// The unit's 8^3 sums as a rank-P update on the matrix cores:
 //   C[kk][ii + 8 jj] = sum_p gz[p][kk] * (gx[p][ii] gy[p][jj])
 // four particles per v_mfma_f32_16x16x4_f32 (rows kk, 8 of 16 used), four
-// 16-column tiles for the 64 (ii, jj) pairs.  Node values of 32 particles at
-// a time are staged in LDS; missing particles of a group contribute zeros.
+// 16-column tiles for the 64 (ii, jj) pairs.  The node values of 64
+// particles at a time (one per lane) are staged in LDS, rows of missing
+// particles zero, so that every operand read is unconditional; the next
+// chunk's offsets are loaded while the current chunk is multiplied.
 __global__ void __launch_bounds__(64)
     k_dep_tiles(const int32_t *__restrict__ P, int64_t npad, const float *__restrict__ nodes,
                 const vh_unit *__restrict__ units, const int32_t *__restrict__ cnt, int nk, int n,
                 float spacing, float nbdt, int ncol, float *__restrict__ tiles) {
-  __shared__ float vals[24 * DEP_STAGE];
+  __shared__ float vals[DEP_ROW * DEP_STAGE];
   const int l = threadIdx.x;
   const int row = l & 15, kq = l >> 4;       // A: row kk, particle kq of the group
   const int ii = l & 7, jh = (l >> 3) & 1;   // B: column l & 15 -> (ii, jj = 2 t + jh)
+  const float amask = row < 8 ? 1.f : 0.f;   // rows 8..15 of A are padding
   const float half = spacing * 0.5f;
   const int nunits = cnt[CNT_NUNITS];
   for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
@@ -54,34 +58,40 @@ __global__ void __launch_bounds__(64)
     vh_f4d acc[4];
 #pragma unroll
     for (int tt = 0; tt < 4; ++tt) acc[tt] = (vh_f4d){0.f, 0.f, 0.f, 0.f};
+    float dx, dy, dz;
+    vh_load_delta(P, lo + l < hi ? lo + l : lo, dx, dy, dz);
     for (int cur = lo; cur < hi; cur += DEP_STAGE) {
       const int chunk = min(DEP_STAGE, hi - cur);
-      __syncthreads();
-      if (l < chunk) {
-        const int64_t s = cur + l;
-        float dx, dy, dz;
-        vh_load_delta(P, s, dx, dy, dz);
-        const int c1 = table_col(dx, half, spacing, nbdt, ncol) - 1;
-        const int c2 = table_col(dy, half, spacing, nbdt, ncol) - 1;
-        const int c3 = table_col(dz, half, spacing, nbdt, ncol) - 1;
-        float *v = vals + 24 * l;
+      const float live = l < chunk ? 1.f : 0.f;
+      const int c1 = table_col(dx, half, spacing, nbdt, ncol) - 1;
+      const int c2 = table_col(dy, half, spacing, nbdt, ncol) - 1;
+      const int c3 = table_col(dz, half, spacing, nbdt, ncol) - 1;
+      const float4 *n1 = (const float4 *)(nodes + c1 * 8), *n2 = (const float4 *)(nodes + c2 * 8),
+                   *n3 = (const float4 *)(nodes + c3 * 8);
+      const float4 g[6] = {n1[0], n1[1], n2[0], n2[1], n3[0], n3[1]};
+      {  // the next chunk's offsets, in flight during this chunk's products
+        const int sn = cur + DEP_STAGE + l;
+        vh_load_delta(P, sn < hi ? sn : lo, dx, dy, dz);
+      }
+      __syncthreads();   // the previous chunk's products have read vals
+      float *v = vals + DEP_ROW * l;
 #pragma unroll
-        for (int a = 0; a < 8; ++a) {
-          v[a] = nodes[c1 * 8 + a];
-          v[8 + a] = nodes[c2 * 8 + a];
-          v[16 + a] = nodes[c3 * 8 + a];
-        }
+      for (int a = 0; a < 6; ++a) {
+        v[4 * a] = g[a].x * live;
+        v[4 * a + 1] = g[a].y * live;
+        v[4 * a + 2] = g[a].z * live;
+        v[4 * a + 3] = g[a].w * live;
       }
       __syncthreads();
-      for (int q0 = 0; q0 < chunk; q0 += 4) {
-        const int p = q0 + kq;
-        const bool live = p < chunk;
-        const float *v = vals + 24 * (live ? p : 0);
-        const float a = (live && row < 8) ? v[16 + row] : 0.f;
-        const float gxv = live ? v[ii] : 0.f;
+      const int ngroups = (chunk + 3) >> 2;
+#pragma unroll 2
+      for (int q = 0; q < ngroups; ++q) {
+        const float *vp = vals + DEP_ROW * (4 * q + kq);
+        const float a = vp[16 + (row & 7)] * amask;
+        const float gxv = vp[ii];
 #pragma unroll
         for (int tt = 0; tt < 4; ++tt) {
-          const float b = gxv * v[8 + 2 * tt + jh];
+          const float b = gxv * vp[8 + 2 * tt + jh];
           acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[tt], 0, 0, 0);
         }
       }
