@@ -329,43 +329,46 @@ __global__ void __launch_bounds__(64 * CIC_WAVES)
     }
     const bool anchored = anc[0] > 0 && anc[1] > 0 && anc[2] > 0;
     long long sum[4] = {0, 0, 0, 0};
+    vh_pos rn = vh_load_pos(P, un.lo + l < un.hi ? un.lo + l : un.lo);
     for (int c0 = un.lo; c0 < un.hi; c0 += 64) {
-      const int s = c0 + l;
+      const bool live = c0 + l < un.hi;
+      const vh_pos r = rn;
+      {  // the next chunk's records, in flight during this chunk's products
+        const int sn = c0 + 64 + l;
+        rn = vh_load_pos(P, sn < un.hi ? sn : un.lo);
+      }
       float X[4] = {0.f, 0.f, 0.f, 0.f}, Y[4] = {0.f, 0.f, 0.f, 0.f}, Z[4] = {0.f, 0.f, 0.f, 0.f};
-      if (s < un.hi) {
-        int ix[3];
-        float ax[3];
-        bool inside = true;
-        const vh_pos r = vh_load_pos(P, s);
+      int ix[3];
+      float ax[3];
+      bool inside = live;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const float p = x0f + (float)(r.k[d] - 1) * hf + r.d[d];
-          ix[d] = locate_cell(gt, n, x0t, iwt, cells, nbin, p, ax[d]);
-          inside = inside && ix[d] > 0;
-        }
-        if (inside) {
-          const int q0 = ix[0] - anc[0], q1 = ix[1] - anc[1], q2 = ix[2] - anc[2];
-          const bool fits = anchored && q0 >= 0 && q0 <= 2 && q1 >= 0 && q1 <= 2 && q2 >= 0 && q2 <= 2;
-          if (fits) {
+      for (int d = 0; d < 3; ++d) {
+        const float p = x0f + (float)(r.k[d] - 1) * hf + r.d[d];
+        ix[d] = locate_cell(gt, n, x0t, iwt, cells, nbin, p, ax[d]);
+        inside = inside && ix[d] > 0;
+      }
+      if (inside) {
+        const int q0 = ix[0] - anc[0], q1 = ix[1] - anc[1], q2 = ix[2] - anc[2];
+        const bool fits = anchored && q0 >= 0 && q0 <= 2 && q1 >= 0 && q1 <= 2 && q2 >= 0 && q2 <= 2;
+        if (fits) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              X[e] = e == q0 ? ax[0] : (e == q0 + 1 ? 1.f - ax[0] : 0.f);
-              Y[e] = e == q1 ? ax[1] : (e == q1 + 1 ? 1.f - ax[1] : 0.f);
-              Z[e] = e == q2 ? ax[2] : (e == q2 + 1 ? 1.f - ax[2] : 0.f);
-            }
-          } else {
-            const float bx = 1.f - ax[0], by = 1.f - ax[1], bz = 1.f - ax[2];
-            const int64_t i0 = ix[0] - 1, j0 = ix[1] - 1, k0 = ix[2] - 1;
-            const int64_t nn = n;
-            fx_add(acc, i0 + nn * (j0 + nn * k0), ax[0] * ax[1] * ax[2], scale);
-            fx_add(acc, i0 + nn * (j0 + nn * (k0 + 1)), ax[0] * ax[1] * bz, scale);
-            fx_add(acc, i0 + nn * ((j0 + 1) + nn * k0), ax[0] * by * ax[2], scale);
-            fx_add(acc, i0 + nn * ((j0 + 1) + nn * (k0 + 1)), ax[0] * by * bz, scale);
-            fx_add(acc, (i0 + 1) + nn * (j0 + nn * k0), bx * ax[1] * ax[2], scale);
-            fx_add(acc, (i0 + 1) + nn * (j0 + nn * (k0 + 1)), bx * ax[1] * bz, scale);
-            fx_add(acc, (i0 + 1) + nn * ((j0 + 1) + nn * k0), bx * by * ax[2], scale);
-            fx_add(acc, (i0 + 1) + nn * ((j0 + 1) + nn * (k0 + 1)), bx * by * bz, scale);
+          for (int e = 0; e < 4; ++e) {
+            X[e] = e == q0 ? ax[0] : (e == q0 + 1 ? 1.f - ax[0] : 0.f);
+            Y[e] = e == q1 ? ax[1] : (e == q1 + 1 ? 1.f - ax[1] : 0.f);
+            Z[e] = e == q2 ? ax[2] : (e == q2 + 1 ? 1.f - ax[2] : 0.f);
           }
+        } else {
+          const float bx = 1.f - ax[0], by = 1.f - ax[1], bz = 1.f - ax[2];
+          const int64_t i0 = ix[0] - 1, j0 = ix[1] - 1, k0 = ix[2] - 1;
+          const int64_t nn = n;
+          fx_add(acc, i0 + nn * (j0 + nn * k0), ax[0] * ax[1] * ax[2], scale);
+          fx_add(acc, i0 + nn * (j0 + nn * (k0 + 1)), ax[0] * ax[1] * bz, scale);
+          fx_add(acc, i0 + nn * ((j0 + 1) + nn * k0), ax[0] * by * ax[2], scale);
+          fx_add(acc, i0 + nn * ((j0 + 1) + nn * (k0 + 1)), ax[0] * by * bz, scale);
+          fx_add(acc, (i0 + 1) + nn * (j0 + nn * k0), bx * ax[1] * ax[2], scale);
+          fx_add(acc, (i0 + 1) + nn * (j0 + nn * (k0 + 1)), bx * ax[1] * bz, scale);
+          fx_add(acc, (i0 + 1) + nn * ((j0 + 1) + nn * k0), bx * by * ax[2], scale);
+          fx_add(acc, (i0 + 1) + nn * ((j0 + 1) + nn * (k0 + 1)), bx * by * bz, scale);
         }
       }
       // the particle's row: X[x] Y[y] at x + 4 y, then Z; the wave's own rows
@@ -380,17 +383,20 @@ __global__ void __launch_bounds__(64 * CIC_WAVES)
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-      const int np = min(64, un.hi - c0);
-      vh_f4d d = {0.f, 0.f, 0.f, 0.f};
-      for (int m = 0; 4 * m < np; ++m) {
-        const float *pr = st + CIC_STRIDE * (4 * m + kq);
-        const float av = pr[xy];
-        const float zv = pr[16 + zc];
-        const float bv = grp == kq ? zv : 0.f;
-        d = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, d, 0, 0, 0);
+      // groups of four particles, alternately into two accumulators (two
+      // independent product chains); rows past the chunk are zero
+      const int ng = (min(64, un.hi - c0) + 3) >> 2;
+      vh_f4d d0 = {0.f, 0.f, 0.f, 0.f}, d1 = {0.f, 0.f, 0.f, 0.f};
+      for (int m = 0; m < ng; m += 2) {
+        const float *pa = st + CIC_STRIDE * (4 * m + kq), *pb = pa + 4 * CIC_STRIDE;
+        const float a0 = pa[xy], z0 = pa[16 + zc];
+        const float a1 = pb[xy], z1 = pb[16 + zc];   // group m + 1 (zero rows past ng)
+        d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, grp == kq ? z0 : 0.f, d0, 0, 0, 0);
+        d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, grp == kq ? z1 : 0.f, d1, 0, 0, 0);
       }
 #pragma unroll
-      for (int v = 0; v < 4; ++v) sum[v] += (long long)rint((double)d[v] * scale);
+      for (int v = 0; v < 4; ++v)
+        sum[v] += (long long)rint((double)d0[v] * scale) + (long long)rint((double)d1[v] * scale);
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
