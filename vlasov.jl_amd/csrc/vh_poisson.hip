@@ -104,7 +104,22 @@ __global__ void __launch_bounds__(576)
   if (p + per < pend) fetch(R, p + per);
   int buf = 0;
   for (; p < pend; p += per, buf ^= 1) {
-    // the products of this panel while the next one's loads are in flight
+    // the products of this panel while the next one's loads are in flight;
+    // the epilogue's operands (the Hadamard factors, or the values to add
+    // to) are loaded first, so that their latency hides behind the products
+    const int64_t j0 = p * NC;
+    float pre[NCT][4];
+    if (EPI == EPI_HADAMARD || EPI == EPI_ACCUMULATE) {
+      const float *srcp = EPI == EPI_HADAMARD ? aux : out;
+      const int64_t last = (int64_t)n * m - 1;
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int64_t idx = (int64_t)(r * 16 + 4 * lk + v) * m + j0 + ct * 16 + lc;
+          pre[ct][v] = srcp[idx < last ? idx : last];   // clamped: never under a condition
+        }
+    }
     const float *B = Bs + buf * pbuf;
     vh_f4 acc[NCT];
 #pragma unroll
@@ -127,7 +142,6 @@ __global__ void __launch_bounds__(576)
     // the loads of the panel after next, issued before this panel's stores
     if (p + 2 * per < pend) fetch(R, p + 2 * per);
     if (r < mt) {
-      const int64_t j0 = p * NC;
 #pragma unroll
       for (int ct = 0; ct < NCT; ++ct) {
         const int64_t j = j0 + ct * 16 + lc;
@@ -141,13 +155,12 @@ __global__ void __launch_bounds__(576)
           if (EPI == EPI_STORE) {
             out[idx] = val;
           } else if (EPI == EPI_HADAMARD) {
-            out[idx] = val * aux[idx];
+            out[idx] = val * pre[ct][v];
           } else if (EPI == EPI_ACCUMULATE) {
-            out[idx] = out[idx] + val;
-          } else {  // EPI_FILL_INTERIOR: idx is (i, j, k) of the interior, i fastest
-            const int64_t nn = n;
-            const int64_t i = idx % nn, jj = (idx / nn) % nn, kz = idx / (nn * nn);
-            out[(i + 1) + nfull * ((jj + 1) + (int64_t)nfull * (kz + 1))] = val;
+            out[idx] = pre[ct][v] + val;
+          } else {  // EPI_FILL_INTERIOR: idx = i + n (jj + n kz) of the interior, kz = orow
+            const int jc = (int)j, i = jc % n, jj = jc / n;
+            out[(i + 1) + nfull * ((jj + 1) + (int64_t)nfull * (orow + 1))] = val;
           }
         }
       }
