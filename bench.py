@@ -107,12 +107,13 @@ def run_crossing(va, npart: int, nfine: int, mode: str, warmup: int, steps: int,
     nst = len(va._lib.STAGES)
     field_bit = 1 << va._lib.STAGES.index("field_kernel")
 
-    def read_profile():
+    def read_profile(nsteps):
+        """ms per step of every stage timed over the last `nsteps` steps"""
         ms = np.zeros(nst)
         cnt = np.zeros(nst, dtype=np.int64)
         va._lib.call("vh_profile_read", sim.ctx.handle, ms, cnt)
         va._lib.call("vh_profile_enable", sim.ctx.handle, 0)
-        return {s: (ms[i] / cnt[i] if cnt[i] else 0.0) for i, s in enumerate(va._lib.STAGES)}
+        return {s: ms[i] / nsteps for i, s in enumerate(va._lib.STAGES)}
 
     e_first = None
     stage_from = max(0, warmup - 20)
@@ -124,7 +125,7 @@ def run_crossing(va, npart: int, nfine: int, mode: str, warmup: int, steps: int,
         if b is not None:
             e_first = b
     sim.ctx.sync()
-    stages = read_profile() if profile else {}
+    stages = read_profile(warmup - stage_from) if profile else {}
     log(f"warm-up {warmup} steps done")
     if profile:
         va._lib.call("vh_profile_enable", sim.ctx.handle, field_bit)
@@ -139,7 +140,7 @@ def run_crossing(va, npart: int, nfine: int, mode: str, warmup: int, steps: int,
             log(f"step {warmup + i + 1}: x_ion = {sim.projectile.position[0]:.1f}")
     sim.ctx.sync()
     elapsed = time.perf_counter() - t1
-    field_ms = read_profile()["field_kernel"] if profile else 0.0
+    field_ms = read_profile(steps)["field_kernel"] if profile else 0.0
     q = sim.enclosed_charge(8.0)
     nocc, outcount = sim.acc.sort()
     out = dict(elapsed=elapsed, stages=stages, field_ms=field_ms, q=q, nocc=nocc,

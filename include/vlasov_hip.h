@@ -105,6 +105,11 @@ int vh_spline_coefficients(vh_mesh *m);
 /* effective_potential!(csol, rho, dm, jel): csol += S^-1^3 (V_xc(rho) +
  * V_jel(r)) (src/devicemesh.jl:171-183; src/meanfield.jl:116-147). */
 int vh_effective_potential(vh_mesh *m, double nions, double radius);
+/* spline_coefficients!(c, phi, dm) followed by effective_potential!(c, rho,
+ * dm, jel) on the same mesh (src/devicemesh.jl:126-127, 171-183) as one
+ * transform, csol = S^-1 (phi + V_xc(rho) + V_jel): three rotating products
+ * instead of six (the sums are rounded in another order: FP32 agreement). */
+int vh_potential_coefficients(vh_mesh *m, double nions, double radius);
 /* total_charge(rho, dm) in FP64 (src/devicemesh.jl:112-113). */
 int vh_total_charge(vh_mesh *m, double *q);
 /* multipole(rho, dm): q, center[3], quadrupole[6] in FP64
@@ -216,8 +221,9 @@ int vh_projectile_step(vh_accel *a, double dt);
 /* One step of the resident loop without host synchronisation, the sequence
  * of _update_forces_resident! and step! (src/simulation.jl:472-541; the
  * driver's nine calls): vh_deposit_smoothed, vh_deposit_cic,
- * vh_poisson_nested, vh_spline_coefficients (fine), vh_effective_potential
- * (both levels, jellium of jel_nions ions and radius jel_radius), the forces
+ * vh_poisson_nested, vh_potential_coefficients (fine: the spline
+ * coefficients and the mean field in one transform), vh_effective_potential
+ * (coarse; jellium of jel_nions ions and radius jel_radius), the forces
  * with the device projectile (vh_forces_resident + vh_projectile_step) or,
  * without vh_projectile_set, without projectile, and vh_verlet.  One call
  * from the host per step. */

@@ -180,6 +180,29 @@ def test_poisson_chain(setup, orc):
     assert np.max(np.abs(dmf.csol - want_eff)) / scale < REL
 
 
+def test_potential_coefficients_fused(setup, orc):
+    """csol = S^-1 (phi + V) in one transform against spline_coefficients
+    followed by effective_potential, and against the FP64 oracle."""
+    s = setup
+    acc, dmf, fine = s["acc"], s["dmf"], s["fine"]
+    acc.upload(s["rec"])
+    acc.deposit_smoothed(dmf, s["w"])
+    acc.deposit_cic(s["dmc"], s["w"])
+    va.poisson_nested(dmf, s["dmc"])
+    dmf.spline_coefficients()
+    hartree = dmf.csol
+    dmf.effective_potential(s["jel"])
+    separate = dmf.csol
+    dmf.potential_coefficients(s["jel"])
+    fused = dmf.csol
+    rho = dmf.rho.astype(np.float64)
+    extra = orc.spline_coefficients(orc.effective_extra(rho, fine, s["jel"]), fine)
+    want = orc.spline_coefficients(dmf.phi.astype(np.float64), fine) + extra
+    scale = max(np.max(np.abs(hartree)), np.max(np.abs(extra)))
+    assert np.max(np.abs(fused - want)) / scale < REL
+    assert np.max(np.abs(fused - separate)) / scale < REL
+
+
 def test_spline_coefficients_mfma_layout(setup, orc):
     """The rotating product on a random grid, FP32 MFMA vs FP64."""
     s = setup

@@ -68,6 +68,7 @@ _SIGNATURES = {
     "vh_projectile_get": [_vp, _f64p],
     "vh_forces_resident": [_vp, _vp, _vp, C.c_double],
     "vh_projectile_step": [_vp, C.c_double],
+    "vh_potential_coefficients": [_vp, C.c_double, C.c_double],
     "vh_step": [_vp, _vp, _vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                 C.c_double],
     "vh_interaction_energy": [_vp, _vp, _vp, C.c_double, C.c_double, C.POINTER(C.c_double)],

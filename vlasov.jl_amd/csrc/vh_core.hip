@@ -761,6 +761,15 @@ extern "C" int vh_effective_potential(vh_mesh *m, double nions, double radius) {
   return VH_OK;
 }
 
+extern "C" int vh_potential_coefficients(vh_mesh *m, double nions, double radius) {
+  VH_REQUIRE(m != nullptr, "vh_potential_coefficients: null mesh");
+  VH_REQUIRE(radius > 0, "vh_potential_coefficients: jellium radius must be positive");
+  VH_CHECK_HIP(hipSetDevice(m->ctx->device));
+  vh_region r(m->ctx, VH_ST_SPLINE);
+  VH_TRY(vh_potential_coefficients_impl(m, nions, radius));
+  return VH_OK;
+}
+
 extern "C" int vh_total_charge(vh_mesh *m, double *q) {
   VH_REQUIRE(m && q, "vh_total_charge: null argument");
   VH_CHECK_HIP(hipSetDevice(m->ctx->device));
@@ -944,8 +953,7 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
   VH_TRY(vh_deposit_smoothed(a, fine, weight, nullptr));
   VH_TRY(vh_deposit_cic(a, coarse, weight));
   VH_TRY(vh_poisson_nested(fine, coarse));
-  VH_TRY(vh_spline_coefficients(fine));
-  VH_TRY(vh_effective_potential(fine, jel_nions, jel_radius));
+  VH_TRY(vh_potential_coefficients(fine, jel_nions, jel_radius));
   VH_TRY(vh_effective_potential(coarse, jel_nions, jel_radius));
   if (a->has_dproj) {
     VH_TRY(vh_forces_resident(a, fine, coarse, weight));

@@ -280,6 +280,7 @@ int vh_poisson_single_impl(vh_mesh *m);
 int vh_poisson_nested_impl(vh_mesh *fine, vh_mesh *coarse);
 int vh_spline_coefficients_impl(vh_mesh *m, const float *src, float *dst);
 int vh_effective_potential_impl(vh_mesh *m, double nions, double radius);
+int vh_potential_coefficients_impl(vh_mesh *m, double nions, double radius);
 int vh_total_charge_impl(vh_mesh *m, const float *rho, double *dq);
 int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, const double *proj,
                    int soft_mode, double ball_radius, const double *dproj);

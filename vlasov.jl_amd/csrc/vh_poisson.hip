@@ -579,7 +579,8 @@ int vh_poisson_nested_impl(vh_mesh *fine, vh_mesh *coarse) {
 // mean field (src/meanfield.jl:116-147): V_xc(rho) + V_jel(r) at the points
 // ---------------------------------------------------------------------------
 __global__ void k_effective(float *__restrict__ extra, const float *__restrict__ rho,
-                            const float *__restrict__ g, int n, float nions, float radius) {
+                            const float *__restrict__ g, int n, float nions, float radius,
+                            const float *__restrict__ add) {
   const float xc_ex = (float)(-0.98474502184269641);          // -cbrt(3/pi)
   const float xc_co = (float)(-0.0333);
   const float xc_sc = (float)(11.4 * 1.6119919540164696);     // 11.4 cbrt(4 pi / 3)
@@ -592,17 +593,28 @@ __global__ void k_effective(float *__restrict__ extra, const float *__restrict__
     const float vxc = xc_ex * cr + xc_co * log1pf(xc_sc * cr);
     const float rr = r / radius;
     const float vj = r < radius ? -nions * (3.0f - rr * rr) / (2.0f * radius) : -nions / r;
-    extra[t] = vxc + vj;
+    extra[t] = add ? add[t] + (vxc + vj) : vxc + vj;
   }
 }
 
 int vh_effective_potential_impl(vh_mesh *m, double nions, double radius) {
   vh_context *c = m->ctx;
   hipLaunchKernelGGL(k_effective, dim3(2048), dim3(256), 0, c->stream, m->extra, m->rho, m->colloc,
-                     m->n, (float)nions, (float)radius);
+                     m->n, (float)nions, (float)radius, (const float *)nullptr);
   VH_LAUNCH_CHECK();
   // extra -> work -> extra -> csol (+=)
   VH_TRY(apply_all(c, m->Sinv_op, m->n, m->extra, m->work, m->extra, m->csol, EPI_ACCUMULATE,
                    nullptr, 0));
+  return VH_OK;
+}
+
+// spline_coefficients! then effective_potential! as one transform:
+// csol = S^-1 (phi + V_xc(rho) + V_jel), three products instead of six
+int vh_potential_coefficients_impl(vh_mesh *m, double nions, double radius) {
+  vh_context *c = m->ctx;
+  hipLaunchKernelGGL(k_effective, dim3(2048), dim3(256), 0, c->stream, m->extra, m->rho, m->colloc,
+                     m->n, (float)nions, (float)radius, (const float *)m->phi);
+  VH_LAUNCH_CHECK();
+  VH_TRY(apply_all(c, m->Sinv_op, m->n, m->extra, m->work, m->extra, m->csol, EPI_STORE, nullptr, 0));
   return VH_OK;
 }

@@ -109,6 +109,11 @@ class DeviceMesh:
         """csol += S^-1 (V_xc(rho) + V_jel) (src/devicemesh.jl:171-183)."""
         _lib.call("vh_effective_potential", self.handle, float(jel.nions), float(jel.radius))
 
+    def potential_coefficients(self, jel: Jellium):
+        """spline_coefficients then effective_potential as one transform,
+        csol = S^-1 (phi + V_xc(rho) + V_jel): three products instead of six."""
+        _lib.call("vh_potential_coefficients", self.handle, float(jel.nions), float(jel.radius))
+
     def total_charge(self) -> float:
         q = C.c_double()
         _lib.call("vh_total_charge", self.handle, C.byref(q))

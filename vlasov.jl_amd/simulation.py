@@ -153,9 +153,14 @@ class Simulation:
             self.nout = nout
         self.acc.deposit_cic(self.dmc, w)
         poisson_nested(self.dmf, self.dmc)        # leaves csol_c = S^-1 phi_c
-        self.dmf.spline_coefficients()
-        hartree = self.acc.interaction_energy(self.dmf, self.dmc, w) / 2 if energy else None
-        self.dmf.effective_potential(self.jellium)
+        hartree = None
+        if energy:
+            # the Hartree energy needs the coefficients of phi alone
+            self.dmf.spline_coefficients()
+            hartree = self.acc.interaction_energy(self.dmf, self.dmc, w) / 2
+            self.dmf.effective_potential(self.jellium)
+        else:
+            self.dmf.potential_coefficients(self.jellium)
         self.dmc.effective_potential(self.jellium)
         if self._proj_on_device:
             self.acc.forces_resident(self.dmf, self.dmc, w)
