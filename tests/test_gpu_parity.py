@@ -313,6 +313,13 @@ def test_verlet_bit_exact(setup, orc):
     wd = orc.verlet_f32(want, f, 0.5, s["w"], s["h"], s["x0"], 100.0)
     assert got.tobytes() == want.tobytes()
     assert np.allclose([d["kinetic"], d["escaped"], *d["angular"]], wd, rtol=1e-10, atol=1e-12)
+    # the push leaves the cell keys and first-digit histograms of the moved
+    # records; the sort that starts from them is the stable sort to the bit
+    nocc, nout = acc.sort()
+    perm, occ, bnd = orc.cellsort(want, s["fine"].axis.nknots)
+    assert acc.download().tobytes() == want[perm].tobytes()
+    occ_d, bnd_d, out_d = acc.sort_lists(nocc, nout)
+    assert np.array_equal(occ_d, occ + 1) and np.array_equal(bnd_d, bnd)
 
 
 def test_enclosed_count(setup, orc):

@@ -475,6 +475,7 @@ extern "C" int vh_particles_upload(vh_accel *a, const void *records) {
   VH_REQUIRE(a && records, "vh_particles_upload: null argument");
   VH_CHECK_HIP(hipSetDevice(a->ctx->device));
   int32_t *stage = a->parts[1 - a->cur];
+  a->keys_fresh = 0;  // new records: the keys of the last push are stale
   VH_CHECK_HIP(hipMemcpyAsync(stage, records, (size_t)a->npart * 48, hipMemcpyHostToDevice,
                               a->ctx->stream));
   hipLaunchKernelGGL(k_records_in, dim3(2048), dim3(256), 0, a->ctx->stream, stage,
@@ -909,7 +910,8 @@ extern "C" int vh_step(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weigh
   const double key[VH_GRAPH_KEY] = {weight, dt, mass, rcmax, jel_nions, jel_radius,
                                     (double)(uintptr_t)fine, (double)(uintptr_t)coarse,
                                     (double)a->has_dproj, (double)a->exact_field, a->proj_charge,
-                                    a->proj_scale, (double)a->proj_soft_mode, a->proj_ball_radius};
+                                    a->proj_scale, (double)a->proj_soft_mode, a->proj_ball_radius,
+                                    (double)a->keys_fresh};
   if (memcmp(key, a->graph_key, sizeof key) != 0) {
     drop_graphs(a);
     memcpy(a->graph_key, key, sizeof key);
@@ -943,7 +945,8 @@ extern "C" int vh_step(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weigh
     return VH_OK;
   }
   VH_CHECK_HIP(hipGraphLaunch(a->graph[b], st));
-  a->cur = 1 - b;  // what the recorded sort did on the host
+  a->cur = 1 - b;  // what the recorded sort and push did on the host
+  a->keys_fresh = vh_verlet_tiles(a) ? 1 : 0;
   return VH_OK;
 }
 

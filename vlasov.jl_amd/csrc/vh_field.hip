@@ -895,44 +895,74 @@ __device__ __forceinline__ void verlet1(int kq, float dq, int ok, float od, floa
   gap = (float)(kn - ok) * h + (dn - od);
 }
 
+// KEYS: the push also writes each particle's new cell key to keys[] and
+// counts it in the histogram of the first radix digit of its sort tile
+// (4096 consecutive slots), counts[d nblocks + tile], zero on entry, so that
+// the next sort starts from them.  A wave's 64 slots lie in one tile and,
+// the slots being in cell order, mostly share a digit: each round adds the
+// lanes that share the digit of the first lane left with one atomic.
+template <bool KEYS>
 __global__ void __launch_bounds__(256)
     k_verlet(int32_t *__restrict__ P, int64_t npad, int64_t n, const float *__restrict__ force,
              float a, float h, float x0, float pfac, float r2max, float inv2M,
-             double *__restrict__ part) {
+             double *__restrict__ part, uint32_t *__restrict__ keys, uint32_t *__restrict__ counts,
+             int nk, int nbins0, int64_t nblocks) {
   __shared__ double lds[8];
   int32_t *Q = vh_prev_half(P, npad);
   double ek = 0.0, eo = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const vh_pos cur = vh_load_pos(P, i), old = vh_load_pos(Q, i);
-    const int k1 = cur.k[0], k2 = cur.k[1], k3 = cur.k[2];
-    const float d1 = cur.d[0], d2 = cur.d[1], d3 = cur.d[2];
-    const int o1 = old.k[0], o2 = old.k[1], o3 = old.k[2];
-    const float e1 = old.d[0], e2 = old.d[1], e3 = old.d[2];
-    int n1, n2, n3;
-    float m1, m2, m3, g1, g2, g3;
-    verlet1(k1, d1, o1, e1, a, force[i], h, n1, m1, g1);
-    verlet1(k2, d2, o2, e2, a, force[npad + i], h, n2, m2, g2);
-    verlet1(k3, d3, o3, e3, a, force[2 * npad + i], h, n3, m3, g3);
-    const float px = g1 * pfac, py = g2 * pfac, pz = g3 * pfac;
-    const float e = (px * px + py * py + pz * pz) * inv2M;
-    ek += e;
-    const float qx = x0 + (float)(k1 - 1) * h + d1;
-    const float qy = x0 + (float)(k2 - 1) * h + d2;
-    const float qz = x0 + (float)(k3 - 1) * h + d3;
-    if (qx * qx + qy * qy + qz * qz > r2max) eo += e;
-    lx += qy * pz - qz * py;
-    ly += qz * px - qx * pz;
-    lz += qx * py - qy * px;
-    vh_pos nw;
-    nw.k[0] = n1;
-    nw.k[1] = n2;
-    nw.k[2] = n3;
-    nw.d[0] = m1;
-    nw.d[1] = m2;
-    nw.d[2] = m3;
-    vh_store_pos(P, i, nw);
-    vh_store_pos(Q, i, cur);
+  const uint32_t mask = (uint32_t)(nbins0 - 1);
+  // whole waves walk the slots, so that a wave's lanes stay in one tile
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t nw = (n + 63) / 64 * 64;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nw; i += stride) {
+    const bool live = i < n;
+    uint32_t key = 0u;
+    if (live) {
+      const vh_pos cur = vh_load_pos(P, i), old = vh_load_pos(Q, i);
+      const int k1 = cur.k[0], k2 = cur.k[1], k3 = cur.k[2];
+      const float d1 = cur.d[0], d2 = cur.d[1], d3 = cur.d[2];
+      const int o1 = old.k[0], o2 = old.k[1], o3 = old.k[2];
+      const float e1 = old.d[0], e2 = old.d[1], e3 = old.d[2];
+      int n1, n2, n3;
+      float m1, m2, m3, g1, g2, g3;
+      verlet1(k1, d1, o1, e1, a, force[i], h, n1, m1, g1);
+      verlet1(k2, d2, o2, e2, a, force[npad + i], h, n2, m2, g2);
+      verlet1(k3, d3, o3, e3, a, force[2 * npad + i], h, n3, m3, g3);
+      const float px = g1 * pfac, py = g2 * pfac, pz = g3 * pfac;
+      const float e = (px * px + py * py + pz * pz) * inv2M;
+      ek += e;
+      const float qx = x0 + (float)(k1 - 1) * h + d1;
+      const float qy = x0 + (float)(k2 - 1) * h + d2;
+      const float qz = x0 + (float)(k3 - 1) * h + d3;
+      if (qx * qx + qy * qy + qz * qz > r2max) eo += e;
+      lx += qy * pz - qz * py;
+      ly += qz * px - qx * pz;
+      lz += qx * py - qy * px;
+      vh_pos nw_;
+      nw_.k[0] = n1;
+      nw_.k[1] = n2;
+      nw_.k[2] = n3;
+      nw_.d[0] = m1;
+      nw_.d[1] = m2;
+      nw_.d[2] = m3;
+      vh_store_pos(P, i, nw_);
+      vh_store_pos(Q, i, cur);
+      key = cell_key0(n1, n2, n3, nk);
+      if (KEYS) keys[i] = key;
+    }
+    if (KEYS) {
+      const uint32_t dg = key & mask;
+      uint32_t *col = counts + (i >> 12);   // this wave's tile
+      uint64_t left = __ballot(live);
+      for (int it = 0; it < 4 && left; ++it) {
+        const int lead = __ffsll((unsigned long long)left) - 1;
+        const uint32_t d0 = __shfl(dg, lead, 64);
+        const uint64_t same = __ballot(live && dg == d0) & left;
+        if ((threadIdx.x & 63) == lead) atomicAdd(col + (int64_t)d0 * nblocks, (uint32_t)__popcll(same));
+        left &= ~same;
+      }
+      if (live && (left >> (threadIdx.x & 63) & 1ull)) atomicAdd(col + (int64_t)dg * nblocks, 1u);
+    }
   }
   double vals[5] = {ek, eo, lx, ly, lz};
   for (int v = 0; v < 5; ++v) {
@@ -950,11 +980,24 @@ int vh_verlet_impl(vh_accel *a, double dt, double mass, double rcmax) {
   const float pfac = (float)(mass / (2.0 * dt));
   const float r2max = (float)(rcmax * rcmax);
   const float inv2M = (float)(1.0 / (2.0 * mass));
-  hipLaunchKernelGGL(k_verlet, dim3(VERLET_BLOCKS), dim3(256), 0, c->stream, a->parts[a->cur],
-                     a->npad, a->npart, a->force, fa, fh, fx0, pfac, r2max, inv2M, a->dpart);
+  // the first radix digit, as the sort takes it
+  int bits0 = a->digit_bits < a->bits_total ? a->digit_bits : a->bits_total;
+  if (bits0 < 1) bits0 = 1;
+  const int grid = VERLET_BLOCKS;
+  if (vh_verlet_tiles(a)) {
+    VH_TRY(vh_fill_i32(c, (int32_t *)a->counts, (1 << bits0) * a->nblocks, 0));
+    hipLaunchKernelGGL(k_verlet<true>, dim3(grid), dim3(256), 0, c->stream, a->parts[a->cur],
+                       a->npad, a->npart, a->force, fa, fh, fx0, pfac, r2max, inv2M, a->dpart,
+                       a->keys[0], a->counts, a->nk, 1 << bits0, a->nblocks);
+    a->keys_fresh = 1;
+  } else {
+    hipLaunchKernelGGL(k_verlet<false>, dim3(grid), dim3(256), 0, c->stream, a->parts[a->cur],
+                       a->npad, a->npart, a->force, fa, fh, fx0, pfac, r2max, inv2M, a->dpart,
+                       (uint32_t *)nullptr, (uint32_t *)nullptr, a->nk, 1, (int64_t)1);
+    a->keys_fresh = 0;
+  }
   VH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_reduce_cols, dim3(5), dim3(256), 0, c->stream, a->dpart, VERLET_BLOCKS, 5,
-                     a->dscal);
+  hipLaunchKernelGGL(k_reduce_cols, dim3(5), dim3(256), 0, c->stream, a->dpart, grid, 5, a->dscal);
   VH_LAUNCH_CHECK();
   return VH_OK;
 }

@@ -167,7 +167,7 @@ struct vh_unit {
   int32_t g;     // occupied index of the cell
 };
 
-#define VH_GRAPH_KEY 14
+#define VH_GRAPH_KEY 15
 
 struct vh_accel {
   vh_context *ctx;
@@ -214,10 +214,12 @@ struct vh_accel {
   int64_t maxocc;              // min(ncell, npart)
   uint32_t *cellscratch;       // 4 x (maxocc + 2) scan scratch of the cell classification
   int sorted_once;             // host flag: the lists of a sort exist
+  int keys_fresh;              // host flag: vh_verlet left keys[0] and the first pass's
+                               // tile histograms of the current records in counts
   int exact_field;             // VH_OPT_EXACT_FIELD: the reference's FMA order
   int step_graph;              // VH_OPT_STEP_GRAPH
   hipGraphExec_t graph[2];     // vh_step recorded with parts[cur] = parts[0], parts[1]
-  double graph_key[14];        // the arguments they were recorded with (VH_GRAPH_KEY)
+  double graph_key[15];        // the arguments they were recorded with (VH_GRAPH_KEY)
   // the projectile kept on the device (vh_projectile_set)
   double *dproj;               // 16 doubles, see k_projectile_step
   int has_dproj;
@@ -228,6 +230,15 @@ struct vh_accel {
 // ---------------------------------------------------------------------------
 // small device helpers
 // ---------------------------------------------------------------------------
+// the sort key of a cell (src/kernels.jl:902-908): clamp(k, 1, nk) per axis,
+// 0-based linear key kx + nk (ky + nk kz)
+__device__ __forceinline__ uint32_t cell_key0(int kx, int ky, int kz, int nk) {
+  kx = kx < 1 ? 1 : (kx > nk ? nk : kx);
+  ky = ky < 1 ? 1 : (ky > nk ? nk : ky);
+  kz = kz < 1 ? 1 : (kz > nk ? nk : kz);
+  return (uint32_t)((kx - 1) + nk * ((ky - 1) + nk * (kz - 1)));
+}
+
 #define VH_WAVE 64
 
 __device__ __forceinline__ int vh_lane() { return __lane_id(); }
@@ -286,4 +297,6 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
                    int soft_mode, double ball_radius, const double *dproj);
 int vh_projectile_step_impl(vh_accel *a, double dt);
 int vh_verlet_impl(vh_accel *a, double dt, double mass, double rcmax);
+// the push leaves the next sort's keys and first-digit histograms
+inline bool vh_verlet_tiles(const vh_accel *a) { return a->nbins * a->nblocks <= (int64_t)1 << 31; }
 int vh_energy_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, double enclosed);

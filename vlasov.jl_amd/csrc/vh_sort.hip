@@ -139,18 +139,12 @@ int vh_scan_exclusive(vh_context *c, uint32_t *data, int64_t n, uint32_t *tmp, i
 // ---------------------------------------------------------------------------
 // keys
 // ---------------------------------------------------------------------------
-// clamp(k, 1, nk) per axis, 0-based linear key kx + nk (ky + nk kz)
-__device__ __forceinline__ uint32_t cell_key0(int kx, int ky, int kz, int nk) {
-  kx = kx < 1 ? 1 : (kx > nk ? nk : kx);
-  ky = ky < 1 ? 1 : (ky > nk ? nk : ky);
-  kz = kz < 1 ? 1 : (kz > nk ? nk : kz);
-  return (uint32_t)((kx - 1) + nk * ((ky - 1) + nk * (kz - 1)));
-}
-
-__device__ __forceinline__ uint32_t load_key(bool first, const uint32_t *__restrict__ keys,
+// mode 1: the key from the record (first pass after a move that left no
+// keys); otherwise from the key array
+__device__ __forceinline__ uint32_t load_key(int mode, const uint32_t *__restrict__ keys,
                                              const int32_t *__restrict__ P, int64_t npad, int64_t i,
                                              int nk) {
-  if (first) {
+  if (mode == 1) {
     const int2 *p = reinterpret_cast<const int2 *>(P + 6 * i);
     const int2 a = p[0], b = p[1];
     return cell_key0(a.x, a.y, b.x, nk);
@@ -208,7 +202,7 @@ __global__ void __launch_bounds__(SORT_THREADS)
     uint32_t k = 0, v = 0, d = 0;
     if (valid) {
       k = load_key(first, keys_in, P, npad, i, nk);
-      v = first ? (uint32_t)i : vals_in[i];
+      v = first ? (uint32_t)i : vals_in[i];   // first pass (modes 1, 2): identity
       d = (k >> shift) & mask;
     }
     uint64_t peers = __ballot(valid);
@@ -379,17 +373,25 @@ int vh_sort_impl(vh_accel *a) {
   VH_TRY(vh_fill_i32(c, a->counters, CNT_COUNT, 0));
   // --- radix passes over (key, index) ---
   int src = 0;
+  // after vh_verlet's tile pass the keys of the current records and the
+  // first pass's tile histograms are there already (keys[0], counts)
+  const bool fresh = a->keys_fresh;
+  a->keys_fresh = 0;
   for (int p = 0; p < a->passes; ++p) {
     int shift = p * a->digit_bits;
     int bits = a->digit_bits;
     if (shift + bits > a->bits_total) bits = a->bits_total - shift;
     if (bits < 1) bits = 1;
     int nbins = 1 << bits;
-    int first = (p == 0);
-    hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)a->nblocks), dim3(SORT_THREADS),
-                       nbins * sizeof(uint32_t), st, a->keys[src], a->parts[a->cur], a->npad, first,
-                       a->nk, n, shift, nbins, a->nblocks, a->counts);
-    VH_LAUNCH_CHECK();
+    // first: 1 keys from the records, 2 keys from keys[0]; both with
+    // identity values
+    int first = p == 0 ? (fresh ? 2 : 1) : 0;
+    if (first != 2) {
+      hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)a->nblocks), dim3(SORT_THREADS),
+                         nbins * sizeof(uint32_t), st, a->keys[src], a->parts[a->cur], a->npad,
+                         first, a->nk, n, shift, nbins, a->nblocks, a->counts);
+      VH_LAUNCH_CHECK();
+    }
     VH_TRY(vh_scan_exclusive(c, a->counts, (int64_t)nbins * a->nblocks, a->scan_tmp,
                              a->scan_tmp_len, nullptr));
     hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)a->nblocks), dim3(SORT_THREADS),
