@@ -33,7 +33,7 @@ FIELD_FLOP_PER_PARTICLE = 4900.0      # 2300 FMA + 300 MUL of contract_spline_10
 # counter passes over `bench.py --steps 20 --warmup 5` (FETCH_SIZE 54 532 KiB x 2.000
 # + WRITE_SIZE 102 888 KiB x 1.000, the factors that make a 1 GiB device copy read
 # and write 1 GiB); recorded with the command in profiles/hbm_traffic_xenon_8M_130.md
-FIELD_TRAFFIC_BYTES_8M = (54532 * 2.0 + 102888 * 1.0) * 1024
+FIELD_TRAFFIC_BYTES_8M = (103271 * 2.0 + 105417 * 1.0) * 1024
 METRIC = "ms/step & particle-steps/s @8M,130^3; capture-q err vs 80M plain run"
 
 
