@@ -214,6 +214,44 @@ def test_spline_coefficients_mfma_layout(setup, orc):
     assert relerr(s["dmf"].csol, want) < REL
 
 
+@pytest.mark.parametrize("nfine", [16, 17, 33, 64, 80, 99])
+def test_rotating_product_shapes(setup, orc, nfine):
+    """Every epilogue of the rotating product at the launch shapes it takes:
+    one panel per block (n = 34, 36), several (n = 68, 130), a last row tile
+    with two rows (n = 34, 66, 130, 162), two block rows with four source
+    pairs per thread (n = 162, 200), against the FP64 oracle.  S^-1 on a random grid (plain stores), the single-level
+    Poisson solve (Hadamard and interior-fill epilogues, at n - 2) and the
+    mean field added to a random csol (accumulating epilogue).  The mean
+    field is not added to S^-1 of the solved potential here: that potential
+    (|phi| ~ 4e3 for a random density) loses 2.5e-5 of its scale at n = 130
+    to FP32 rounding in S^-1 even with numpy's own FP32 products, so a bar at
+    FP32 rounding of the product would measure the operator, not the kernel."""
+    s = setup
+    mesh = va.grids.SplineMesh.of(va.grids.uniform_axis(-30.0, 30.0, nfine))
+    n = mesh.n
+    rng = np.random.default_rng(nfine)
+    dm = va.DeviceMesh(s["ctx"], mesh)
+    try:
+        g = rng.standard_normal(n ** 3).astype(np.float32)
+        dm.upload(va._lib.VH_GRID_PHI, g)
+        dm.spline_coefficients()
+        assert relerr(dm.csol, orc.spline_coefficients(g.astype(np.float64), mesh)) < REL
+        rho = rng.random(n ** 3).astype(np.float32)
+        dm.upload(va._lib.VH_GRID_RHO, rho)
+        dm.poisson()
+        want_phi, _ = orc.poisson_single(rho.astype(np.float64), mesh)
+        assert relerr(dm.phi, want_phi) < REL
+        base = rng.standard_normal(n ** 3).astype(np.float32)
+        dm.upload(va._lib.VH_GRID_CSOL, base)
+        dm.effective_potential(s["jel"])
+        extra = orc.spline_coefficients(orc.effective_extra(rho.astype(np.float64), mesh, s["jel"]),
+                                         mesh)
+        scale = max(np.max(np.abs(base)), np.max(np.abs(extra)))
+        assert np.max(np.abs(dm.csol - (base + extra))) / scale < REL
+    finally:
+        dm.close()
+
+
 def _forces_setup(s, orc, seed=5):
     rng = np.random.default_rng(seed)
     fine, coarse = s["fine"], s["coarse"]

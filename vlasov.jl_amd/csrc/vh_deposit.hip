@@ -133,46 +133,105 @@ __global__ void __launch_bounds__(256)
 }
 
 // every grid point from the tiles that cover it; thread = 2x2x2 block of
-// points (2m-1, 2m) per axis, whose covering cells are m-2 .. m+1 (0-based).
-// Also the FP64 charge sum(rho m0x m0y m0z) per block.
+// points (2m-1, 2m) per axis, whose covering cells are m-2 .. m+1 (0-based),
+// 64 cells.  A block takes 8 x 8 x 4 such blocks; the tile offsets of the
+// 11 x 11 x 7 cells around them are staged in LDS (-1: no particle), each
+// thread keeps a 64-bit mask of its occupied cells and reads the tiles of
+// four cells at a time (all loads issued before the sums).  Each point adds
+// its cells in (z, y, x) order: a fixed order, so the density is
+// reproducible.  Also the FP64 charge sum(rho m0x m0y m0z) per block.
+#define CMB_TX 8
+#define CMB_TY 8
+#define CMB_TZ 4
 __global__ void __launch_bounds__(256)
     k_dep_combine(const float *__restrict__ tiles, const int32_t *__restrict__ cellmap,
                   const int32_t *__restrict__ unitstart, int nk, int n,
                   const double *__restrict__ m0, float *__restrict__ rho,
                   double *__restrict__ part) {
+  constexpr int EX = CMB_TX + 3, EY = CMB_TY + 3, EZ = CMB_TZ + 3;
+  __shared__ int32_t offs[EX * EY * EZ];
   __shared__ double lds[8];
   const int nb = nk + 1;
-  const int64_t total = (int64_t)nb * nb * nb;
+  const int tbx = (nb + CMB_TX - 1) / CMB_TX, tby = (nb + CMB_TY - 1) / CMB_TY;
+  const int bt = blockIdx.x;
+  const int bx0 = (bt % tbx) * CMB_TX, by0 = ((bt / tbx) % tby) * CMB_TY;
+  const int bz0 = (bt / (tbx * tby)) * CMB_TZ;
+  for (int e = threadIdx.x; e < EX * EY * EZ; e += blockDim.x) {
+    const int ex = e % EX, ey = (e / EX) % EY, ez = e / (EX * EY);
+    const int cx = bx0 - 2 + ex, cy = by0 - 2 + ey, cz = bz0 - 2 + ez;
+    int off = -1;
+    if (cx >= 0 && cy >= 0 && cz >= 0 && cx < nk && cy < nk && cz < nk) {
+      const int g = cellmap[cx + nk * (cy + nk * cz)];
+      // the cell's sums, merged into the tile of its first unit
+      if (g >= 0) off = unitstart[g];
+    }
+    offs[e] = off;
+  }
+  __syncthreads();
+  const int tx = threadIdx.x % CMB_TX, ty = (threadIdx.x / CMB_TX) % CMB_TY;
+  const int tz = threadIdx.x / (CMB_TX * CMB_TY);
+  const int mx = bx0 + tx, my = by0 + ty, mz = bz0 + tz;
   double q = 0.0;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int mx = (int)(t % nb), my = (int)((t / nb) % nb), mz = (int)(t / ((int64_t)nb * nb));
+  if (mx < nb && my < nb && mz < nb) {
+    // bit (z 4 + y) 4 + x: cell (mx - 2 + x, my - 2 + y, mz - 2 + z), LDS
+    // entry (tx + x, ty + y, tz + z); ascending bits = (z, y, x) order
+    uint64_t mask = 0;
+#pragma unroll
+    for (int z = 0; z < 4; ++z)
+#pragma unroll
+      for (int y = 0; y < 4; ++y)
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+          if (offs[(tx + x) + EX * ((ty + y) + EY * (tz + z))] >= 0)
+            mask |= 1ull << ((z * 4 + y) * 4 + x);
+    // the dual moments of the thread's points, loaded ahead (clamped indices:
+    // points outside the grid are not stored)
+    double wx[2], wy[2], wz[2];
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      const int px = min(max(2 * mx - 1 + d, 0), n - 1), py = min(max(2 * my - 1 + d, 0), n - 1);
+      const int pz = min(max(2 * mz - 1 + d, 0), n - 1);
+      wx[d] = m0[px];
+      wy[d] = m0[py];
+      wz[d] = m0[pz];
+    }
     float acc[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) acc[i] = 0.f;
-    for (int cz = mz - 2; cz <= mz + 1; ++cz) {
-      if (cz < 0 || cz >= nk) continue;
-      const int lz = 2 * (mz - cz) + 2;
-      for (int cy = my - 2; cy <= my + 1; ++cy) {
-        if (cy < 0 || cy >= nk) continue;
-        const int ly = 2 * (my - cy) + 2;
-        for (int cx = mx - 2; cx <= mx + 1; ++cx) {
-          if (cx < 0 || cx >= nk) continue;
-          const int g = cellmap[cx + nk * (cy + nk * cz)];
-          if (g < 0) continue;
-          const int lx = 2 * (mx - cx) + 2;
-          // the cell's sums, merged into the tile of its first unit
-          const float *tl = tiles + (int64_t)unitstart[g] * 512;
+    while (mask) {
+      // the next four occupied cells (slots past the last repeat the first
+      // and add nothing): tile of cell x, y, z at local (6-2x, 6-2y, 6-2z)
+      const float *tp[4];
+      bool ok[4];
+      int b0 = 0;
 #pragma unroll
-          for (int dz = 0; dz < 2; ++dz)
-#pragma unroll
-            for (int dy = 0; dy < 2; ++dy) {
-              const float2 v = *(const float2 *)(tl + lx + 8 * ((ly + dy) + 8 * (lz + dz)));
-              acc[(dz * 2 + dy) * 2 + 0] += v.x;
-              acc[(dz * 2 + dy) * 2 + 1] += v.y;
-            }
-        }
+      for (int u = 0; u < 4; ++u) {
+        ok[u] = mask != 0;
+        const int b = ok[u] ? __builtin_ctzll(mask) : b0;
+        if (u == 0) b0 = b;
+        mask &= mask - 1;
+        const int x = b & 3, y = (b >> 2) & 3, z = b >> 4;
+        const int off = offs[(tx + x) + EX * ((ty + y) + EY * (tz + z))];
+        tp[u] = tiles + (int64_t)off * 512 + (6 - 2 * x) + 8 * ((6 - 2 * y) + 8 * (6 - 2 * z));
       }
+      float2 v[4][2][2];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+          for (int dy = 0; dy < 2; ++dy) v[u][dz][dy] = *(const float2 *)(tp[u] + 8 * dy + 64 * dz);
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+          for (int dy = 0; dy < 2; ++dy) {
+            if (ok[u]) {
+              acc[(dz * 2 + dy) * 2 + 0] += v[u][dz][dy].x;
+              acc[(dz * 2 + dy) * 2 + 1] += v[u][dz][dy].y;
+            }
+          }
     }
 #pragma unroll
     for (int dz = 0; dz < 2; ++dz)
@@ -182,13 +241,13 @@ __global__ void __launch_bounds__(256)
         for (int dx = 0; dx < 2; ++dx) {
           const int px = 2 * mx - 1 + dx, py = 2 * my - 1 + dy, pz = 2 * mz - 1 + dz;
           if (px < 0 || py < 0 || pz < 0 || px >= n || py >= n || pz >= n) continue;
-          const float v = acc[(dz * 2 + dy) * 2 + dx];
-          rho[px + (int64_t)n * (py + (int64_t)n * pz)] = v;
-          q += (double)v * m0[px] * m0[py] * m0[pz];
+          const float val = acc[(dz * 2 + dy) * 2 + dx];
+          rho[px + (int64_t)n * (py + (int64_t)n * pz)] = val;
+          q += (double)val * wx[dx] * wy[dy] * wz[dz];
         }
   }
-  double s = vh_block_sum_d(q, lds);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  double sq = vh_block_sum_d(q, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = sq;
 }
 
 // fixed-order sum of nparts partials (one block)
@@ -212,7 +271,6 @@ __global__ void k_dep_scale(float *__restrict__ rho, int64_t len, const double *
     rho[i] *= sc;
 }
 
-#define COMBINE_BLOCKS 1024
 
 int vh_deposit_fine_impl(vh_accel *a, vh_mesh *fine, double charge) {
   vh_context *c = a->ctx;
@@ -229,10 +287,15 @@ int vh_deposit_fine_impl(vh_accel *a, vh_mesh *fine, double charge) {
   hipLaunchKernelGGL(k_dep_merge, dim3(mgrid), dim3(256), 0, st, a->tiles, a->fitlist, a->unitstart,
                      a->counters);
   VH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_dep_combine, dim3(COMBINE_BLOCKS), dim3(256), 0, st, a->tiles, a->cellmap,
-                     a->unitstart, a->nk, a->n, fine->dual, fine->rho, a->dpart);
+  const int nb = a->nk + 1;
+  const int cblocks = ((nb + CMB_TX - 1) / CMB_TX) * ((nb + CMB_TY - 1) / CMB_TY) *
+                      ((nb + CMB_TZ - 1) / CMB_TZ);
+  VH_REQUIRE(cblocks <= a->dpart_len, "deposit: %d combine blocks beyond %lld", cblocks,
+             (long long)a->dpart_len);
+  hipLaunchKernelGGL(k_dep_combine, dim3(cblocks), dim3(CMB_TX * CMB_TY * CMB_TZ), 0, st, a->tiles,
+                     a->cellmap, a->unitstart, a->nk, a->n, fine->dual, fine->rho, a->dpart);
   VH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, a->dpart, COMBINE_BLOCKS, 0, 1,
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, a->dpart, cblocks, 0, 1,
                      a->dscal + 16);
   VH_LAUNCH_CHECK();
   int64_t len = (int64_t)a->n * a->n * a->n;

@@ -236,10 +236,14 @@ __global__ void __launch_bounds__(64)
 //     share a particle.
 // v_mfma_f32_16x16x4_f32: lane l supplies A[l%16][l/16] and B[l/16][l%16] and
 // receives D[4(l/16)+v][l%16], v = 0..3.  Rows are laid out so that the four
-// D registers and 8 tiles of a lane hold, for its particle, the rows
-// q = 4t + v (q < 30) as (jj = q % 10, kk = l/16 + 4 (q / 10)): jj is known at
-// compile time, kk depends on the lane group only.  K = ii runs over 3 steps
-// of 4 (10 of 12 used), rows over 8 tiles of 16 (100 of 128 used).
+// D registers and 7 tiles of a lane hold, for its particle, the slots
+// q = 4t + v of its lane group g = l/16: q < 10 is (jj = q, kk = g), q < 20 is
+// (jj = q - 10, kk = g + 4), q < 25 is (jj = q - 20 + 5 (g & 1),
+// kk = 8 + g / 2), q >= 25 is padding: each group takes two planes of kk and
+// half of plane 8 or 9, 25 rows in 28 slots.  The accumulator of a slot is
+// known at compile time; only the last half plane's y weights and the z
+// weights depend on the group.  K = ii runs over 3 steps of 4 (10 of 12
+// used), rows over 7 tiles of 16 (100 of 112 used).
 // The sums run in another order than the reference's FMA chain, so the result
 // agrees to FP32 rounding, not to the bit; k_field keeps the exact order.
 //
@@ -270,42 +274,46 @@ __device__ __forceinline__ void ft_bops(const float *row, int grp, float b[6]) {
 }
 
 // The contraction of NS sub-groups at once (sg0 .. sg0 + NS - 1), pipelined
-// over the 8 row tiles: tile t's products (T_o and T_g chains of every
+// over the 7 row tiles: tile t's products (T_o and T_g chains of every
 // sub-group interleaved over the three K steps) are issued, then the y
 // contraction of tile t - 1, whose products are complete, runs on the VALU
-// while tile t's are in flight.  Slot q = 4t + v of a lane holds plane
-// kk = grp + 4 (q / 10), jj = q % 10.  Then the z contraction, the sum over
-// the four lane groups of a particle, and the store of -w F into the chunk's
-// slot (all four lanes store the same value).
+// while tile t's are in flight.  Slot q = 4t + v of a lane adds into
+// accumulator m = q / 10 (planes g, g + 4 and the half plane).  Then the z
+// contraction, the sum over the four lane groups of a particle, and the
+// store of -w F into the chunk's slot (all four lanes store the same value).
 template <int NS>
-__device__ __forceinline__ void ft_contract(const float A[8][3], const float *tabj, int sg0, int grp,
+__device__ __forceinline__ void ft_contract(const float A[7][3], const float *tabj, int sg0, int grp,
                                             int j16, float w, float (*fld)[64]) {
-  const bool top = grp < 2;
-  const int gt = top ? grp : 0;
-  float b[NS][6], y[NS][20], z[NS][6];
+  const int half = 5 * (grp & 1), hp = grp >> 1;
+  float b[NS][6], y[NS][20], yh[NS][10], z[NS][6];
 #pragma unroll
   for (int u = 0; u < NS; ++u) {
     const float *row = tabj + 16 * (sg0 + u) * FT_STRIDE;
     ft_bops(row, grp, b[u]);
 #pragma unroll
     for (int a = 0; a < 20; ++a) y[u][a] = row[20 + a];  // oy, then gy
-    const float z2 = row[48 + gt], z5 = row[58 + gt];
+    // the half plane's y weights: oy and gy at jj = half .. half + 4
+#pragma unroll
+    for (int a = 0; a < 5; ++a) {
+      yh[u][a] = row[20 + half + a];
+      yh[u][5 + a] = row[30 + half + a];
+    }
     z[u][0] = row[40 + grp];
     z[u][1] = row[44 + grp];
-    z[u][2] = top ? z2 : 0.f;
+    z[u][2] = row[48 + hp];
     z[u][3] = row[50 + grp];
     z[u][4] = row[54 + grp];
-    z[u][5] = top ? z5 : 0.f;
+    z[u][5] = row[58 + hp];
   }
   float ux[NS][3], uy[NS][3], uz[NS][3];
 #pragma unroll
   for (int u = 0; u < NS; ++u)
 #pragma unroll
     for (int m = 0; m < 3; ++m) ux[u][m] = uy[u][m] = uz[u][m] = 0.f;
-  ft_f4 To[NS][8], Tg[NS][8];
+  ft_f4 To[NS][7], Tg[NS][7];
 #pragma unroll
-  for (int t = 0; t <= 8; ++t) {
-    if (t < 8) {
+  for (int t = 0; t <= 7; ++t) {
+    if (t < 7) {
 #pragma unroll
       for (int u = 0; u < NS; ++u) {
         To[u][t] = ft_f4{0.f, 0.f, 0.f, 0.f};
@@ -327,11 +335,16 @@ __device__ __forceinline__ void ft_contract(const float A[8][3], const float *ta
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int q = 4 * tp + v;
-          if (q < 30) {
+          if (q < 20) {
             const int m = q / 10, jj = q % 10;
             ux[u][m] = __builtin_fmaf(y[u][jj], Tg[u][tp][v], ux[u][m]);
             uy[u][m] = __builtin_fmaf(y[u][10 + jj], To[u][tp][v], uy[u][m]);
             uz[u][m] = __builtin_fmaf(y[u][jj], To[u][tp][v], uz[u][m]);
+          } else if (q < 25) {
+            const int i = q - 20;
+            ux[u][2] = __builtin_fmaf(yh[u][i], Tg[u][tp][v], ux[u][2]);
+            uy[u][2] = __builtin_fmaf(yh[u][5 + i], To[u][tp][v], uy[u][2]);
+            uz[u][2] = __builtin_fmaf(yh[u][i], To[u][tp][v], uz[u][2]);
           }
         }
       }
@@ -343,7 +356,7 @@ __device__ __forceinline__ void ft_contract(const float A[8][3], const float *ta
   __builtin_amdgcn_sched_group_barrier(0x100, 16 * NS, 0);  // DS reads
   __builtin_amdgcn_sched_group_barrier(0x008, 6 * NS, 0);   // MFMA
 #pragma unroll
-  for (int i = 0; i < 42 * NS; ++i) {
+  for (int i = 0; i < 36 * NS; ++i) {
     __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
     __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // VALU
   }
@@ -397,17 +410,21 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     const int bx = 2 * kx - 5, by = 2 * ky - 5, bz = 2 * kz - 5;
     const bool ok = bx >= 1 && by >= 1 && bz >= 1 && bx + 9 <= n && by + 9 <= n && bz + 9 <= n;
     const int lo = un.lo, hi = un.hi;
-    // A operands of the cell: row j16 of tile t, k = ii = 4 s + grp
-    float A[8][3];
+    // A operands of the cell: row j16 of tile t (slot q = 4 t + j16 % 4 of
+    // lane group g = j16 / 4), k = ii = 4 s + grp
+    float A[7][3];
     if (ok) {
       const float *tb = csol + (bx - 1) + nn * ((by - 1) + nn * (bz - 1));
+      const int g = j16 >> 2;
 #pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        const int q = 4 * t + (j16 & 3), m = q / 10, jj = q % 10, kk = (j16 >> 2) + 4 * m;
+      for (int t = 0; t < 7; ++t) {
+        const int q = 4 * t + (j16 & 3);
+        const int jj = q < 10 ? q : q < 20 ? q - 10 : q - 20 + 5 * (g & 1);
+        const int kk = q < 10 ? g : q < 20 ? g + 4 : 8 + (g >> 1);
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
           const int ii = 4 * s + grp;
-          const bool in = q < 30 && kk < 10 && ii < 10;
+          const bool in = q < 25 && ii < 10;
           // padding reads a zero word: the loaded value is used as it is, so
           // the loads stay in flight until the first product
           A[t][s] = *(in ? tb + ii + nn * (jj + nn * kk) : zero);

@@ -6,7 +6,8 @@
 // by A along its first dimension and written transposed, which rotates the
 // dimensions; three in a row apply A along all three axes.  Here it is one
 // MFMA kernel (v_mfma_f32_16x16x4_f32, FP32 in and out) whose block owns all
-// n output rows of 32 output columns, so every source column is read once;
+// n output rows of its panels of 16 output columns, so every source column is
+// read once;
 // the Hadamard product by 1/(l_i+l_j+l_k) (src/tensorsolver.jl:291), the copy
 // into the interior of phi (src/kernels.jl:804-813) and the `csol .+= ...` of
 // the mean field (src/devicemesh.jl:182) are fused into its epilogue.
@@ -19,8 +20,8 @@ typedef float vh_f4 __attribute__((ext_vector_type(4)));
 // The product C = A S (C n x m, row-major out[i' m + j]; S[k][j] = src[j n + k])
 // by v_mfma_f32_16x16x4_f32 (lane l: A[l%16][k], B[k][l%16], k from l/16).
 // A block holds W row tiles of 16, one wave each (W = all MT row tiles up to
-// 9, else half of them), and walks panels of NC = 16 NCT output columns,
-// panel p, p + gridDim.x, ... (a few blocks per compute unit):
+// 9, else half of them), and takes panels of 16 output columns (one block
+// per compute unit; the blocks of one XCD take consecutive panels):
 //   * each wave keeps the A operands of its row tile for the whole K range in
 //     registers, loaded once per block, with K permuted so that lane group
 //     g = l/16 owns the contiguous slice k = g KQ + s, s < KQ (KQ odd,
@@ -29,43 +30,54 @@ typedef float vh_f4 __attribute__((ext_vector_type(4)));
 //     is 64 consecutive floats, one per lane, so the wave's reads are
 //     coalesced;
 //   * a panel's source columns are contiguous in memory: the block reads
-//     them with coalesced 8-byte loads into one of two LDS buffers, column
-//     stride ldb >= 4 KQ, ldb = 2 (mod 32), so that the 32 lanes of a
-//     half-wave (16 columns x 2 lane groups, odd KQ) hit 32 different banks;
-//     rows k >= n are zero, so every operand address is a fixed offset;
-//   * software pipeline: a panel's loads are issued a whole panel ahead of
-//     its products and land in the other LDS buffer while the current panel
-//     is multiplied, so that HBM latency, matrix-core work and stores of
-//     successive panels overlap.
-template <int KQ, int NCT, int EPI>
+//     them with coalesced 8-byte loads into an LDS buffer, column stride
+//     ldb >= 4 KQ, ldb = 2 (mod 32), so that the 32 lanes of a half-wave
+//     (16 columns x 2 lane groups, odd KQ) hit 32 different banks; rows
+//     k >= n are zero, so every operand address is a fixed offset;
+//   * the loads of up to NP panels (a block's whole share at the sizes of
+//     the step) are issued at once, together with the A operands, into
+//     registers; panel i is written to its LDS buffer and multiplied as soon
+//     as its own loads have landed, while the later panels' loads are still
+//     in flight.  A block walking its panels one ahead waited a memory
+//     latency per panel (13 us per product at n = 130); here it waits about
+//     one per block.
+// PF = source pairs per thread and panel (8 n <= PF blockDim): 2 when every
+// row tile has its wave (MT <= 9), else 4.
+// Panels in flight per block: NP panels of 16 x ldb floats stay within
+// 64 KiB of LDS (ldb <= 162 for KQ <= 33, 194 for KQ = 41, 290), and the
+// registers that hold them within the 168 of 9 waves per block.
+constexpr int gemm_np(int kq) { return kq <= 33 ? 5 : kq <= 41 ? 4 : 3; }
+template <int KQ, int PF, int EPI>
 __global__ void __launch_bounds__(576)
     k_rot_gemm(const float *__restrict__ A, const float *__restrict__ src, float *__restrict__ out,
-               int n, int64_t m, int ldb, const float *__restrict__ aux, int nfull) {
-  extern __shared__ float Bs[];   // two panels of NC x ldb
-  constexpr int NC = 16 * NCT;    // output columns of a panel
-  constexpr int PF = 4 * NCT;     // source pairs per thread and panel (NC/2 n <= PF blockDim)
+               int n, int m, int ldb, const float *__restrict__ aux, int nfull) {
+  // 32-bit indices throughout: n m < 2^31 (checked at launch)
+  extern __shared__ float Bs[];   // NP panels of NC x ldb
+  constexpr int NC = 16;          // output columns of a panel
+  constexpr int NCT = 1;
+  constexpr int NP = gemm_np(KQ);  // panels in flight per block
   const int nthr = blockDim.x;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int lc = l & 15, lk = l >> 4;
   const int mt = (n + 15) >> 4;
   const int r = blockIdx.y * (nthr >> 6) + w;
-  const int64_t npanel = (m + NC - 1) / NC;
+  const int npanel = (m + NC - 1) / NC;
   // XCD-aware walk (gridDim.x is a multiple of 8; workgroups go to the eight
   // XCDs in turn): the blocks of one XCD take consecutive panels, so that the
   // 64-byte pieces they write of each output row meet in that XCD's L2
   const int xcd = blockIdx.x & 7, per = gridDim.x >> 3;
-  const int64_t chunk = (npanel + 7) / 8;
-  int64_t p = xcd * chunk + (blockIdx.x >> 3);
-  const int64_t pend = (xcd + 1) * chunk < npanel ? (xcd + 1) * chunk : npanel;
+  const int chunk = (npanel + 7) / 8;
+  int p = xcd * chunk + (blockIdx.x >> 3);
+  const int pend = (xcd + 1) * chunk < npanel ? (xcd + 1) * chunk : npanel;
   if (p >= pend) return;  // the whole block
   // A operands of row tile r (waves past the last tile read tile mt - 1 and
   // only move panels)
   float a[KQ];
-  const float *ar = A + ((int64_t)(r < mt ? r : mt - 1) * KQ) * 64 + l;
+  const float *ar = A + (r < mt ? r : mt - 1) * KQ * 64 + l;
 #pragma unroll
   for (int s = 0; s < KQ; ++s) a[s] = ar[64 * s];
   const float2 *src2 = (const float2 *)src;
-  const int64_t tot2 = m * n / 2;
+  const int tot2 = m * n / 2;
   const int per2 = NC / 2 * n;
   const int pbuf = NC * ldb;
   // Every load is issued, at an address clamped into the source; pairs
@@ -73,58 +85,56 @@ __global__ void __launch_bounds__(576)
   // LDS.  (Using the loaded value right away, or a load under a condition,
   // makes the compiler wait for the load at once, which serialises the
   // pipeline.)
-  auto fetch = [&](float2 (&pv)[PF], int64_t q) {
-    const int64_t base2 = q * NC * (int64_t)n / 2;
+  auto fetch = [&](float2 (&pv)[PF], int q) {
+    const int base2 = q * NC * n / 2;
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
-      const int64_t g = base2 + threadIdx.x + u * nthr;
+      const int g = base2 + threadIdx.x + u * nthr;
       pv[u] = src2[g < tot2 ? g : tot2 - 1];
     }
   };
-  auto stash = [&](const float2 (&pv)[PF], int64_t q, float *B) {
-    const int64_t base2 = q * NC * (int64_t)n / 2;
+  // Every pair is written: the ones past the panel go to a spare LDS word
+  // pair after the buffers (a write under a condition lets the compiler
+  // move its load under the condition, where it is waited for at once).
+  // The destinations are the same in every buffer.
+  int dst[PF];
+#pragma unroll
+  for (int u = 0; u < PF; ++u) {
+    const int e = threadIdx.x + u * nthr;
+    const int f = 2 * e, cc = f / n, k = f - cc * n;   // n even: pairs never straddle a column
+    dst[u] = e < per2 ? cc * ldb + k : NP * pbuf;
+  }
+  auto stash = [&](const float2 (&pv)[PF], int q, float *B) {
+    const int base2 = q * NC * n / 2;
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
       const int e = threadIdx.x + u * nthr;
-      if (e < per2) {
-        const int f = 2 * e, cc = f / n, k = f - cc * n;   // n even: pairs never straddle a column
-        *(float2 *)(B + cc * ldb + k) = base2 + e < tot2 ? pv[u] : make_float2(0.f, 0.f);
-      }
+      float *d = dst[u] < NP * pbuf ? B + dst[u] : Bs + NP * pbuf;
+      *(float2 *)d = base2 + e < tot2 ? pv[u] : make_float2(0.f, 0.f);
     }
   };
-  // rows k in [n, ldb) of both buffers stay zero (the panels fill k < n)
-  for (int e = threadIdx.x; e < 2 * NC * (ldb - n); e += nthr) {
-    const int row = e / (ldb - n), k = n + e % (ldb - n);
-    Bs[row * ldb + k] = 0.f;
-  }
-  float2 R[PF];
-  fetch(R, p);
-  stash(R, p, Bs);
-  __syncthreads();
-  if (p + per < pend) fetch(R, p + per);
-  int buf = 0;
-  for (; p < pend; p += per, buf ^= 1) {
-    // the products of this panel while the next one's loads are in flight;
-    // the epilogue's operands (the Hadamard factors, or the values to add
-    // to) are loaded first, so that their latency hides behind the products
-    const int64_t j0 = p * NC;
-    float pre[NCT][4];
-    if (EPI == EPI_HADAMARD || EPI == EPI_ACCUMULATE) {
-      const float *srcp = EPI == EPI_HADAMARD ? aux : out;
-      const int64_t last = (int64_t)n * m - 1;
-#pragma unroll
-      for (int ct = 0; ct < NCT; ++ct)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int64_t idx = (int64_t)(r * 16 + 4 * lk + v) * m + j0 + ct * 16 + lc;
-          pre[ct][v] = srcp[idx < last ? idx : last];   // clamped: never under a condition
-        }
+  // the fused epilogue of output (orow, j), orow < n, j < m
+  auto emit = [&](int orow, int j, float val, float pv) {
+    const int idx = orow * m + j;
+    if (EPI == EPI_STORE) {
+      out[idx] = val;
+    } else if (EPI == EPI_HADAMARD) {
+      out[idx] = val * pv;
+    } else if (EPI == EPI_ACCUMULATE) {
+      out[idx] = pv + val;
+    } else {  // EPI_FILL_INTERIOR: idx = i + n (jj + n kz) of the interior, kz = orow
+      const int i = j % n, jj = j / n;
+      out[(i + 1) + nfull * ((jj + 1) + nfull * (orow + 1))] = val;
     }
-    const float *B = Bs + buf * pbuf;
-    vh_f4 acc[NCT];
-#pragma unroll
-    for (int ct = 0; ct < NCT; ++ct) acc[ct] = (vh_f4){0.f, 0.f, 0.f, 0.f};
+  };
+  // the products of panel q from LDS buffer B and the fused epilogue, whose
+  // operands (the Hadamard factors, or the values to add to) are in pre
+  auto panel = [&](int q, const float *B, const float (&pre)[4]) {
+    const int j0 = q * NC;
     if (r < mt) {
+      vh_f4 acc[NCT];
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) acc[ct] = (vh_f4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int s = 0; s < KQ; ++s) {
         const int k = lk * KQ + s;  // k >= n: zero A, zero LDS padding
@@ -132,40 +142,72 @@ __global__ void __launch_bounds__(576)
         for (int ct = 0; ct < NCT; ++ct)
           acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], B[(16 * ct + lc) * ldb + k], acc[ct], 0, 0, 0);
       }
-    }
-    // the next panel into the other buffer before this one's stores, so that
-    // waiting for its loads does not wait for the stores; the scheduling
-    // barrier keeps that wait after the products
-    __builtin_amdgcn_sched_barrier(0);
-    const bool more = p + per < pend;
-    if (more) stash(R, p + per, Bs + (buf ^ 1) * pbuf);
-    // the loads of the panel after next, issued before this panel's stores
-    if (p + 2 * per < pend) fetch(R, p + 2 * per);
-    if (r < mt) {
 #pragma unroll
       for (int ct = 0; ct < NCT; ++ct) {
-        const int64_t j = j0 + ct * 16 + lc;
+        const int j = j0 + ct * 16 + lc;
         if (j >= m) continue;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int orow = r * 16 + 4 * lk + v;
-          if (orow >= n) continue;
-          const int64_t idx = (int64_t)orow * m + j;
-          const float val = acc[ct][v];
-          if (EPI == EPI_STORE) {
-            out[idx] = val;
-          } else if (EPI == EPI_HADAMARD) {
-            out[idx] = val * pre[ct][v];
-          } else if (EPI == EPI_ACCUMULATE) {
-            out[idx] = pre[ct][v] + val;
-          } else {  // EPI_FILL_INTERIOR: idx = i + n (jj + n kz) of the interior, kz = orow
-            const int jc = (int)j, i = jc % n, jj = jc / n;
-            out[(i + 1) + nfull * ((jj + 1) + (int64_t)nfull * (orow + 1))] = val;
-          }
+          if (orow < n) emit(orow, j, acc[ct][v], pre[v]);
         }
       }
     }
-    __syncthreads();
+  };
+  // rows k in [n, ldb) of every buffer stay zero (the panels fill k < n)
+  for (int e = threadIdx.x; e < NP * NC * (ldb - n); e += nthr) {
+    const int row = e / (ldb - n), k = n + e % (ldb - n);
+    Bs[row * ldb + k] = 0.f;
+  }
+  // The loads of a round's panels and of their epilogue operands, all issued
+  // before the round's first wait, after the A operands', in the order the
+  // panels use them (waiting for panel i never waits for a later one's).
+  // Slots past the block's last panel load the round's first panel again
+  // (in the cache by then) and are not multiplied.
+  float2 R[NP][PF];
+  float pre[NP][4];
+  auto fetch_round = [&](int p0) {
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      const int q = p0 + i * per < pend ? p0 + i * per : p0;
+      fetch(R[i], q);
+      if (EPI == EPI_HADAMARD || EPI == EPI_ACCUMULATE) {
+        const float *srcp = EPI == EPI_HADAMARD ? aux : out;
+        const int last = n * m - 1;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int idx = (r * 16 + 4 * lk + v) * m + q * NC + lc;
+          pre[i][v] = srcp[idx < last ? idx : last];   // clamped: never under a condition
+        }
+      }
+    }
+  };
+  auto run_round = [&](int p0) {
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      const int pi = p0 + i * per;
+      if (pi >= pend) break;   // the whole block
+      float *B = Bs + i * pbuf;
+      // waits for panel i's loads only: the later panels' stay in flight
+      stash(R[i], pi, B);
+      // the epilogue operands are taken here (an empty asm that reads them):
+      // left alone, the compiler moves their loads down to the conditional
+      // stores, where each is waited for at once, with every later load
+      if (EPI == EPI_HADAMARD || EPI == EPI_ACCUMULATE)
+        asm volatile("" ::"v"(pre[i][0]), "v"(pre[i][1]), "v"(pre[i][2]), "v"(pre[i][3]));
+      __syncthreads();
+      panel(pi, B, pre[i]);
+    }
+  };
+  // the first round in straight-line code: at a loop header the compiler
+  // would wait for every load in flight (the A operands' too) before the
+  // round's loads are issued
+  fetch_round(p);
+  run_round(p);
+  for (p += NP * per; p < pend; p += NP * per) {
+    __syncthreads();   // every wave is done with the buffers of the last round
+    fetch_round(p);
+    run_round(p);
   }
 }
 
@@ -196,46 +238,56 @@ int vh_gemm_pack(vh_context *c, float **dst, const double *colmajor, int n) {
   return VH_OK;
 }
 
-// launch shape: column tiles per panel, waves (row tiles) per block, blocks
-// per compute unit
-#define GEMM_NCT 1
+// launch shape: waves (row tiles) per block; one block per compute unit
 #define GEMM_WAVES(mt) ((mt) <= 9 ? (mt) : ((mt) + 1) / 2)
-#define GEMM_BPC 1
 
 int vh_rotate_gemm(vh_context *c, const float *A, const float *src, float *out, int n, int64_t m,
                    int epi, const float *aux, int nfull) {
   const int mt = (n + 15) / 16;
   VH_REQUIRE(mt >= 1 && mt <= 17, "rotating product: n = %d beyond 272", n);
+  VH_REQUIRE(m >= 1 && (int64_t)n * m < ((int64_t)1 << 31),
+             "rotating product: n m = %lld beyond 32-bit indices", (long long)((int64_t)n * m));
   VH_REQUIRE(n % 2 == 0 && ((uintptr_t)src & 7) == 0,
              "rotating product: n (%d) must be even and the source 8-byte aligned", n);
   const int kq = gemm_kq(n);
   // smallest >= 4 kq (the K range the lanes read) with ldb = 2 (mod 32)
   const int ldb = 4 * kq + ((2 - (4 * kq) % 32) + 32) % 32;
   const int wv = GEMM_WAVES(mt);
-  constexpr int nc = 16 * GEMM_NCT;
-  VH_REQUIRE(nc / 2 * n <= 4 * GEMM_NCT * 64 * wv, "rotating product: panel of n = %d too large", n);
+  constexpr int nc = 16;
+  // source pairs per thread and panel: 8 n <= 128 wv when every row tile has
+  // its wave (wv = mt >= n/16), 8 n <= 256 wv otherwise (wv >= n/32)
+  const int pf = mt <= 9 ? 2 : 4;
+  VH_REQUIRE(nc / 2 * n <= pf * 64 * wv, "rotating product: panel of n = %d too large", n);
   const int gy = (mt + wv - 1) / wv;
   const int64_t npanel = (m + nc - 1) / nc;
-  int64_t gx = ((int64_t)c->num_cus * GEMM_BPC + gy - 1) / gy;
+  int64_t gx = ((int64_t)c->num_cus + gy - 1) / gy;
   if (gx > npanel) gx = npanel;
   gx = (gx + 7) / 8 * 8;
   dim3 grid((unsigned)gx, (unsigned)gy), block(64 * wv);
-  const size_t lds = sizeof(float) * 2 * nc * (size_t)ldb;
+  const size_t lds = sizeof(float) * (gemm_np(kq) * nc * (size_t)ldb + 2);   // + spare pair
+  VH_REQUIRE(lds <= 65536, "rotating product: %zu B of LDS for n = %d", lds, n);
   VH_REQUIRE(epi >= EPI_STORE && epi <= EPI_ACCUMULATE, "rotating product: epilogue %d", epi);
-#define LAUNCH(T, E)                                                                               \
-  hipLaunchKernelGGL((k_rot_gemm<T, GEMM_NCT, E>), grid, block, lds, c->stream, A, src, out, n, m, \
+#define LAUNCH(T, P, E)                                                                            \
+  hipLaunchKernelGGL((k_rot_gemm<T, P, E>), grid, block, lds, c->stream, A, src, out, n, (int)m,    \
                      ldb, aux, nfull)
-#define CASE(T)                                                                                    \
-  case T:                                                                                          \
-    if (epi == EPI_STORE) LAUNCH(T, EPI_STORE);                                                    \
-    else if (epi == EPI_HADAMARD) LAUNCH(T, EPI_HADAMARD);                                         \
-    else if (epi == EPI_ACCUMULATE) LAUNCH(T, EPI_ACCUMULATE);                                     \
-    else LAUNCH(T, EPI_FILL_INTERIOR);                                                             \
-    break;
+#define EPIS(T, P)                                                                                 \
+  if (epi == EPI_STORE) LAUNCH(T, P, EPI_STORE);                                                   \
+  else if (epi == EPI_HADAMARD) LAUNCH(T, P, EPI_HADAMARD);                                        \
+  else if (epi == EPI_ACCUMULATE) LAUNCH(T, P, EPI_ACCUMULATE);                                    \
+  else LAUNCH(T, P, EPI_FILL_INTERIOR);
+  // KQ <= 33: n <= 132, mt <= 9; KQ = 41: n in (132, 164]; KQ >= 49: mt >= 11
+#define CASE2(T) case T: EPIS(T, 2) break;
+#define CASE4(T) case T: EPIS(T, 4) break;
   switch (kq) {
-    CASE(5) CASE(9) CASE(15) CASE(17) CASE(25) CASE(33) CASE(41) CASE(49) CASE(57) CASE(65) CASE(69)
+    CASE2(5) CASE2(9) CASE2(15) CASE2(17) CASE2(25) CASE2(33)
+    case 41:
+      if (pf == 2) { EPIS(41, 2) } else { EPIS(41, 4) }
+      break;
+    CASE4(49) CASE4(57) CASE4(65) CASE4(69)
   }
-#undef CASE
+#undef CASE2
+#undef CASE4
+#undef EPIS
 #undef LAUNCH
   VH_LAUNCH_CHECK();
   return VH_OK;
@@ -254,58 +306,90 @@ static int apply_all(vh_context *c, const float *A, int n, const float *src, flo
 // ---------------------------------------------------------------------------
 // multipole moments (src/poisson.jl:55-189), FP64
 // ---------------------------------------------------------------------------
+// One wave per (j, k) column of the grid, four columns at a time: its lanes
+// read each column's n points coalesced (i = l + 64 q, q < Q) and add each
+// point's ten products to their own FP64 accumulators; the column weights
+// (products of the dual moments at j and k) are the same for the whole wave.
+// All loads of the four columns are issued before the sums, at clamped
+// addresses (points past the column or columns past the grid count zero).
+// The dual moments sit in LDS.  Sums run in a fixed order for a given n
+// (lane, then wave, then block, then the fixed-order reduction of the
+// block partials).
+#define MOM_MAXN 288
+template <int Q>
 __global__ void __launch_bounds__(256)
     k_moments(const float *__restrict__ c, const double *__restrict__ dual, int n,
               double *__restrict__ part, int nparts) {
-  __shared__ double lds[8];
-  const double *p0 = dual, *p1 = dual + n, *p2 = dual + 2 * n;
+  __shared__ double pd[3 * MOM_MAXN];
+  __shared__ double lds[10][4];
+  for (int i = threadIdx.x; i < 3 * n; i += blockDim.x) pd[i] = dual[i];
+  __syncthreads();
+  const double *p0 = pd, *p1 = pd + n, *p2 = pd + 2 * n;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   double acc[10];
 #pragma unroll
   for (int v = 0; v < 10; ++v) acc[v] = 0.0;
-  const int64_t njk = (int64_t)n * n;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < njk;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int j = (int)(t % n), k = (int)(t / n);
-    const double a = p0[j] * p0[k], b = p1[j] * p0[k], cc = p0[j] * p1[k];
-    const double d = p2[j] * p0[k], e = p0[j] * p2[k], f = p1[j] * p1[k];
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    const float *col = c + t * n;
-    for (int i = 0; i < n; ++i) {
-      const double v = col[i];
-      s0 += v * p0[i];
-      s1 += v * p1[i];
-      s2 += v * p2[i];
+  const int njk = n * n;
+  for (int t0 = 4 * (blockIdx.x * 4 + w); t0 < njk; t0 += 16 * gridDim.x) {
+    float val[4][Q];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float *col = c + (int64_t)min(t0 + u, njk - 1) * n;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) val[u][q] = col[min(l + 64 * q, n - 1)];
     }
-    acc[0] += s0 * a;
-    acc[1] += s1 * a;
-    acc[2] += s0 * b;
-    acc[3] += s0 * cc;
-    acc[4] += s2 * a;
-    acc[5] += s0 * d;
-    acc[6] += s0 * e;
-    acc[7] += s1 * b;
-    acc[8] += s1 * cc;
-    acc[9] += s0 * f;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int t = min(t0 + u, njk - 1), j = t % n, k = t / n;
+      const double live = t0 + u < njk ? 1.0 : 0.0;
+      const double a = live * p0[j] * p0[k], b = live * p1[j] * p0[k];
+      const double cc = live * p0[j] * p1[k], d = live * p2[j] * p0[k];
+      const double e = live * p0[j] * p2[k], f = live * p1[j] * p1[k];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int i = min(l + 64 * q, n - 1);
+        const double v = l + 64 * q < n ? (double)val[u][q] : 0.0;
+        const double x0 = v * p0[i], x1 = v * p1[i], x2 = v * p2[i];
+        acc[0] += x0 * a;
+        acc[1] += x1 * a;
+        acc[2] += x0 * b;
+        acc[3] += x0 * cc;
+        acc[4] += x2 * a;
+        acc[5] += x0 * d;
+        acc[6] += x0 * e;
+        acc[7] += x1 * b;
+        acc[8] += x1 * cc;
+        acc[9] += x0 * f;
+      }
+    }
   }
+#pragma unroll
   for (int v = 0; v < 10; ++v) {
-    const double s = vh_block_sum_d(acc[v], lds);
-    if (threadIdx.x == 0) part[(int64_t)v * nparts + blockIdx.x] = s;
+    const double sw = vh_wave_sum(acc[v]);
+    if (l == 0) lds[v][w] = sw;
+  }
+  __syncthreads();
+  if (threadIdx.x < 10) {
+    const int v = threadIdx.x;
+    part[(int64_t)v * nparts + blockIdx.x] = ((lds[v][0] + lds[v][1]) + lds[v][2]) + lds[v][3];
   }
 }
 
+// one wave per value: its lanes sum the block partials of that value in a
+// fixed order (lane strides, then the wave sum)
 __global__ void k_reduce_rows(const double *__restrict__ part, int nparts, int nvals,
                               double *__restrict__ out) {
-  __shared__ double lds[16];
-  for (int v = 0; v < nvals; ++v) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = blockDim.x >> 6;
+  for (int v = w; v < nvals; v += nw) {
     double s = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[(int64_t)v * nparts + i];
-    const double r = vh_block_sum_d(s, lds);
-    if (threadIdx.x == 0) out[v] = r;
+    for (int i = l; i < nparts; i += 64) s += part[(int64_t)v * nparts + i];
+    s = vh_wave_sum(s);
+    if (l == 0) out[v] = s;
   }
 }
 
 // _multipole (src/poisson.jl:168-189): mp = [q, cx, cy, cz, qxx, qyy, qzz, qxy, qxz, qyz]
-__global__ void k_multipole(const double *__restrict__ mom, double *__restrict__ mp) {
+__device__ void multipole_of(const double *mom, double *mp) {
   const double q = mom[0], d100 = mom[1], d010 = mom[2], d001 = mom[3];
   const double m200 = mom[4], m020 = mom[5], m002 = mom[6], mxy = mom[7], mxz = mom[8], myz = mom[9];
   double bx = 0.0, by = 0.0, bz = 0.0;
@@ -327,17 +411,44 @@ __global__ void k_multipole(const double *__restrict__ mom, double *__restrict__
   for (int i = 0; i < 6; ++i) mp[4 + i] = quad[i] - shift[i];
 }
 
+// the ten moments from the block partials (one wave per moment, as
+// k_reduce_rows) into mom[0..9], then the multipole into mp
+__global__ void __launch_bounds__(640)
+    k_moments_finish(const double *__restrict__ part, int nparts, double *__restrict__ mom,
+                     double *__restrict__ mp) {
+  __shared__ double sm[10];
+  const int v = threadIdx.x >> 6, l = threadIdx.x & 63;
+  double s = 0.0;
+  for (int i = l; i < nparts; i += 64) s += part[(int64_t)v * nparts + i];
+  s = vh_wave_sum(s);
+  if (l == 0) {
+    sm[v] = s;
+    mom[v] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) multipole_of(sm, mp);
+}
+
 #define MOM_BLOCKS 256
 
 int vh_multipole_impl(vh_mesh *m) {
   vh_context *c = m->ctx;
-  hipLaunchKernelGGL(k_moments, dim3(MOM_BLOCKS), dim3(256), 0, c->stream, m->rho, m->dual, m->n,
-                     m->partials, MOM_BLOCKS);
+  const int n = m->n;
+  VH_REQUIRE(n <= MOM_MAXN, "multipole: n = %d beyond %d", n, MOM_MAXN);
+  // one wave per four columns: about one pass per wave (partials: 10 n^2)
+  const int nb = (n * n + 15) / 16 < 4096 ? (n * n + 15) / 16 : 4096;
+  if (n <= 128)
+    hipLaunchKernelGGL(k_moments<2>, dim3(nb), dim3(256), 0, c->stream, m->rho, m->dual, n,
+                       m->partials, nb);
+  else if (n <= 192)
+    hipLaunchKernelGGL(k_moments<3>, dim3(nb), dim3(256), 0, c->stream, m->rho, m->dual, n,
+                       m->partials, nb);
+  else
+    hipLaunchKernelGGL(k_moments<5>, dim3(nb), dim3(256), 0, c->stream, m->rho, m->dual, n,
+                       m->partials, nb);
   VH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, c->stream, m->partials, MOM_BLOCKS, 10,
-                     m->scal);
-  VH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_multipole, dim3(1), dim3(1), 0, c->stream, m->scal, m->scal + 10);
+  hipLaunchKernelGGL(k_moments_finish, dim3(1), dim3(640), 0, c->stream, m->partials, nb,
+                     m->scal, m->scal + 10);
   VH_LAUNCH_CHECK();
   return VH_OK;
 }
