@@ -221,7 +221,7 @@ def main():
                                f"dt=0.5, steps {args.warmup + 1}-{args.warmup + args.steps} of 700",
                    "particles": args.particles, "grid": 2 * args.nfine + 2, "softening": args.mode},
         "roofline": {"bound": "mfma",
-                     "engine": "FP32 MFMA 16x16x4 (table contraction) + VALU FP32 (second stage)",
+                     "engine": "FP32 MFMA 16x16x4 (table contraction) + VALU FP32 (second stage); the launch also pushes its particles (Verlet, keys of the next sort)",
                      "kernel": "k_field_mfma",
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,

@@ -460,26 +460,38 @@ def test_resident_projectile_matches_host(config1):
             s.close()
 
 
-def test_native_step_matches_separate_calls(config1):
-    """vh_step (one native call per step) against the same step made of the
-    separate entry points, without projectile: identical clouds after three
-    steps (the same kernels in the same order)."""
+@pytest.mark.parametrize("with_projectile", [False, True])
+def test_native_step_matches_separate_calls(config1, with_projectile):
+    """vh_step (one native call per step, the field and fallback kernels
+    pushing the particles they computed the force of) against the same step
+    made of the separate entry points (forces, projectile step, vh_verlet),
+    without and with the projectile on the device: identical clouds, forces
+    and projectile states after three steps, each step's sort starting from
+    the keys and histograms the push left."""
     p, fine, coarse, sm = config1
     n = p.nparticles
     w = p.nelectrons / n
     pos, mom = va.sample_thomas_fermi(va.read_radial_profile(), n, w, seed=-7)
-    a = va.Simulation(p, None, tables=(fine, coarse, sm), positions=pos, momenta=mom)
-    b = va.Simulation(p, None, tables=(fine, coarse, sm), positions=pos, momenta=mom)
+    sims = []
     try:
+        for _ in range(2):
+            proj = va.xenon_projectile(x0=-20.0, impact=10.0) if with_projectile else None
+            sims.append(va.Simulation(p, None, projectile=proj, tables=(fine, coarse, sm),
+                                      positions=pos, momenta=mom))
+        a, b = sims
         for _ in range(3):
             a.step()
             b.update_forces(advance=True)
             b.acc.verlet(p.dt, b.mass, p.rcmax, diagnostics=False)
         assert a.acc.download().tobytes() == b.acc.download().tobytes()
         assert a.acc.forces().tobytes() == b.acc.forces().tobytes()
+        if with_projectile:
+            pa, pb = a.projectile, b.projectile
+            assert np.array_equal(pa.position, pb.position)
+            assert np.array_equal(pa.velocity, pb.velocity)
     finally:
-        a.close()
-        b.close()
+        for s in sims:
+            s.close()
 
 
 def test_step_graph_replays_match_eager(config1):

@@ -898,6 +898,8 @@ static void drop_graphs(vh_accel *a) {
 
 static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, double dt,
                       double mass, double rcmax, double jel_nions, double jel_radius);
+// the field and fallback kernels push (the matrix-core field kernel)
+static bool step_fuses_push(const vh_accel *a) { return !a->exact_field; }
 
 extern "C" int vh_step(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, double dt,
                        double mass, double rcmax, double jel_nions, double jel_radius) {
@@ -911,7 +913,7 @@ extern "C" int vh_step(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weigh
                                     (double)(uintptr_t)fine, (double)(uintptr_t)coarse,
                                     (double)a->has_dproj, (double)a->exact_field, a->proj_charge,
                                     a->proj_scale, (double)a->proj_soft_mode, a->proj_ball_radius,
-                                    (double)a->keys_fresh};
+                                    (double)(a->keys_fresh + 2 * a->hist_fresh)};
   if (memcmp(key, a->graph_key, sizeof key) != 0) {
     drop_graphs(a);
     memcpy(a->graph_key, key, sizeof key);
@@ -946,7 +948,10 @@ extern "C" int vh_step(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weigh
   }
   VH_CHECK_HIP(hipGraphLaunch(a->graph[b], st));
   a->cur = 1 - b;  // what the recorded sort and push did on the host
-  a->keys_fresh = vh_verlet_tiles(a) ? 1 : 0;
+  // what the recorded push left: keys, and the histograms too when
+  // vh_verlet pushed
+  a->keys_fresh = step_fuses_push(a) || vh_verlet_tiles(a) ? 1 : 0;
+  a->hist_fresh = !step_fuses_push(a) && vh_verlet_tiles(a) ? 1 : 0;
   return VH_OK;
 }
 
@@ -958,6 +963,30 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
   VH_TRY(vh_poisson_nested(fine, coarse));
   VH_TRY(vh_potential_coefficients(fine, jel_nions, jel_radius));
   VH_TRY(vh_effective_potential(coarse, jel_nions, jel_radius));
+  if (step_fuses_push(a)) {
+    // the field and fallback kernels push the particles they computed the
+    // force of: the same records, forces and keys as forces then vh_verlet
+    // (which reads back every record and force), without that pass
+    VH_REQUIRE(fine->n == a->n && fine->ctx == a->ctx && coarse->ctx == a->ctx,
+               "vh_step: meshes do not match the accelerator");
+    if (!a->sorted_once) {
+      vh_set_error("vh_step: the cloud has not been sorted");
+      return VH_ESTATE;
+    }
+    const double fa = (double)(float)(dt * dt / mass);
+    {
+      vh_region r(a->ctx, VH_ST_FIELD);
+      if (a->has_dproj) {
+        const double proj[5] = {0.0, 0.0, 0.0, a->proj_charge, a->proj_scale};
+        VH_TRY(vh_forces_impl(a, fine, coarse, weight, proj, a->proj_soft_mode,
+                              a->proj_ball_radius, a->dproj, fa));
+      } else {
+        VH_TRY(vh_forces_impl(a, fine, coarse, weight, nullptr, VH_SOFT_DEVICE, 0.0, nullptr, fa));
+      }
+    }
+    if (a->has_dproj) VH_TRY(vh_projectile_step(a, dt));
+    return VH_OK;
+  }
   if (a->has_dproj) {
     VH_TRY(vh_forces_resident(a, fine, coarse, weight));
     VH_TRY(vh_projectile_step(a, dt));

@@ -103,6 +103,61 @@ struct vh_proj_args {
   int pot;            // the pair-potential sum is wanted (matrix-core kernel)
 };
 
+// ---------------------------------------------------------------------------
+// packed Verlet (src/kernels.jl:985-1025), one axis: the exact FP32 sequence
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void verlet1(int kq, float dq, int ok, float od, float a, float f, float h,
+                                        int &kn, float &dn, float &gap) {
+  kn = 2 * kq - ok;
+  dn = 2.0f * dq - od + a * f;
+  const int m = (int)rintf(dn / h);
+  kn += m;
+  dn -= (float)m * h;
+  gap = (float)(kn - ok) * h + (dn - od);
+}
+
+// The push of a particle by the kernel that computed its force (one step
+// without diagnostics): the same verlet1 sequence as k_verlet.  The new
+// record goes to the current half, the old one to the previous half, and
+// the new cell key to keys[] for the next sort (which then counts its first
+// digit from the keys).
+struct vh_push_args {
+  int on;              // 0: forces only
+  float a, h;          // dt^2 / mass, the cell width
+  int nk;              // cells per axis (the key)
+  uint32_t *keys;      // new cell key per slot
+};
+
+__device__ __forceinline__ uint32_t push_record(const vh_pos &cur, const vh_pos &old, float f0,
+                                               float f1, float f2, const vh_push_args &pu,
+                                               vh_pos &nw) {
+  float g1, g2, g3;
+  verlet1(cur.k[0], cur.d[0], old.k[0], old.d[0], pu.a, f0, pu.h, nw.k[0], nw.d[0], g1);
+  verlet1(cur.k[1], cur.d[1], old.k[1], old.d[1], pu.a, f1, pu.h, nw.k[1], nw.d[1], g2);
+  verlet1(cur.k[2], cur.d[2], old.k[2], old.d[2], pu.a, f2, pu.h, nw.k[2], nw.d[2], g3);
+  return cell_key0(nw.k[0], nw.k[1], nw.k[2], pu.nk);
+}
+
+// Stores through a buffer descriptor over `bytes` bytes at p: the hardware's
+// range check drops the stores of lanes past the end.  A chunk's stores are
+// then issued by every lane with no branch, so the compiler's count of the
+// memory operations in flight stays exact (with stores under a lane
+// condition it waits for every store before the next chunk's loads).
+typedef int vh_i2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t vh_rsrc(const void *p, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, bytes, 0x00020000);
+}
+__device__ __forceinline__ void vh_bstore_f(__amdgpu_buffer_rsrc_t r, int lane, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_int(v), r, 4 * lane, 0, 0);
+}
+__device__ __forceinline__ void vh_bstore_pos(__amdgpu_buffer_rsrc_t r, int lane, const vh_pos &v) {
+  const int o = 24 * lane;
+  __builtin_amdgcn_raw_buffer_store_b64((vh_i2){v.k[0], v.k[1]}, r, o, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b64((vh_i2){v.k[2], __float_as_int(v.d[0])}, r, o + 8, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b64((vh_i2){__float_as_int(v.d[1]), __float_as_int(v.d[2])}, r,
+                                        o + 16, 0, 0);
+}
+
 __global__ void __launch_bounds__(64)
     k_field(const int32_t *__restrict__ P, int64_t npad, const float *__restrict__ csol, int n,
             const float *__restrict__ ovl, const float *__restrict__ grad,
@@ -383,11 +438,13 @@ __device__ __forceinline__ void ft_contract(const float A[7][3], const float *ta
 }
 
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
-    k_field_mfma(const int32_t *__restrict__ P, int64_t npad, const float *__restrict__ csol, int n,
+    k_field_mfma(int32_t *__restrict__ P, int64_t npad, const float *__restrict__ csol, int n,
                  const float4 *__restrict__ og, const float *__restrict__ zero,
                  const vh_unit *__restrict__ units, const int32_t *__restrict__ cnt, int nk,
                  float x0, float h, float spacing, float nbdt, float w, int ncol,
-                 vh_proj_args pa, float *__restrict__ force, double *__restrict__ part) {
+                 vh_proj_args pa, vh_push_args pu, float *__restrict__ force,
+                 double *__restrict__ part) {
+  int32_t *Q = vh_prev_half(P, npad);
   typedef ft_f4 f4;
   if (pa.dev) {
     pa.px = (float)pa.dev[0];
@@ -442,6 +499,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
       const int s = base + l;
       const bool mine = l < nlive;
       const vh_pos rp = rnext;
+      // the previous positions, for the push after the contraction
+      const vh_pos qp = vh_load_pos(Q, mine ? s : lo);
       const float dx0 = rp.d[0], dy0 = rp.d[1], dz0 = rp.d[2];
       // everything that reads the record comes before the first branch (a
       // load still in flight at a branch is waited for with every later one)
@@ -523,11 +582,22 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
         for (int sg = 0; sg < nsg; ++sg) ft_contract<1>(A, tabj, sg, grp, j16, w, fld);
       }
       __syncthreads();
-      // phase C: field minus projectile term, coalesced stores
-      if (mine) {
-        force[s] = ok ? fld[0][l] - fx2 : nan;
-        force[npad + s] = ok ? fld[1][l] - fy2 : nan;
-        force[2 * npad + s] = ok ? fld[2][l] - fz2 : nan;
+      // phase C: field minus projectile term, coalesced stores of the live
+      // lanes; with the push, the records and keys of a cell that fits (the
+      // particles of the others are pushed by the fallback)
+      const float f0 = ok ? fld[0][l] - fx2 : nan, f1 = ok ? fld[1][l] - fy2 : nan;
+      const float f2 = ok ? fld[2][l] - fz2 : nan;
+      vh_bstore_f(vh_rsrc(force + base, 4 * nlive), l, f0);
+      vh_bstore_f(vh_rsrc(force + npad + base, 4 * nlive), l, f1);
+      vh_bstore_f(vh_rsrc(force + 2 * npad + base, 4 * nlive), l, f2);
+      {
+        const int np = pu.on && ok ? nlive : 0;
+        vh_pos nw;
+        const uint32_t key = push_record(rp, qp, f0, f1, f2, pu, nw);
+        vh_bstore_pos(vh_rsrc(P + 6 * (int64_t)base, 24 * np), l, nw);
+        vh_bstore_pos(vh_rsrc(Q + 6 * (int64_t)base, 24 * np), l, rp);
+        const uint32_t *kb = pu.on ? pu.keys + base : (const uint32_t *)P;
+        __builtin_amdgcn_raw_buffer_store_b32((int)key, vh_rsrc(kb, 4 * np), 4 * l, 0, 0);
       }
     }
   }
@@ -597,10 +667,11 @@ struct vh_fallback_args {
   const double *dev;  // device-resident projectile state (position at 0..2), or null
 };
 
-__global__ void k_fallback(const int32_t *__restrict__ P, int64_t npad, const int32_t *__restrict__ outlist,
+__global__ void k_fallback(int32_t *__restrict__ P, int64_t npad, const int32_t *__restrict__ outlist,
                            const int32_t *__restrict__ cnt, double x0, double h,
                            const float *__restrict__ csolc, const double *__restrict__ kc, int nkc,
-                           int nc, vh_fallback_args fa, float *__restrict__ force) {
+                           int nc, vh_fallback_args fa, vh_push_args pu, float *__restrict__ force) {
+  int32_t *Q = vh_prev_half(P, npad);
   const int nout = cnt[CNT_OUTCOUNT];
   if (fa.dev) {
     fa.px = fa.dev[0];
@@ -664,6 +735,13 @@ __global__ void k_fallback(const int32_t *__restrict__ P, int64_t npad, const in
     force[s] = (float)f0;
     force[npad + s] = (float)f1;
     force[2 * npad + s] = (float)f2;
+    if (pu.on) {
+      const vh_pos old = vh_load_pos(Q, s);
+      vh_pos nw;
+      pu.keys[s] = push_record(rp, old, (float)f0, (float)f1, (float)f2, pu, nw);
+      vh_store_pos(P, s, nw);
+      vh_store_pos(Q, s, rp);
+    }
   }
 }
 
@@ -812,9 +890,21 @@ int vh_energy_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, d
 }
 
 int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, const double *proj,
-                   int soft_mode, double ball_radius, const double *dproj) {
+                   int soft_mode, double ball_radius, const double *dproj, double push_a) {
   vh_context *c = a->ctx;
   hipStream_t st = c->stream;
+  vh_push_args pu;
+  pu.on = 0;
+  pu.a = pu.h = 0.f;
+  pu.nk = a->nk;
+  pu.keys = nullptr;
+  if (push_a > 0.0) {
+    VH_REQUIRE(!a->exact_field, "forces with the push: the matrix-core field kernel only");
+    pu.on = 1;
+    pu.a = (float)push_a;
+    pu.h = (float)a->h;
+    pu.keys = a->keys[0];
+  }
   vh_proj_args pa;
   pa.dev = dproj;
   pa.px = pa.py = pa.pz = 0.f;
@@ -846,7 +936,7 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
     hipLaunchKernelGGL(k_field_mfma, dim3(grid), dim3(64), 0, st, a->parts[a->cur], a->npad,
                        fine->csol, a->n, (const float4 *)a->og, a->zero, a->units, a->counters,
                        a->nk, (float)a->x0, (float)a->h, a->spacing, (float)a->nbdt,
-                       (float)weight, a->ncol, pa, a->force, a->dpart);
+                       (float)weight, a->ncol, pa, pu, a->force, a->dpart);
   vh_prof_end(c, tok);
   VH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_reduce_cols, dim3(4), dim3(1024), 0, st, a->dpart, grid, 4, a->dscal + 8);
@@ -864,8 +954,12 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
   fa.dev = dproj;
   hipLaunchKernelGGL(k_fallback, dim3(1024), dim3(256), 0, st, a->parts[a->cur], a->npad,
                      a->outlist, a->counters, a->x0, a->h, coarse->csol, coarse->knots64, coarse->nk,
-                     coarse->n, fa, a->force);
+                     coarse->n, fa, pu, a->force);
   VH_LAUNCH_CHECK();
+  if (pu.on) {
+    a->keys_fresh = 1;   // keys of the pushed records; the histograms are the sort's
+    a->hist_fresh = 0;
+  }
   return VH_OK;
 }
 
@@ -902,16 +996,6 @@ int vh_projectile_step_impl(vh_accel *a, double dt) {
 // ---------------------------------------------------------------------------
 // packed Verlet (src/kernels.jl:985-1025)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void verlet1(int kq, float dq, int ok, float od, float a, float f, float h,
-                                        int &kn, float &dn, float &gap) {
-  kn = 2 * kq - ok;
-  dn = 2.0f * dq - od + a * f;
-  const int m = (int)rintf(dn / h);
-  kn += m;
-  dn -= (float)m * h;
-  gap = (float)(kn - ok) * h + (dn - od);
-}
-
 // KEYS: the push also writes each particle's new cell key to keys[] and
 // counts it in the histogram of the first radix digit of its sort tile
 // (4096 consecutive slots), counts[d nblocks + tile], zero on entry, so that
@@ -1007,6 +1091,7 @@ int vh_verlet_impl(vh_accel *a, double dt, double mass, double rcmax) {
                        a->npad, a->npart, a->force, fa, fh, fx0, pfac, r2max, inv2M, a->dpart,
                        a->keys[0], a->counts, a->nk, 1 << bits0, a->nblocks);
     a->keys_fresh = 1;
+    a->hist_fresh = 1;
   } else {
     hipLaunchKernelGGL(k_verlet<false>, dim3(grid), dim3(256), 0, c->stream, a->parts[a->cur],
                        a->npad, a->npart, a->force, fa, fh, fx0, pfac, r2max, inv2M, a->dpart,

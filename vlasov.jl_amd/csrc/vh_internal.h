@@ -214,8 +214,8 @@ struct vh_accel {
   int64_t maxocc;              // min(ncell, npart)
   uint32_t *cellscratch;       // 4 x (maxocc + 2) scan scratch of the cell classification
   int sorted_once;             // host flag: the lists of a sort exist
-  int keys_fresh;              // host flag: vh_verlet left keys[0] and the first pass's
-                               // tile histograms of the current records in counts
+  int keys_fresh;              // host flag: the push left keys[0] of the current records
+  int hist_fresh;              // host flag: and the first pass's tile histograms in counts
   int exact_field;             // VH_OPT_EXACT_FIELD: the reference's FMA order
   int step_graph;              // VH_OPT_STEP_GRAPH
   hipGraphExec_t graph[2];     // vh_step recorded with parts[cur] = parts[0], parts[1]
@@ -293,8 +293,11 @@ int vh_spline_coefficients_impl(vh_mesh *m, const float *src, float *dst);
 int vh_effective_potential_impl(vh_mesh *m, double nions, double radius);
 int vh_potential_coefficients_impl(vh_mesh *m, double nions, double radius);
 int vh_total_charge_impl(vh_mesh *m, const float *rho, double *dq);
+// push_a > 0: the field and fallback kernels also push their particles
+// (Verlet factor dt^2 / mass) and leave the next sort's keys; requires the
+// matrix-core field kernel and vh_verlet_tiles
 int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, const double *proj,
-                   int soft_mode, double ball_radius, const double *dproj);
+                   int soft_mode, double ball_radius, const double *dproj, double push_a = 0.0);
 int vh_projectile_step_impl(vh_accel *a, double dt);
 int vh_verlet_impl(vh_accel *a, double dt, double mass, double rcmax);
 // the push leaves the next sort's keys and first-digit histograms

@@ -373,10 +373,10 @@ int vh_sort_impl(vh_accel *a) {
   VH_TRY(vh_fill_i32(c, a->counters, CNT_COUNT, 0));
   // --- radix passes over (key, index) ---
   int src = 0;
-  // after vh_verlet's tile pass the keys of the current records and the
-  // first pass's tile histograms are there already (keys[0], counts)
-  const bool fresh = a->keys_fresh;
-  a->keys_fresh = 0;
+  // after a push the keys of the current records are in keys[0], and after
+  // vh_verlet's tile pass also the first pass's tile histograms (counts)
+  const bool fresh = a->keys_fresh, hist = fresh && a->hist_fresh;
+  a->keys_fresh = a->hist_fresh = 0;
   for (int p = 0; p < a->passes; ++p) {
     int shift = p * a->digit_bits;
     int bits = a->digit_bits;
@@ -386,7 +386,7 @@ int vh_sort_impl(vh_accel *a) {
     // first: 1 keys from the records, 2 keys from keys[0]; both with
     // identity values
     int first = p == 0 ? (fresh ? 2 : 1) : 0;
-    if (first != 2) {
+    if (!(p == 0 && hist)) {
       hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)a->nblocks), dim3(SORT_THREADS),
                          nbins * sizeof(uint32_t), st, a->keys[src], a->parts[a->cur], a->npad,
                          first, a->nk, n, shift, nbins, a->nblocks, a->counts);
