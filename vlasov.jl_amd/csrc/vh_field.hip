@@ -138,18 +138,7 @@ __device__ __forceinline__ uint32_t push_record(const vh_pos &cur, const vh_pos 
   return cell_key0(nw.k[0], nw.k[1], nw.k[2], pu.nk);
 }
 
-// Stores through a buffer descriptor over `bytes` bytes at p: the hardware's
-// range check drops the stores of lanes past the end.  A chunk's stores are
-// then issued by every lane with no branch, so the compiler's count of the
-// memory operations in flight stays exact (with stores under a lane
-// condition it waits for every store before the next chunk's loads).
-typedef int vh_i2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t vh_rsrc(const void *p, int bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, bytes, 0x00020000);
-}
-__device__ __forceinline__ void vh_bstore_f(__amdgpu_buffer_rsrc_t r, int lane, float v) {
-  __builtin_amdgcn_raw_buffer_store_b32(__float_as_int(v), r, 4 * lane, 0, 0);
-}
+// the record of lane `lane` through a buffer of records (vh_rsrc)
 __device__ __forceinline__ void vh_bstore_pos(__amdgpu_buffer_rsrc_t r, int lane, const vh_pos &v) {
   const int o = 24 * lane;
   __builtin_amdgcn_raw_buffer_store_b64((vh_i2){v.k[0], v.k[1]}, r, o, 0, 0);

@@ -137,6 +137,26 @@ __device__ __forceinline__ void vh_load_delta(const int32_t *__restrict__ half, 
   dz = __int_as_float(c.y);
 }
 
+// Stores through a buffer descriptor over `bytes` bytes at p (bytes <
+// 2^31): the hardware's range check drops a store whose offset lies past the
+// end.  Stores of lanes that must not write (past a chunk, outside a grid)
+// are then issued by every lane with no branch, so the compiler's count of
+// the memory operations in flight stays exact: with a store under a lane or
+// wave condition it waits, at the next use of a later load, for every store
+// issued before that load as well.
+#define VH_OOB 0x7ffffff0
+typedef int vh_i2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t vh_rsrc(const void *p, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, bytes, 0x00020000);
+}
+__device__ __forceinline__ void vh_bstore_f(__amdgpu_buffer_rsrc_t r, int lane, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_int(v), r, 4 * lane, 0, 0);
+}
+// the store of float element `idx`, or none when !valid
+__device__ __forceinline__ void vh_bstore_at(__amdgpu_buffer_rsrc_t r, bool valid, int idx, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_int(v), r, valid ? 4 * idx : VH_OOB, 0, 0);
+}
+
 __device__ __forceinline__ void vh_store_pos(int32_t *__restrict__ half, int64_t s, const vh_pos &r) {
   int2 *p = reinterpret_cast<int2 *>(half + 6 * s);
   p[0] = make_int2(r.k[0], r.k[1]);

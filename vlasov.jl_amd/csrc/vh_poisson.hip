@@ -51,10 +51,9 @@ template <int KQ, int PF, int EPI>
 __global__ void __launch_bounds__(576)
     k_rot_gemm(const float *__restrict__ A, const float *__restrict__ src, float *__restrict__ out,
                int n, int m, int ldb, const float *__restrict__ aux, int nfull) {
-  // 32-bit indices throughout: n m < 2^31 (checked at launch)
+  // 32-bit indices throughout: 4 n m < 2^31 (checked at launch)
   extern __shared__ float Bs[];   // NP panels of NC x ldb
   constexpr int NC = 16;          // output columns of a panel
-  constexpr int NCT = 1;
   constexpr int NP = gemm_np(KQ);  // panels in flight per block
   const int nthr = blockDim.x;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -85,12 +84,18 @@ __global__ void __launch_bounds__(576)
   // LDS.  (Using the loaded value right away, or a load under a condition,
   // makes the compiler wait for the load at once, which serialises the
   // pipeline.)
+  // (relaxed atomic loads at workgroup scope: plain loads in the
+  // instruction stream, which the compiler does not move down to their
+  // first use, so a round's loads all stay ahead of its first wait)
+  const unsigned long long *src8 = (const unsigned long long *)src2;
   auto fetch = [&](float2 (&pv)[PF], int q) {
     const int base2 = q * NC * n / 2;
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
       const int g = base2 + threadIdx.x + u * nthr;
-      pv[u] = src2[g < tot2 ? g : tot2 - 1];
+      const unsigned long long v =
+          __hip_atomic_load(src8 + (g < tot2 ? g : tot2 - 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      pv[u] = make_float2(__uint_as_float((unsigned)v), __uint_as_float((unsigned)(v >> 32)));
     }
   };
   // Every pair is written: the ones past the panel go to a spare LDS word
@@ -114,45 +119,37 @@ __global__ void __launch_bounds__(576)
     }
   };
   // the fused epilogue of output (orow, j), orow < n, j < m
+  // the fused epilogue of output (orow, j): stored only for orow < n, j < m
+  // (waves past the last row tile compute with its operands and store
+  // nothing), through a range-checked buffer with no branch
+  const __amdgpu_buffer_rsrc_t ro =
+      vh_rsrc(out, 4 * (EPI == EPI_FILL_INTERIOR ? nfull * nfull * nfull : n * m));
   auto emit = [&](int orow, int j, float val, float pv) {
-    const int idx = orow * m + j;
+    const bool valid = orow < n && j < m;
     if (EPI == EPI_STORE) {
-      out[idx] = val;
+      vh_bstore_at(ro, valid, orow * m + j, val);
     } else if (EPI == EPI_HADAMARD) {
-      out[idx] = val * pv;
+      vh_bstore_at(ro, valid, orow * m + j, val * pv);
     } else if (EPI == EPI_ACCUMULATE) {
-      out[idx] = pv + val;
+      vh_bstore_at(ro, valid, orow * m + j, pv + val);
     } else {  // EPI_FILL_INTERIOR: idx = i + n (jj + n kz) of the interior, kz = orow
       const int i = j % n, jj = j / n;
-      out[(i + 1) + nfull * ((jj + 1) + nfull * (orow + 1))] = val;
+      vh_bstore_at(ro, valid, (i + 1) + nfull * ((jj + 1) + nfull * (orow + 1)), val);
     }
   };
   // the products of panel q from LDS buffer B and the fused epilogue, whose
   // operands (the Hadamard factors, or the values to add to) are in pre
   auto panel = [&](int q, const float *B, const float (&pre)[4]) {
     const int j0 = q * NC;
-    if (r < mt) {
-      vh_f4 acc[NCT];
+    vh_f4 acc = (vh_f4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ct = 0; ct < NCT; ++ct) acc[ct] = (vh_f4){0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int s = 0; s < KQ; ++s) {
-        const int k = lk * KQ + s;  // k >= n: zero A, zero LDS padding
-#pragma unroll
-        for (int ct = 0; ct < NCT; ++ct)
-          acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], B[(16 * ct + lc) * ldb + k], acc[ct], 0, 0, 0);
-      }
-#pragma unroll
-      for (int ct = 0; ct < NCT; ++ct) {
-        const int j = j0 + ct * 16 + lc;
-        if (j >= m) continue;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int orow = r * 16 + 4 * lk + v;
-          if (orow < n) emit(orow, j, acc[ct][v], pre[v]);
-        }
-      }
+    for (int s = 0; s < KQ; ++s) {
+      const int k = lk * KQ + s;  // k >= n: zero A, zero LDS padding
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], B[lc * ldb + k], acc, 0, 0, 0);
     }
+    const int j = j0 + lc;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) emit(r * 16 + 4 * lk + v, j, acc[v], pre[v]);
   };
   // rows k in [n, ldb) of every buffer stay zero (the panels fill k < n)
   for (int e = threadIdx.x; e < NP * NC * (ldb - n); e += nthr) {
@@ -245,8 +242,8 @@ int vh_rotate_gemm(vh_context *c, const float *A, const float *src, float *out, 
                    int epi, const float *aux, int nfull) {
   const int mt = (n + 15) / 16;
   VH_REQUIRE(mt >= 1 && mt <= 17, "rotating product: n = %d beyond 272", n);
-  VH_REQUIRE(m >= 1 && (int64_t)n * m < ((int64_t)1 << 31),
-             "rotating product: n m = %lld beyond 32-bit indices", (long long)((int64_t)n * m));
+  VH_REQUIRE(m >= 1 && 4 * (int64_t)(nfull > 0 ? (int64_t)nfull * nfull * nfull : n * m) < VH_OOB,
+             "rotating product: n m = %lld beyond 32-bit byte offsets", (long long)((int64_t)n * m));
   VH_REQUIRE(n % 2 == 0 && ((uintptr_t)src & 7) == 0,
              "rotating product: n (%d) must be even and the source 8-byte aligned", n);
   const int kq = gemm_kq(n);
