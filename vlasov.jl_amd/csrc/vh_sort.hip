@@ -152,82 +152,118 @@ __device__ __forceinline__ uint32_t load_key(int mode, const uint32_t *__restric
   return keys[i];
 }
 
-// per-tile digit histogram, written digit-major: counts[d * nblocks + b]
+// per-tile digit histogram, written digit-major: counts[d * nblocks + b].
+// The tile's keys are loaded first (clamped addresses, so no load waits
+// under a condition).  Keys in a tile mostly come in runs of one cell (the
+// cloud is kept in cell order; the second pass reads the first pass's
+// output, in which equal low digits keep that order), so a wave's lanes
+// mostly share a digit: lanes sharing the digit of the first lane left add
+// with one LDS atomic, four rounds, then one atomic per lane left (same-
+// address atomics of a whole wave would serialise).
 __global__ void __launch_bounds__(SORT_THREADS)
     k_radix_hist(const uint32_t *__restrict__ keys, const int32_t *__restrict__ P, int64_t npad,
                  int first, int nk, int64_t n, int shift, int nbins, int64_t nblocks,
                  uint32_t *__restrict__ counts) {
   extern __shared__ uint32_t hist[];
   for (int d = threadIdx.x; d < nbins; d += SORT_THREADS) hist[d] = 0;
-  __syncthreads();
-  int64_t base = (int64_t)blockIdx.x * SORT_THREADS * SORT_ITEMS;
-  uint32_t mask = (uint32_t)nbins - 1;
+  const int64_t base = (int64_t)blockIdx.x * SORT_THREADS * SORT_ITEMS;
+  const uint32_t mask = (uint32_t)nbins - 1;
+  uint32_t kv[SORT_ITEMS];
+#pragma unroll
   for (int it = 0; it < SORT_ITEMS; ++it) {
-    int64_t i = base + it * SORT_THREADS + threadIdx.x;
-    if (i < n) {
-      uint32_t k = load_key(first, keys, P, npad, i, nk);
-      atomicAdd(&hist[(k >> shift) & mask], 1u);
+    const int64_t i = base + it * SORT_THREADS + threadIdx.x;
+    kv[it] = load_key(first, keys, P, npad, i < n ? i : n - 1, nk);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int it = 0; it < SORT_ITEMS; ++it) {
+    const bool valid = base + it * SORT_THREADS + threadIdx.x < n;
+    const uint32_t d = (kv[it] >> shift) & mask;
+    uint64_t left = __ballot(valid);
+    for (int r = 0; r < 4 && left; ++r) {
+      const int lead = __ffsll((unsigned long long)left) - 1;
+      const uint32_t d0 = __shfl(d, lead, 64);
+      const uint64_t same = __ballot(valid && d == d0) & left;
+      if (lane == lead) atomicAdd(&hist[d0], (uint32_t)__popcll(same));
+      left &= ~same;
     }
+    if (valid && (left >> lane & 1ull)) atomicAdd(&hist[d], 1u);
   }
   __syncthreads();
   for (int d = threadIdx.x; d < nbins; d += SORT_THREADS)
     counts[(int64_t)d * nblocks + blockIdx.x] = hist[d];
 }
 
-// stable scatter of one tile.  Element e of the tile is (round r, thread t),
-// e = r * 256 + t; within a round ranks follow (wave, lane) order.
+// Stable scatter of one tile of 4096 elements.  Each wave takes a
+// contiguous quarter (1024 elements, 16 rounds of 64) and ranks it alone:
+// within a round, lanes of equal digit by ballots (rank among them in lane
+// order); across its rounds, a running count per digit in a wave-private
+// LDS row.  Then, once per tile, the four waves' counts become per-wave
+// offsets (exclusive sums in wave order) and every element goes to
+// goff[d] + offset of its wave + its rank in the wave: the tile's order,
+// wave-major then round then lane, is the elements' order, so the scatter
+// is stable.  Three barriers per tile.
 __global__ void __launch_bounds__(SORT_THREADS)
     k_radix_scatter(const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
                     const int32_t *__restrict__ P, int64_t npad, int first, int nk, int64_t n,
                     int shift, int bits, int64_t nblocks, const uint32_t *__restrict__ offsets,
                     uint32_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out) {
   extern __shared__ uint32_t sm[];
-  int nbins = 1 << bits;
-  uint32_t *goff = sm;                 // nbins
-  uint32_t *run = sm + nbins;          // nbins
-  uint32_t *cnt = sm + 2 * nbins;      // 4 * nbins
+  const int nbins = 1 << bits;
+  uint32_t *goff = sm;           // nbins: the tile's first slot per digit
+  uint32_t *wrun = sm + nbins;   // 4 x nbins: per-wave counts, then per-wave offsets
   for (int d = threadIdx.x; d < nbins; d += SORT_THREADS) {
     goff[d] = offsets[(int64_t)d * nblocks + blockIdx.x];
-    run[d] = 0;
-    cnt[d] = cnt[nbins + d] = cnt[2 * nbins + d] = cnt[3 * nbins + d] = 0;
+    wrun[d] = wrun[nbins + d] = wrun[2 * nbins + d] = wrun[3 * nbins + d] = 0;
   }
-  __syncthreads();
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  uint32_t mask = (uint32_t)nbins - 1;
-  int64_t base = (int64_t)blockIdx.x * SORT_THREADS * SORT_ITEMS;
+  const uint32_t mask = (uint32_t)nbins - 1;
+  const int64_t wbase = (int64_t)blockIdx.x * SORT_THREADS * SORT_ITEMS + w * 64 * SORT_ITEMS;
+  uint32_t kk[SORT_ITEMS], vv[SORT_ITEMS];
+#pragma unroll
   for (int r = 0; r < SORT_ITEMS; ++r) {
-    int64_t i = base + r * SORT_THREADS + threadIdx.x;
-    bool valid = i < n;
-    uint32_t k = 0, v = 0, d = 0;
-    if (valid) {
-      k = load_key(first, keys_in, P, npad, i, nk);
-      v = first ? (uint32_t)i : vals_in[i];   // first pass (modes 1, 2): identity
-      d = (k >> shift) & mask;
-    }
+    const int64_t i = wbase + r * 64 + lane, ic = i < n ? i : n - 1;
+    kk[r] = load_key(first, keys_in, P, npad, ic, nk);
+    vv[r] = first ? (uint32_t)i : vals_in[ic];   // first pass (modes 1, 2): identity
+  }
+  __syncthreads();
+  uint32_t *mine = wrun + w * nbins;
+  uint32_t pos[SORT_ITEMS];
+#pragma unroll
+  for (int r = 0; r < SORT_ITEMS; ++r) {
+    const bool valid = wbase + r * 64 + lane < n;
+    const uint32_t d = (kk[r] >> shift) & mask;
     uint64_t peers = __ballot(valid);
     for (int b = 0; b < bits; ++b) {
-      uint64_t bb = __ballot((d >> b) & 1u);
+      const uint64_t bb = __ballot((d >> b) & 1u);
       peers &= ((d >> b) & 1u) ? bb : ~bb;
     }
-    uint32_t rank = (uint32_t)__popcll(peers & lt);
-    uint32_t tot = (uint32_t)__popcll(peers);
-    bool leader = valid && rank == 0;
-    if (leader) cnt[w * nbins + d] = tot;
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = run[d] + rank;
-      for (int ww = 0; ww < w; ++ww) pos += cnt[ww * nbins + d];
-      uint32_t dst = goff[d] + pos;
-      keys_out[dst] = k;
-      vals_out[dst] = v;
+    const uint32_t rank = (uint32_t)__popcll(peers & lt);
+    // every lane reads its digit's count before the group's first lane
+    // advances it (one wave: the LDS accesses are in program order)
+    const uint32_t before = mine[d];
+    pos[r] = before + rank;
+    if (valid && rank == 0) mine[d] = before + (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < nbins; d += SORT_THREADS) {
+    const uint32_t c0 = wrun[d], c1 = wrun[nbins + d], c2 = wrun[2 * nbins + d];
+    wrun[d] = 0;
+    wrun[nbins + d] = c0;
+    wrun[2 * nbins + d] = c0 + c1;
+    wrun[3 * nbins + d] = c0 + c1 + c2;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < SORT_ITEMS; ++r) {
+    if (wbase + r * 64 + lane < n) {
+      const uint32_t d = (kk[r] >> shift) & mask;
+      const uint32_t dst = goff[d] + mine[d] + pos[r];
+      keys_out[dst] = kk[r];
+      vals_out[dst] = vv[r];
     }
-    __syncthreads();
-    if (leader) {
-      atomicAdd(&run[d], tot);
-      cnt[w * nbins + d] = 0;
-    }
-    __syncthreads();
   }
 }
 
