@@ -285,7 +285,7 @@ extern "C" int vh_accel_destroy(vh_accel *a) {
   void *ptrs[] = {a->ovl, a->grad, a->nodes, a->og, a->ovl64, a->knots64, a->parts[0], a->parts[1], a->force,
                   a->keys[0], a->keys[1], a->vals[0], a->vals[1], a->counts, a->scan_tmp,
                   a->occupied, a->bounds, a->cellmap, a->fitlist, a->nonfit, a->nonfit_prefix,
-                  a->outlist, a->counters, a->flags, a->tiles, a->dpart, a->dscal, a->cellscratch,
+                  a->outlist, a->counters, a->cellrun, a->tiles, a->dpart, a->dscal, a->cellscratch,
                   a->dproj, a->units, a->unitstart, a->zero};
   for (void *p : ptrs) vh_free(p);
   delete a;
@@ -376,7 +376,7 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
   int64_t maxocc = a->ncell < npart ? a->ncell : npart;
   a->maxocc = maxocc;
   a->maxunits = maxocc + (npart + VH_UNIT - 1) / VH_UNIT;
-  MK(vh_alloc((void **)&a->cellscratch, sizeof(uint32_t) * 4 * (maxocc + 2)));
+  MK(vh_alloc((void **)&a->cellscratch, sizeof(uint32_t) * 5 * (a->ncell + 1)));
   MK(vh_alloc((void **)&a->units, sizeof(vh_unit) * (size_t)a->maxunits));
   MK(vh_alloc((void **)&a->unitstart, sizeof(int32_t) * (maxocc + 2)));
   MK(vh_alloc((void **)&a->occupied, sizeof(int32_t) * (maxocc + 1)));
@@ -387,7 +387,7 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
   MK(vh_alloc((void **)&a->nonfit_prefix, sizeof(int32_t) * (maxocc + 2)));
   MK(vh_alloc((void **)&a->outlist, np * 4));
   MK(vh_alloc((void **)&a->counters, sizeof(int32_t) * 64));
-  MK(vh_alloc((void **)&a->flags, (np + 1) * 4));
+  MK(vh_alloc((void **)&a->cellrun, sizeof(int32_t) * 2 * a->ncell));
   MK(vh_alloc((void **)&a->tiles, sizeof(float) * 512 * (size_t)a->maxunits));
   a->dpart_len = 1 << 16;
   MK(vh_alloc((void **)&a->dpart, sizeof(double) * 8 * a->dpart_len));
@@ -395,6 +395,7 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
   MK(vh_alloc((void **)&a->zero, sizeof(float) * 64));
   MK(vh_fill_f32(ctx, a->zero, 64, 0.f));
   MK(vh_fill_i32(ctx, a->counters, 64, 0));
+  MK(vh_fill_i32(ctx, a->cellrun, 2 * a->ncell, -1));
   MK(vh_fill_f32(ctx, a->force, (int64_t)np * 3, 0.f));
   MK(vh_fill_i32(ctx, a->parts[0], (int64_t)np * VH_REC_WORDS, 0));
   MK(vh_fill_i32(ctx, a->parts[1], (int64_t)np * VH_REC_WORDS, 0));

@@ -224,7 +224,8 @@ struct vh_accel {
   int32_t *nonfit_prefix;      // exclusive prefix of their particle counts (+ total)
   int32_t *outlist;            // 0-based slots of out-of-stencil particles
   int32_t *counters;           // CNT_*
-  uint32_t *flags;             // npad scratch (run starts)
+  int32_t *cellrun;            // per cell: first slot, end slot of its run in the sorted
+                               // cloud (first -1: no particle; reset after each sort)
   float *tiles;                // 512 per work unit (fine deposit)
   double *dpart;               // per-block FP64 partials of the reductions
   int64_t dpart_len;
@@ -232,7 +233,7 @@ struct vh_accel {
   int cic_shift;               // fixed-point scale 2^shift of the coarse deposit
   float lo, hi;                // E(knots[2] + h/2), E(knots[end-1] - h/2): deposit bounds
   int64_t maxocc;              // min(ncell, npart)
-  uint32_t *cellscratch;       // 4 x (maxocc + 2) scan scratch of the cell classification
+  uint32_t *cellscratch;       // 5 x (ncell + 1) scan scratch of the cell classification
   int sorted_once;             // host flag: the lists of a sort exist
   int keys_fresh;              // host flag: the push left keys[0] of the current records
   int hist_fresh;              // host flag: and the first pass's tile histograms in counts

@@ -292,98 +292,89 @@ __global__ void k_permute(const int32_t *__restrict__ src, int32_t *__restrict__
   if (vh_lane() == 0 && local) atomicAdd(&cnt[CNT_NOUT], local);
 }
 
-// run starts of the sorted keys
-__global__ void k_run_flags(const uint32_t *__restrict__ keys, int64_t n, uint32_t *flags) {
-  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
-       s += (int64_t)gridDim.x * blockDim.x)
-    flags[s] = (s == 0 || keys[s] != keys[s - 1]) ? 1u : 0u;
-}
-
-// flags hold the exclusive scan: occupied[g] = key, bounds[g] = first slot
-__global__ void k_run_write(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ idx,
-                            int64_t n, int32_t *occupied, int32_t *bounds, int32_t *cnt) {
+// The runs of the sorted keys, indexed by cell: run[2 c] = first slot,
+// run[2 c + 1] = end.  run[2 c] stays -1 for a cell with no particle
+// (k_cell_lists resets every entry it reads).
+__global__ void k_run_mark(const uint32_t *__restrict__ keys, int64_t n, int32_t *run) {
   for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
        s += (int64_t)gridDim.x * blockDim.x) {
-    bool start = (s == 0 || keys[s] != keys[s - 1]);
-    if (start) {
-      occupied[idx[s]] = (int32_t)keys[s];
-      bounds[idx[s]] = (int32_t)s;
-    }
-    if (s == n - 1) {
-      uint32_t nocc = idx[s] + (start ? 1u : 0u);
-      bounds[nocc] = (int32_t)n;
-      cnt[CNT_NOCC] = (int32_t)nocc;
-    }
+    const uint32_t k = keys[s];
+    if (s == 0 || keys[s - 1] != k) run[2 * (int64_t)k] = (int32_t)s;
+    if (s == n - 1 || keys[s + 1] != k) run[2 * (int64_t)k + 1] = (int32_t)(s + 1);
   }
 }
 
-// per occupied cell: does the 10^3 stencil fit (src/kernels.jl:1047-1052)?
-// Fitting cells enter the deposit map; the particle counts of the others are
-// compacted into the out-of-stencil list; every cell is cut into work units
-// of at most VH_UNIT slots.  Entries past n_occ are zero so that the
-// exclusive scans over cap + 1 entries end with the totals.
-__global__ void k_cell_classify(const int32_t *__restrict__ occupied, const int32_t *__restrict__ bounds,
-                                const int32_t *__restrict__ cnt, int nk, int n, int32_t *cellmap,
-                                uint32_t *fitidx, uint32_t *nfidx, uint32_t *nfpre, uint32_t *nunit,
-                                int64_t len) {
-  int nocc = cnt[CNT_NOCC];
-  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < len;
-       g += (int64_t)gridDim.x * blockDim.x) {
-    if (g >= nocc) {
-      fitidx[g] = 0;
-      nfidx[g] = 0;
-      nfpre[g] = 0;
-      nunit[g] = 0;
+// per cell: occupied?  does its 10^3 stencil fit (src/kernels.jl:1047-1052)?
+// Five flag arrays over ncell + 1 entries (the last zero, so that the
+// exclusive scans end with the totals): occupied, fitting, non-fitting,
+// particles of a non-fitting cell (the out-of-stencil list), work units of
+// at most VH_UNIT slots.
+__global__ void k_cell_classify(const int32_t *__restrict__ run, int ncell, int nk, int n,
+                                uint32_t *occidx, uint32_t *fitidx, uint32_t *nfidx, uint32_t *nfpre,
+                                uint32_t *nunit) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= ncell; c += gridDim.x * blockDim.x) {
+    const int first = c < ncell ? run[2 * (int64_t)c] : -1;
+    if (first < 0) {
+      occidx[c] = fitidx[c] = nfidx[c] = nfpre[c] = nunit[c] = 0u;
       continue;
     }
-    int c0 = occupied[g];
-    int kx = c0 % nk + 1, ky = (c0 / nk) % nk + 1, kz = c0 / (nk * nk) + 1;
-    int bx = 2 * kx - 5, by = 2 * ky - 5, bz = 2 * kz - 5;
-    bool ok = bx >= 1 && by >= 1 && bz >= 1 && bx + 9 <= n && by + 9 <= n && bz + 9 <= n;
-    const uint32_t count = (uint32_t)(bounds[g + 1] - bounds[g]);
-    fitidx[g] = ok ? 1u : 0u;
-    nfidx[g] = ok ? 0u : 1u;
-    nfpre[g] = ok ? 0u : count;
-    nunit[g] = (count + VH_UNIT - 1) / VH_UNIT;
-    cellmap[c0] = ok ? (int32_t)g : -1;
+    const int kx = c % nk + 1, ky = (c / nk) % nk + 1, kz = c / (nk * nk) + 1;
+    const int bx = 2 * kx - 5, by = 2 * ky - 5, bz = 2 * kz - 5;
+    const bool ok = bx >= 1 && by >= 1 && bz >= 1 && bx + 9 <= n && by + 9 <= n && bz + 9 <= n;
+    const uint32_t count = (uint32_t)(run[2 * (int64_t)c + 1] - first);
+    occidx[c] = 1u;
+    fitidx[c] = ok ? 1u : 0u;
+    nfidx[c] = ok ? 0u : 1u;
+    nfpre[c] = ok ? 0u : count;
+    nunit[c] = (count + VH_UNIT - 1) / VH_UNIT;
   }
 }
 
-// after the scans: scatter the fitting / non-fitting lists and the work
-// units, write the totals
-__global__ void k_cell_lists(const uint32_t *__restrict__ fitidx, const uint32_t *__restrict__ nfidx,
-                             const uint32_t *__restrict__ nfpre, const uint32_t *__restrict__ ustart,
-                             const int32_t *__restrict__ occupied, const int32_t *__restrict__ bounds,
-                             int32_t *fitlist, int32_t *nonfit, int32_t *nonfit_prefix,
-                             vh_unit *units, int32_t *unitstart, int32_t *cnt, int64_t len) {
-  int nocc = cnt[CNT_NOCC];
-  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < len;
-       g += (int64_t)gridDim.x * blockDim.x) {
-    if (g < nocc) {
-      if (fitidx[g + 1] != fitidx[g]) fitlist[fitidx[g]] = (int32_t)g;
-      if (nfidx[g + 1] != nfidx[g]) {
-        nonfit[nfidx[g]] = (int32_t)g;
-        nonfit_prefix[nfidx[g]] = (int32_t)nfpre[g];
-      }
-      const int32_t key = occupied[g], lo = bounds[g], hi = bounds[g + 1];
-      uint32_t u = ustart[g];
-      unitstart[g] = (int32_t)u;
-      for (int32_t s = lo; s < hi; s += VH_UNIT, ++u) {
-        vh_unit un;
-        un.key = key;
-        un.lo = s;
-        un.hi = hi - s > VH_UNIT ? s + VH_UNIT : hi;
-        un.g = (int32_t)g;
-        units[u] = un;
-      }
-    } else if (g == nocc) {
-      cnt[CNT_NFIT] = (int32_t)fitidx[g];
-      cnt[CNT_NNONFIT] = (int32_t)nfidx[g];
-      cnt[CNT_OUTCOUNT] = (int32_t)nfpre[g];
-      cnt[CNT_NUNITS] = (int32_t)ustart[g];
-      unitstart[g] = (int32_t)ustart[g];
-      nonfit_prefix[nfidx[g]] = (int32_t)nfpre[g];
+// After the scans: the occupied cells in key order with their bounds, the
+// deposit map, the fitting / non-fitting lists, the work units; the totals.
+__global__ void k_cell_lists(const uint32_t *__restrict__ occidx, const uint32_t *__restrict__ fitidx,
+                             const uint32_t *__restrict__ nfidx, const uint32_t *__restrict__ nfpre,
+                             const uint32_t *__restrict__ ustart, int32_t *run, int ncell, int64_t n,
+                             int32_t *occupied, int32_t *bounds, int32_t *cellmap, int32_t *fitlist,
+                             int32_t *nonfit, int32_t *nonfit_prefix, vh_unit *units,
+                             int32_t *unitstart, int32_t *cnt) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= ncell; c += gridDim.x * blockDim.x) {
+    if (c == ncell) {
+      const uint32_t nocc = occidx[c];
+      cnt[CNT_NOCC] = (int32_t)nocc;
+      bounds[nocc] = (int32_t)n;
+      cnt[CNT_NFIT] = (int32_t)fitidx[c];
+      cnt[CNT_NNONFIT] = (int32_t)nfidx[c];
+      cnt[CNT_OUTCOUNT] = (int32_t)nfpre[c];
+      cnt[CNT_NUNITS] = (int32_t)ustart[c];
+      unitstart[nocc] = (int32_t)ustart[c];
+      nonfit_prefix[nfidx[c]] = (int32_t)nfpre[c];
       cnt[CNT_SORTED] = 1;
+      continue;
+    }
+    const uint32_t g = occidx[c];
+    const bool occ = occidx[c + 1] != g, ok = fitidx[c + 1] != fitidx[c];
+    cellmap[c] = occ && ok ? (int32_t)g : -1;
+    if (!occ) continue;
+    const int32_t lo = run[2 * (int64_t)c], hi = run[2 * (int64_t)c + 1];
+    run[2 * (int64_t)c] = -1;   // for the next sort
+    occupied[g] = c;
+    bounds[g] = lo;
+    if (ok) {
+      fitlist[fitidx[c]] = (int32_t)g;
+    } else {
+      nonfit[nfidx[c]] = (int32_t)g;
+      nonfit_prefix[nfidx[c]] = (int32_t)nfpre[c];
+    }
+    uint32_t u = ustart[c];
+    unitstart[g] = (int32_t)u;
+    for (int32_t s = lo; s < hi; s += VH_UNIT, ++u) {
+      vh_unit un;
+      un.key = c;
+      un.lo = s;
+      un.hi = hi - s > VH_UNIT ? s + VH_UNIT : hi;
+      un.g = (int32_t)g;
+      units[u] = un;
     }
   }
 }
@@ -444,25 +435,19 @@ int vh_sort_impl(vh_accel *a) {
                      perm, n, a->npad, (float)a->x0, (float)a->h, a->lo, a->hi, a->counters);
   VH_LAUNCH_CHECK();
   a->cur = 1 - a->cur;
-  // --- occupied cells and bounds from the runs of sorted keys ---
-  hipLaunchKernelGGL(k_run_flags, dim3(4096), dim3(256), 0, st, skeys, n, a->flags);
+  // --- occupied cells, bounds, classification, lists ---
+  hipLaunchKernelGGL(k_run_mark, dim3(4096), dim3(256), 0, st, skeys, n, a->cellrun);
   VH_LAUNCH_CHECK();
-  VH_TRY(vh_scan_exclusive(c, a->flags, n, a->scan_tmp, a->scan_tmp_len, nullptr));
-  hipLaunchKernelGGL(k_run_write, dim3(4096), dim3(256), 0, st, skeys, a->flags, n, a->occupied,
-                     a->bounds, a->counters);
+  const int64_t len = a->ncell + 1;
+  uint32_t *occidx = a->cellscratch, *fitidx = occidx + len, *nfidx = fitidx + len,
+           *nfpre = nfidx + len, *nunit = nfpre + len;
+  hipLaunchKernelGGL(k_cell_classify, dim3(1024), dim3(256), 0, st, a->cellrun, (int)a->ncell, a->nk,
+                     a->n, occidx, fitidx, nfidx, nfpre, nunit);
   VH_LAUNCH_CHECK();
-  // --- classification of the occupied cells ---
-  int64_t len = a->maxocc + 1;
-  uint32_t *fitidx = a->cellscratch, *nfidx = fitidx + len, *nfpre = nfidx + len,
-           *nunit = nfpre + len;
-  VH_TRY(vh_fill_i32(c, a->cellmap, a->ncell, -1));
-  hipLaunchKernelGGL(k_cell_classify, dim3(1024), dim3(256), 0, st, a->occupied, a->bounds,
-                     a->counters, a->nk, a->n, a->cellmap, fitidx, nfidx, nfpre, nunit, len);
-  VH_LAUNCH_CHECK();
-  VH_TRY(scan_arrays(c, fitidx, len, 4, a->scan_tmp, a->scan_tmp_len));   // the four arrays at once
-  hipLaunchKernelGGL(k_cell_lists, dim3(1024), dim3(256), 0, st, fitidx, nfidx, nfpre, nunit,
-                     a->occupied, a->bounds, a->fitlist, a->nonfit, a->nonfit_prefix, a->units,
-                     a->unitstart, a->counters, len);
+  VH_TRY(scan_arrays(c, occidx, len, 5, a->scan_tmp, a->scan_tmp_len));   // the five at once
+  hipLaunchKernelGGL(k_cell_lists, dim3(1024), dim3(256), 0, st, occidx, fitidx, nfidx, nfpre, nunit,
+                     a->cellrun, (int)a->ncell, n, a->occupied, a->bounds, a->cellmap, a->fitlist,
+                     a->nonfit, a->nonfit_prefix, a->units, a->unitstart, a->counters);
   VH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_outlist, dim3(1024), dim3(256), 0, st, a->nonfit, a->nonfit_prefix,
                      a->bounds, a->counters, a->outlist);
