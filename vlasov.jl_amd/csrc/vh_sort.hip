@@ -47,7 +47,7 @@ __device__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t &excl, uint32_t *ld
   return tot;
 }
 
-// Several arrays at once: array a (blockIdx.y, or blockIdx.x for the sums)
+// Several arrays at once: array a (blockIdx.y)
 // is data + a * n with block sums at sums + a * (nb + 1).
 __global__ void k_scan_reduce(const uint32_t *__restrict__ data, int64_t n, uint32_t *sums,
                               int64_t nb) {
@@ -65,26 +65,17 @@ __global__ void k_scan_reduce(const uint32_t *__restrict__ data, int64_t n, uint
   if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
-// one block per array: exclusive scan of its nb sums in place, total at sums[nb]
-__global__ void k_scan_sums(uint32_t *sums, int64_t nb) {
-  __shared__ uint32_t lds[16];
-  sums += blockIdx.x * (nb + 1);
-  uint32_t carry = 0;
-  for (int64_t base = 0; base < nb; base += blockDim.x) {
-    int64_t i = base + threadIdx.x;
-    uint32_t v = i < nb ? sums[i] : 0;
-    uint32_t ex;
-    uint32_t tot = block_excl_scan_u32(v, ex, lds);
-    if (i < nb) sums[i] = carry + ex;
-    carry += tot;
-  }
-  if (threadIdx.x == 0) sums[nb] = carry;
-}
-
-__global__ void k_scan_down(uint32_t *data, int64_t n, const uint32_t *__restrict__ sums, int64_t nb) {
+// Each block adds the totals of the blocks before it itself (at most a few
+// thousand, read once per block), so no pass scans the block totals; the
+// last block writes the array's total at sums[nb].
+__global__ void k_scan_down(uint32_t *data, int64_t n, uint32_t *sums, int64_t nb) {
   __shared__ uint32_t lds[16];
   data += blockIdx.y * n;
   sums += blockIdx.y * (nb + 1);
+  uint32_t before = 0;
+  for (int64_t i = threadIdx.x; i < (int64_t)blockIdx.x; i += blockDim.x) before += sums[i];
+  uint32_t unused;
+  const uint32_t carry = block_excl_scan_u32(before, unused, lds);
   int64_t base = (int64_t)blockIdx.x * SCAN_THREADS * SCAN_ITEMS;
   // each thread owns SCAN_ITEMS consecutive elements
   int64_t i0 = base + (int64_t)threadIdx.x * SCAN_ITEMS;
@@ -95,12 +86,13 @@ __global__ void k_scan_down(uint32_t *data, int64_t n, const uint32_t *__restric
     s += v[it];
   }
   uint32_t ex;
-  block_excl_scan_u32(s, ex, lds);
-  uint32_t run = sums[blockIdx.x] + ex;
+  const uint32_t tot = block_excl_scan_u32(s, ex, lds);
+  uint32_t run = carry + ex;
   for (int it = 0; it < SCAN_ITEMS; ++it) {
     if (i0 + it < n) data[i0 + it] = run;
     run += v[it];
   }
+  if (blockIdx.x == nb - 1 && threadIdx.x == 0) sums[nb] = carry + tot;
 }
 
 // exclusive scans of `count` consecutive arrays of n entries each, in place
@@ -112,8 +104,6 @@ static int scan_arrays(vh_context *c, uint32_t *data, int64_t n, int count, uint
   if (nb == 0) return VH_OK;
   hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb, count), dim3(SCAN_THREADS), 0, c->stream,
                      data, n, tmp, nb);
-  VH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_scan_sums, dim3(count), dim3(1024), 0, c->stream, tmp, nb);
   VH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb, count), dim3(SCAN_THREADS), 0, c->stream,
                      data, n, tmp, nb);
