@@ -29,11 +29,12 @@ ROOT = Path(__file__).resolve().parent
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FIELD_FLOP_PER_PARTICLE = 4900.0      # 2300 FMA + 300 MUL of contract_spline_10
-# HBM bytes per k_field_mfma launch at 8e6 particles on 130^3, from two rocprofv3
-# counter passes over `bench.py --steps 20 --warmup 5` (FETCH_SIZE 54 532 KiB x 2.000
-# + WRITE_SIZE 102 888 KiB x 1.000, the factors that make a 1 GiB device copy read
-# and write 1 GiB); recorded with the command in profiles/hbm_traffic_xenon_8M_130.md
-FIELD_TRAFFIC_BYTES_8M = (103271 * 2.0 + 105417 * 1.0) * 1024
+# HBM bytes per k_field_mfma launch of the step (forces and push) at 8e6 particles
+# on 130^3, from two rocprofv3 counter passes over `bench.py --steps 20 --warmup 5`
+# (FETCH_SIZE 204 381 KiB x 2.000 + WRITE_SIZE 494 788 KiB x 1.000, the factors that
+# make a 1 GiB device copy read and write 1 GiB); recorded with the command in
+# profiles/hbm_traffic_xenon_8M_130.md
+FIELD_TRAFFIC_BYTES_8M = (204381 * 2.0 + 494788 * 1.0) * 1024
 METRIC = "ms/step & particle-steps/s @8M,130^3; capture-q err vs 80M plain run"
 
 
