@@ -80,6 +80,15 @@ extern "C" int vh_context_create(int device, vh_context **out) {
     vh_set_error("hipStreamCreate: %s", hipGetErrorString(e));
     return VH_EHIP;
   }
+  // the side stream at the default priority, like the main one
+  if ((e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming)) != hipSuccess) {
+    vh_set_error("vh_context_create: %s", hipGetErrorString(e));
+    hipStreamDestroy(c->stream);
+    delete c;
+    return VH_EHIP;
+  }
   hipDeviceProp_t prop;
   VH_CHECK_HIP(hipGetDeviceProperties(&prop, device));
   c->num_cus = prop.multiProcessorCount;
@@ -91,6 +100,10 @@ extern "C" int vh_context_destroy(vh_context *ctx) {
   if (!ctx) return VH_OK;
   VH_CHECK_HIP(hipSetDevice(ctx->device));
   VH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  VH_CHECK_HIP(hipStreamSynchronize(ctx->side));
+  VH_CHECK_HIP(hipEventDestroy(ctx->fork_ev));
+  VH_CHECK_HIP(hipEventDestroy(ctx->join_ev));
+  VH_CHECK_HIP(hipStreamDestroy(ctx->side));
   VH_CHECK_HIP(hipStreamDestroy(ctx->stream));
   delete ctx;
   return VH_OK;
@@ -961,9 +974,31 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
   // _update_forces_resident! (src/simulation.jl:472-531), then step!
   VH_TRY(vh_deposit_smoothed(a, fine, weight, nullptr));
   VH_TRY(vh_deposit_cic(a, coarse, weight));
-  VH_TRY(vh_poisson_nested(fine, coarse));
-  VH_TRY(vh_potential_coefficients(fine, jel_nions, jel_radius));
-  VH_TRY(vh_effective_potential(coarse, jel_nions, jel_radius));
+  vh_context *c = a->ctx;
+  if (c->prof_on & ((1 << VH_ST_POISSON) | (1 << VH_ST_SPLINE) | (1 << VH_ST_MEANFIELD))) {
+    // timing of these stages: the stages one after the other
+    VH_TRY(vh_poisson_nested(fine, coarse));
+    VH_TRY(vh_potential_coefficients(fine, jel_nions, jel_radius));
+    VH_TRY(vh_effective_potential(coarse, jel_nions, jel_radius));
+  } else {
+    VH_REQUIRE(fine->ctx == c && coarse->ctx == c && jel_radius > 0,
+               "vh_step: meshes on another context or jellium radius not positive");
+    VH_TRY(vh_poisson_coarse_part_impl(fine, coarse));
+    // The coarse mean field on the side stream beside the fine solve and the
+    // fine coefficients (no buffer in common): it adds to the coarse
+    // coefficients only after the fine boundary has read them.  Both chains
+    // are latency-bound launches that leave room on the compute units.
+    VH_CHECK_HIP(hipEventRecord(c->fork_ev, c->stream));
+    VH_CHECK_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
+    {
+      vh_on_side on(c);
+      VH_TRY(vh_effective_potential_impl(coarse, jel_nions, jel_radius));
+    }
+    VH_CHECK_HIP(hipEventRecord(c->join_ev, c->side));
+    VH_TRY(vh_poisson_fine_part_impl(fine));
+    VH_TRY(vh_potential_coefficients_impl(fine, jel_nions, jel_radius));
+    VH_CHECK_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
+  }
   if (step_fuses_push(a)) {
     // the field and fallback kernels push the particles they computed the
     // force of: the same records, forces and keys as forces then vh_verlet

@@ -57,6 +57,10 @@ struct vh_prof_rec {
 struct vh_context {
   int device;
   hipStream_t stream;
+  // a second stream for work of a step that is independent of the main
+  // chain (the coarse mean field beside the fine solve), joined by events
+  hipStream_t side;
+  hipEvent_t fork_ev, join_ev;
   int num_cus;
   // timing of regions with events on `stream`
   int prof_on;                          // mask of the stages timed
@@ -310,6 +314,17 @@ int vh_deposit_fine_impl(vh_accel *a, vh_mesh *fine, double charge);
 int vh_deposit_cic_impl(vh_accel *a, vh_mesh *coarse, double charge);
 int vh_poisson_single_impl(vh_mesh *m);
 int vh_poisson_nested_impl(vh_mesh *fine, vh_mesh *coarse);
+// the nested solve in two parts: the coarse level, its coefficients and the
+// fine boundary; then the fine interior
+int vh_poisson_coarse_part_impl(vh_mesh *fine, vh_mesh *coarse);
+int vh_poisson_fine_part_impl(vh_mesh *fine);
+// launches of the scope go to the context's side stream
+struct vh_on_side {
+  vh_context *c;
+  hipStream_t saved;
+  explicit vh_on_side(vh_context *ctx) : c(ctx), saved(ctx->stream) { c->stream = c->side; }
+  ~vh_on_side() { c->stream = saved; }
+};
 int vh_spline_coefficients_impl(vh_mesh *m, const float *src, float *dst);
 int vh_effective_potential_impl(vh_mesh *m, double nions, double radius);
 int vh_potential_coefficients_impl(vh_mesh *m, double nions, double radius);

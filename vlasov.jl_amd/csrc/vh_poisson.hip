@@ -670,7 +670,7 @@ int vh_spline_coefficients_impl(vh_mesh *m, const float *src, float *dst) {
   return VH_OK;
 }
 
-int vh_poisson_nested_impl(vh_mesh *fine, vh_mesh *coarse) {
+int vh_poisson_coarse_part_impl(vh_mesh *fine, vh_mesh *coarse) {
   vh_context *c = fine->ctx;
   VH_TRY(vh_poisson_single_impl(coarse));
   // coarse coefficients: the same S^-1 phi that spline_coefficients! of the
@@ -679,7 +679,14 @@ int vh_poisson_nested_impl(vh_mesh *fine, vh_mesh *coarse) {
   hipLaunchKernelGGL(k_boundary_from_coarse, dim3(512), dim3(256), 0, c->stream, fine->phi,
                      fine->colloc64, fine->n, coarse->csol, coarse->knots64, coarse->nk, coarse->n);
   VH_LAUNCH_CHECK();
-  VH_TRY(solve_interior(fine));
+  return VH_OK;
+}
+
+int vh_poisson_fine_part_impl(vh_mesh *fine) { return solve_interior(fine); }
+
+int vh_poisson_nested_impl(vh_mesh *fine, vh_mesh *coarse) {
+  VH_TRY(vh_poisson_coarse_part_impl(fine, coarse));
+  VH_TRY(vh_poisson_fine_part_impl(fine));
   return VH_OK;
 }
 
