@@ -298,7 +298,7 @@ extern "C" int vh_accel_destroy(vh_accel *a) {
   void *ptrs[] = {a->ovl, a->grad, a->nodes, a->og, a->ovl64, a->knots64, a->parts[0], a->parts[1], a->force,
                   a->keys[0], a->keys[1], a->vals[0], a->vals[1], a->counts, a->scan_tmp,
                   a->occupied, a->bounds, a->cellmap, a->fitlist, a->nonfit, a->nonfit_prefix,
-                  a->outlist, a->counters, a->cellrun, a->tiles, a->dpart, a->dscal, a->cellscratch,
+                  a->outlist, a->counters, a->cellrun, a->tiles, a->cicwins, a->dpart, a->dscal, a->cellscratch,
                   a->dproj, a->units, a->unitstart, a->zero};
   for (void *p : ptrs) vh_free(p);
   delete a;
@@ -402,6 +402,7 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
   MK(vh_alloc((void **)&a->counters, sizeof(int32_t) * 64));
   MK(vh_alloc((void **)&a->cellrun, sizeof(int32_t) * 2 * a->ncell));
   MK(vh_alloc((void **)&a->tiles, sizeof(float) * 512 * (size_t)a->maxunits));
+  MK(vh_alloc((void **)&a->cicwins, sizeof(long long) * 64 * (size_t)a->maxunits));
   a->dpart_len = 1 << 16;
   MK(vh_alloc((void **)&a->dpart, sizeof(double) * 8 * a->dpart_len));
   MK(vh_alloc((void **)&a->dscal, sizeof(double) * 64));
@@ -972,22 +973,39 @@ extern "C" int vh_step(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weigh
 static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, double dt,
                       double mass, double rcmax, double jel_nions, double jel_radius) {
   // _update_forces_resident! (src/simulation.jl:472-531), then step!
-  VH_TRY(vh_deposit_smoothed(a, fine, weight, nullptr));
-  VH_TRY(vh_deposit_cic(a, coarse, weight));
   vh_context *c = a->ctx;
-  if (c->prof_on & ((1 << VH_ST_POISSON) | (1 << VH_ST_SPLINE) | (1 << VH_ST_MEANFIELD))) {
+  const int staged = (1 << VH_ST_DEPOSIT_FINE) | (1 << VH_ST_DEPOSIT_CIC) | (1 << VH_ST_POISSON) |
+                     (1 << VH_ST_SPLINE) | (1 << VH_ST_MEANFIELD);
+  if (c->prof_on & staged) {
     // timing of these stages: the stages one after the other
+    VH_TRY(vh_deposit_smoothed(a, fine, weight, nullptr));
+    VH_TRY(vh_deposit_cic(a, coarse, weight));
     VH_TRY(vh_poisson_nested(fine, coarse));
     VH_TRY(vh_potential_coefficients(fine, jel_nions, jel_radius));
     VH_TRY(vh_effective_potential(coarse, jel_nions, jel_radius));
   } else {
-    VH_REQUIRE(fine->ctx == c && coarse->ctx == c && jel_radius > 0,
-               "vh_step: meshes on another context or jellium radius not positive");
-    VH_TRY(vh_poisson_coarse_part_impl(fine, coarse));
-    // The coarse mean field on the side stream beside the fine solve and the
-    // fine coefficients (no buffer in common): it adds to the coarse
-    // coefficients only after the fine boundary has read them.  Both chains
-    // are latency-bound launches that leave room on the compute units.
+    VH_REQUIRE(fine->n == a->n && fine->ctx == c && coarse->ctx == c && jel_radius > 0,
+               "vh_step: meshes do not match the accelerator, or jellium radius not positive");
+    VH_TRY(sort_timed(a));
+    // Chains that share no buffer run side by side on the two streams: the
+    // coarse level (CIC, solve, coefficients) beside the fine deposit; then
+    // the coarse mean field beside the fine solve and coefficients, forked
+    // after the fine boundary has read the coarse coefficients it adds to.
+    // The rotating products and small kernels of the side chains are
+    // latency-bound launches that fill room the main chain leaves.  (With
+    // the CIC on the main stream before the fork, the fine deposit starved
+    // a product of the coarse solve: 18 us slower a step.)
+    VH_CHECK_HIP(hipEventRecord(c->fork_ev, c->stream));
+    VH_CHECK_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
+    {
+      vh_on_side on(c);
+      VH_TRY(vh_deposit_cic_impl(a, coarse, weight));
+      VH_TRY(vh_poisson_coarse_level_impl(coarse));
+    }
+    VH_CHECK_HIP(hipEventRecord(c->join_ev, c->side));
+    VH_TRY(vh_deposit_fine_impl(a, fine, weight));
+    VH_CHECK_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
+    VH_TRY(vh_poisson_fine_boundary_impl(fine, coarse));
     VH_CHECK_HIP(hipEventRecord(c->fork_ev, c->stream));
     VH_CHECK_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
     {

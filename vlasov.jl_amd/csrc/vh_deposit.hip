@@ -540,7 +540,7 @@ int vh_deposit_cic_impl(vh_accel *a, vh_mesh *m, double charge) {
   const double scale = ldexp(1.0, a->cic_shift);
   // the fine deposit's tiles (2 KiB per unit, consumed by then) hold the
   // 512-byte windows of units of multi-unit cells
-  long long *wins = (long long *)a->tiles;
+  long long *wins = a->cicwins;
   const size_t lds = cic_lds_bytes(m->n, m->nbin);
   VH_REQUIRE(lds <= 65536, "vh_deposit_cic: coarse locate table too large (%d bins)", m->nbin);
   const int grid = (int)((a->maxunits + CIC_WAVES - 1) / CIC_WAVES < 16384

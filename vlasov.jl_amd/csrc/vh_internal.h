@@ -231,6 +231,7 @@ struct vh_accel {
   int32_t *cellrun;            // per cell: first slot, end slot of its run in the sorted
                                // cloud (first -1: no particle; reset after each sort)
   float *tiles;                // 512 per work unit (fine deposit)
+  long long *cicwins;          // 64 per work unit: the CIC windows of multi-unit cells
   double *dpart;               // per-block FP64 partials of the reductions
   int64_t dpart_len;
   double *dscal;               // [0..4] verlet diag, [8..11] field reduction, [16] q, [17] enclosed
@@ -317,6 +318,8 @@ int vh_poisson_nested_impl(vh_mesh *fine, vh_mesh *coarse);
 // the nested solve in two parts: the coarse level, its coefficients and the
 // fine boundary; then the fine interior
 int vh_poisson_coarse_part_impl(vh_mesh *fine, vh_mesh *coarse);
+int vh_poisson_coarse_level_impl(vh_mesh *coarse);   // solve and coefficients
+int vh_poisson_fine_boundary_impl(vh_mesh *fine, vh_mesh *coarse);
 int vh_poisson_fine_part_impl(vh_mesh *fine);
 // launches of the scope go to the context's side stream
 struct vh_on_side {

@@ -670,15 +670,25 @@ int vh_spline_coefficients_impl(vh_mesh *m, const float *src, float *dst) {
   return VH_OK;
 }
 
-int vh_poisson_coarse_part_impl(vh_mesh *fine, vh_mesh *coarse) {
-  vh_context *c = fine->ctx;
+int vh_poisson_coarse_level_impl(vh_mesh *coarse) {
   VH_TRY(vh_poisson_single_impl(coarse));
   // coarse coefficients: the same S^-1 phi that spline_coefficients! of the
   // coarse level computes afterwards, kept in its csol
   VH_TRY(vh_spline_coefficients_impl(coarse, coarse->phi, coarse->csol));
+  return VH_OK;
+}
+
+int vh_poisson_fine_boundary_impl(vh_mesh *fine, vh_mesh *coarse) {
+  vh_context *c = fine->ctx;
   hipLaunchKernelGGL(k_boundary_from_coarse, dim3(512), dim3(256), 0, c->stream, fine->phi,
                      fine->colloc64, fine->n, coarse->csol, coarse->knots64, coarse->nk, coarse->n);
   VH_LAUNCH_CHECK();
+  return VH_OK;
+}
+
+int vh_poisson_coarse_part_impl(vh_mesh *fine, vh_mesh *coarse) {
+  VH_TRY(vh_poisson_coarse_level_impl(coarse));
+  VH_TRY(vh_poisson_fine_boundary_impl(fine, coarse));
   return VH_OK;
 }
 
