@@ -284,13 +284,34 @@ __global__ void k_permute(const int32_t *__restrict__ src, int32_t *__restrict__
 
 // The runs of the sorted keys, indexed by cell: run[2 c] = first slot,
 // run[2 c + 1] = end.  run[2 c] stays -1 for a cell with no particle
-// (k_cell_lists resets every entry it reads).
+// (k_cell_lists resets every entry it reads).  A thread takes four
+// consecutive keys (one 16-byte load) and the key on each side.
 __global__ void k_run_mark(const uint32_t *__restrict__ keys, int64_t n, int32_t *run) {
-  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
-       s += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t k = keys[s];
-    if (s == 0 || keys[s - 1] != k) run[2 * (int64_t)k] = (int32_t)s;
-    if (s == n - 1 || keys[s + 1] != k) run[2 * (int64_t)k + 1] = (int32_t)(s + 1);
+  const int64_t nq = (n + 3) / 4;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nq;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s0 = 4 * q;
+    uint32_t k[6];
+    if (s0 + 4 <= n && ((uintptr_t)keys & 15) == 0) {
+      const uint4 v = *reinterpret_cast<const uint4 *>(keys + s0);
+      k[1] = v.x;
+      k[2] = v.y;
+      k[3] = v.z;
+      k[4] = v.w;
+    } else {
+      for (int j = 0; j < 4; ++j) k[1 + j] = s0 + j < n ? keys[s0 + j] : 0u;
+    }
+    k[0] = s0 > 0 ? keys[s0 - 1] : ~0u;
+    k[5] = s0 + 4 < n ? keys[s0 + 4] : ~0u;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t s = s0 + j;
+      if (s >= n) break;
+      const uint32_t kk = k[1 + j];
+      // (keys are cells < 2^31: ~0 never equals one)
+      if (s == 0 || k[j] != kk) run[2 * (int64_t)kk] = (int32_t)s;
+      if (s == n - 1 || k[2 + j] != kk) run[2 * (int64_t)kk + 1] = (int32_t)(s + 1);
+    }
   }
 }
 
