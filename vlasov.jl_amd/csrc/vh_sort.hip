@@ -147,9 +147,10 @@ __device__ __forceinline__ uint32_t load_key(int mode, const uint32_t *__restric
 // under a condition).  Keys in a tile mostly come in runs of one cell (the
 // cloud is kept in cell order; the second pass reads the first pass's
 // output, in which equal low digits keep that order), so a wave's lanes
-// mostly share a digit: lanes sharing the digit of the first lane left add
-// with one LDS atomic, four rounds, then one atomic per lane left (same-
-// address atomics of a whole wave would serialise).
+// often share a digit: lanes sharing the digit of the first lane add with
+// one LDS atomic, then one atomic per lane left (same-address atomics of a
+// whole wave would serialise; more rounds cost more than they save in the
+// second pass, whose rounds mix many digits).
 __global__ void __launch_bounds__(SORT_THREADS)
     k_radix_hist(const uint32_t *__restrict__ keys, const int32_t *__restrict__ P, int64_t npad,
                  int first, int nk, int64_t n, int shift, int nbins, int64_t nblocks,
@@ -171,10 +172,10 @@ __global__ void __launch_bounds__(SORT_THREADS)
     const bool valid = base + it * SORT_THREADS + threadIdx.x < n;
     const uint32_t d = (kv[it] >> shift) & mask;
     uint64_t left = __ballot(valid);
-    for (int r = 0; r < 4 && left; ++r) {
+    if (left) {
       const int lead = __ffsll((unsigned long long)left) - 1;
       const uint32_t d0 = __shfl(d, lead, 64);
-      const uint64_t same = __ballot(valid && d == d0) & left;
+      const uint64_t same = __ballot(valid && d == d0);
       if (lane == lead) atomicAdd(&hist[d0], (uint32_t)__popcll(same));
       left &= ~same;
     }
