@@ -255,9 +255,7 @@ __global__ void k_reduce_partials(const double *__restrict__ part, int nparts, i
                                   double *__restrict__ out) {
   __shared__ double lds[16];
   for (int v = 0; v < nvals; ++v) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[(int64_t)v * stride + i];
-    double r = vh_block_sum_d(s, lds);
+    double r = vh_block_sum_d(vh_thread_sum(part + (int64_t)v * stride, nparts), lds);
     if (threadIdx.x == 0) out[v] = r;
   }
 }

@@ -740,9 +740,7 @@ __global__ void k_reduce_cols(const double *__restrict__ part, int nparts, int n
                               double *__restrict__ out) {
   __shared__ double lds[16];
   for (int v = blockIdx.x; v < nvals; v += gridDim.x) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[(int64_t)v * nparts + i];
-    const double r = vh_block_sum_d(s, lds);
+    const double r = vh_block_sum_d(vh_thread_sum(part + (int64_t)v * nparts, nparts), lds);
     if (threadIdx.x == 0) out[v] = r;
   }
 }

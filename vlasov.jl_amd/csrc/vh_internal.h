@@ -275,6 +275,24 @@ __device__ __forceinline__ T vh_wave_sum(T v) {
   return v;
 }
 
+// this thread's share of sum p[0..n): p[i] for i = threadIdx.x + k blockDim.x,
+// eight loads in flight (eight accumulators, combined in a fixed order)
+__device__ __forceinline__ double vh_thread_sum(const double *__restrict__ p, int64_t n) {
+  double s[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) s[u] = 0.0;
+  const int64_t st = blockDim.x;
+  int64_t i = threadIdx.x;
+  for (; i + 7 * st < n; i += 8 * st) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s[u] += p[i + u * st];
+  }
+#pragma unroll
+  for (int u = 0; u < 7; ++u)
+    if (i + u * st < n) s[u] += p[i + u * st];
+  return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+}
+
 // fixed-order block reduction of a double (blockDim.x multiple of 64, <= 1024)
 __device__ __forceinline__ double vh_block_sum_d(double v, double *lds) {
   v = vh_wave_sum(v);

@@ -378,10 +378,16 @@ __global__ void k_reduce_rows(const double *__restrict__ part, int nparts, int n
                               double *__restrict__ out) {
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = blockDim.x >> 6;
   for (int v = w; v < nvals; v += nw) {
-    double s = 0.0;
-    for (int i = l; i < nparts; i += 64) s += part[(int64_t)v * nparts + i];
-    s = vh_wave_sum(s);
-    if (l == 0) out[v] = s;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    int i = l;
+    for (; i + 192 < nparts; i += 256)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) s[u] += part[(int64_t)v * nparts + i + 64 * u];
+#pragma unroll
+    for (int u = 0; u < 3; ++u)
+      if (i + 64 * u < nparts) s[u] += part[(int64_t)v * nparts + i + 64 * u];
+    const double t = vh_wave_sum((s[0] + s[1]) + (s[2] + s[3]));
+    if (l == 0) out[v] = t;
   }
 }
 
@@ -415,9 +421,15 @@ __global__ void __launch_bounds__(640)
                      double *__restrict__ mp) {
   __shared__ double sm[10];
   const int v = threadIdx.x >> 6, l = threadIdx.x & 63;
-  double s = 0.0;
-  for (int i = l; i < nparts; i += 64) s += part[(int64_t)v * nparts + i];
-  s = vh_wave_sum(s);
+  double q[4] = {0.0, 0.0, 0.0, 0.0};
+  int i = l;
+  for (; i + 192 < nparts; i += 256)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) q[u] += part[(int64_t)v * nparts + i + 64 * u];
+#pragma unroll
+  for (int u = 0; u < 3; ++u)
+    if (i + 64 * u < nparts) q[u] += part[(int64_t)v * nparts + i + 64 * u];
+  const double s = vh_wave_sum((q[0] + q[1]) + (q[2] + q[3]));
   if (l == 0) {
     sm[v] = s;
     mom[v] = s;
