@@ -994,8 +994,12 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
     // The rotating products and small kernels of the side chains are
     // latency-bound launches that fill room the main chain leaves.  (With
     // the CIC on the main stream before the fork, the fine deposit starved
-    // a product of the coarse solve: 18 us slower a step.)
+    // a product of the coarse solve: 20 us slower a step.)
+    // (the main chain's few launches are submitted before the side chain's
+    // many: the host takes a few microseconds a launch, and the main stream
+    // would idle meanwhile)
     VH_CHECK_HIP(hipEventRecord(c->fork_ev, c->stream));
+    VH_TRY(vh_deposit_fine_impl(a, fine, weight));
     VH_CHECK_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
     {
       vh_on_side on(c);
@@ -1003,18 +1007,17 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
       VH_TRY(vh_poisson_coarse_level_impl(coarse));
     }
     VH_CHECK_HIP(hipEventRecord(c->join_ev, c->side));
-    VH_TRY(vh_deposit_fine_impl(a, fine, weight));
     VH_CHECK_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
     VH_TRY(vh_poisson_fine_boundary_impl(fine, coarse));
     VH_CHECK_HIP(hipEventRecord(c->fork_ev, c->stream));
+    VH_TRY(vh_poisson_fine_part_impl(fine));
+    VH_TRY(vh_potential_coefficients_impl(fine, jel_nions, jel_radius));
     VH_CHECK_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
     {
       vh_on_side on(c);
       VH_TRY(vh_effective_potential_impl(coarse, jel_nions, jel_radius));
     }
     VH_CHECK_HIP(hipEventRecord(c->join_ev, c->side));
-    VH_TRY(vh_poisson_fine_part_impl(fine));
-    VH_TRY(vh_potential_coefficients_impl(fine, jel_nions, jel_radius));
     VH_CHECK_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
   }
   if (step_fuses_push(a)) {
