@@ -360,6 +360,27 @@ def test_verlet_bit_exact(setup, orc):
     assert np.array_equal(occ_d, occ + 1) and np.array_equal(bnd_d, bnd)
 
 
+def test_push_after_sort_gathers_previous_positions(setup, orc):
+    # The sort moves only the current positions; the previous ones stay in
+    # the other buffer until a push (or a second sort) gathers them through
+    # the sort's permutation.  A force upload in between stages its data in
+    # that buffer too.
+    s = setup
+    acc, rec = s["acc"], s["rec"]
+    nk = s["fine"].axis.nknots
+    rng = np.random.default_rng(11)
+    f = (rng.standard_normal((NPART, 3)) * 1e-3).astype(np.float32)
+    for sorts in (1, 2):
+        acc.upload(rec)
+        for _ in range(sorts):
+            acc.sort()
+        want = rec[orc.cellsort(rec, nk)[0]].copy()
+        acc.upload_forces(f)
+        acc.verlet(0.5, s["w"], 100.0)
+        orc.verlet_f32(want, f, 0.5, s["w"], s["h"], s["x0"], 100.0)
+        assert acc.download().tobytes() == want.tobytes(), f"{sorts} sort(s)"
+
+
 def test_enclosed_count(setup, orc):
     s = setup
     acc = s["acc"]

@@ -62,6 +62,18 @@ int vh_fill_i32(vh_context *c, int32_t *p, int64_t n, int32_t v) { return fill_i
 int vh_fill_u64(vh_context *c, uint64_t *p, int64_t n, uint64_t v) { return fill_impl(c, p, n, v); }
 int vh_fill_f32(vh_context *c, float *p, int64_t n, float v) { return fill_impl(c, p, n, v); }
 
+__global__ void k_iota(uint32_t *p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = (uint32_t)i;
+}
+
+static int fill_iota(vh_context *c, uint32_t *p, int64_t n) {
+  hipLaunchKernelGGL(k_iota, dim3(1024), dim3(256), 0, c->stream, p, n);
+  VH_LAUNCH_CHECK();
+  return VH_OK;
+}
+
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
@@ -299,7 +311,7 @@ extern "C" int vh_accel_destroy(vh_accel *a) {
                   a->keys[0], a->keys[1], a->vals[0], a->vals[1], a->counts, a->scan_tmp,
                   a->occupied, a->bounds, a->cellmap, a->fitlist, a->nonfit, a->nonfit_prefix,
                   a->outlist, a->counters, a->cellrun, a->tiles, a->cicwins, a->dpart, a->dscal, a->cellscratch,
-                  a->dproj, a->units, a->unitstart, a->zero};
+                  a->dproj, a->units, a->unitstart, a->zero, a->ident};
   for (void *p : ptrs) vh_free(p);
   delete a;
   return VH_OK;
@@ -407,6 +419,7 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
   MK(vh_alloc((void **)&a->dpart, sizeof(double) * 8 * a->dpart_len));
   MK(vh_alloc((void **)&a->dscal, sizeof(double) * 64));
   MK(vh_alloc((void **)&a->zero, sizeof(float) * 64));
+  MK(vh_alloc((void **)&a->ident, np * 4));
   MK(vh_fill_f32(ctx, a->zero, 64, 0.f));
   MK(vh_fill_i32(ctx, a->counters, 64, 0));
   MK(vh_fill_i32(ctx, a->cellrun, 2 * a->ncell, -1));
@@ -414,6 +427,7 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
   MK(vh_fill_i32(ctx, a->parts[0], (int64_t)np * VH_REC_WORDS, 0));
   MK(vh_fill_i32(ctx, a->parts[1], (int64_t)np * VH_REC_WORDS, 0));
   MK(vh_fill_u64(ctx, (uint64_t *)a->dscal, 64, 0ull));
+  MK(fill_iota(ctx, a->ident, (int64_t)np));
 #undef MK
   VH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   a->cur = 0;
@@ -491,6 +505,7 @@ extern "C" int vh_particles_upload(vh_accel *a, const void *records) {
   VH_CHECK_HIP(hipSetDevice(a->ctx->device));
   int32_t *stage = a->parts[1 - a->cur];
   a->keys_fresh = 0;  // new records: the keys of the last push are stale
+  a->prev_pending = 0;
   VH_CHECK_HIP(hipMemcpyAsync(stage, records, (size_t)a->npart * 48, hipMemcpyHostToDevice,
                               a->ctx->stream));
   hipLaunchKernelGGL(k_records_in, dim3(2048), dim3(256), 0, a->ctx->stream, stage,
@@ -503,6 +518,7 @@ extern "C" int vh_particles_upload(vh_accel *a, const void *records) {
 extern "C" int vh_particles_download(vh_accel *a, void *records) {
   VH_REQUIRE(a && records, "vh_particles_download: null argument");
   VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  VH_TRY(vh_prev_settle(a));  // the staging below overwrites the other buffer
   int32_t *stage = a->parts[1 - a->cur];
   hipLaunchKernelGGL(k_records_out, dim3(2048), dim3(256), 0, a->ctx->stream, a->parts[a->cur],
                      stage, a->npart, a->npad);
@@ -928,7 +944,7 @@ extern "C" int vh_step(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weigh
                                     (double)(uintptr_t)fine, (double)(uintptr_t)coarse,
                                     (double)a->has_dproj, (double)a->exact_field, a->proj_charge,
                                     a->proj_scale, (double)a->proj_soft_mode, a->proj_ball_radius,
-                                    (double)(a->keys_fresh + 2 * a->hist_fresh)};
+                                    (double)(a->keys_fresh + 2 * a->hist_fresh + 4 * a->prev_pending)};
   if (memcmp(key, a->graph_key, sizeof key) != 0) {
     drop_graphs(a);
     memcpy(a->graph_key, key, sizeof key);
@@ -967,6 +983,7 @@ extern "C" int vh_step(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weigh
   // vh_verlet pushed
   a->keys_fresh = step_fuses_push(a) || vh_verlet_tiles(a) ? 1 : 0;
   a->hist_fresh = !step_fuses_push(a) && vh_verlet_tiles(a) ? 1 : 0;
+  a->prev_pending = 0;
   return VH_OK;
 }
 

@@ -126,6 +126,8 @@ struct vh_push_args {
   float a, h;          // dt^2 / mass, the cell width
   int nk;              // cells per axis (the key)
   uint32_t *keys;      // new cell key per slot
+  const int32_t *qsrc;    // the previous positions: slot s's is record qperm[s] of qsrc
+  const uint32_t *qperm;
 };
 
 __device__ __forceinline__ uint32_t push_record(const vh_pos &cur, const vh_pos &old, float f0,
@@ -480,6 +482,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     // the first chunk's records; each later chunk's are loaded during the
     // previous chunk's contraction
     vh_pos rnext = vh_load_pos(P, lo + l < hi ? lo + l : lo);
+    uint32_t qnext = pu.qperm[lo + l < hi ? lo + l : lo];
     for (int base = lo; base < hi; base += 64) {
       const int nlive = min(64, hi - base);
       // phase A, one lane per particle: offsets, the six table columns of the
@@ -489,7 +492,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
       const bool mine = l < nlive;
       const vh_pos rp = rnext;
       // the previous positions, for the push after the contraction
-      const vh_pos qp = vh_load_pos(Q, mine ? s : lo);
+      const vh_pos qp = vh_load_pos(pu.qsrc, qnext);
       const float dx0 = rp.d[0], dy0 = rp.d[1], dz0 = rp.d[2];
       // everything that reads the record comes before the first branch (a
       // load still in flight at a branch is waited for with every later one)
@@ -560,6 +563,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
       {
         const int sn = base + 64 + l;
         rnext = vh_load_pos(P, sn < hi ? sn : lo);
+        qnext = pu.qperm[sn < hi ? sn : lo];
       }
       // phase B: the contraction, one sub-group of 16 particles at a time
       // (two at a time, ft_contract<2>, measured no faster at twice the
@@ -725,7 +729,7 @@ __global__ void k_fallback(int32_t *__restrict__ P, int64_t npad, const int32_t 
     force[npad + s] = (float)f1;
     force[2 * npad + s] = (float)f2;
     if (pu.on) {
-      const vh_pos old = vh_load_pos(Q, s);
+      const vh_pos old = vh_load_pos(pu.qsrc, pu.qperm[s]);
       vh_pos nw;
       pu.keys[s] = push_record(rp, old, (float)f0, (float)f1, (float)f2, pu, nw);
       vh_store_pos(P, s, nw);
@@ -885,6 +889,8 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
   pu.a = pu.h = 0.f;
   pu.nk = a->nk;
   pu.keys = nullptr;
+  pu.qsrc = vh_prev_src(a);
+  pu.qperm = vh_prev_perm(a);
   if (push_a > 0.0) {
     VH_REQUIRE(!a->exact_field, "forces with the push: the matrix-core field kernel only");
     pu.on = 1;
@@ -946,6 +952,7 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
   if (pu.on) {
     a->keys_fresh = 1;   // keys of the pushed records; the histograms are the sort's
     a->hist_fresh = 0;
+    a->prev_pending = 0;  // the push wrote the previous array in slot order
   }
   return VH_OK;
 }
@@ -994,7 +1001,8 @@ __global__ void __launch_bounds__(256)
     k_verlet(int32_t *__restrict__ P, int64_t npad, int64_t n, const float *__restrict__ force,
              float a, float h, float x0, float pfac, float r2max, float inv2M,
              double *__restrict__ part, uint32_t *__restrict__ keys, uint32_t *__restrict__ counts,
-             int nk, int nbins0, int64_t nblocks) {
+             int nk, int nbins0, int64_t nblocks, const int32_t *__restrict__ qsrc,
+             const uint32_t *__restrict__ qperm) {
   __shared__ double lds[8];
   int32_t *Q = vh_prev_half(P, npad);
   double ek = 0.0, eo = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
@@ -1006,7 +1014,7 @@ __global__ void __launch_bounds__(256)
     const bool live = i < n;
     uint32_t key = 0u;
     if (live) {
-      const vh_pos cur = vh_load_pos(P, i), old = vh_load_pos(Q, i);
+      const vh_pos cur = vh_load_pos(P, i), old = vh_load_pos(qsrc, qperm[i]);
       const int k1 = cur.k[0], k2 = cur.k[1], k3 = cur.k[2];
       const float d1 = cur.d[0], d2 = cur.d[1], d3 = cur.d[2];
       const int o1 = old.k[0], o2 = old.k[1], o3 = old.k[2];
@@ -1076,16 +1084,19 @@ int vh_verlet_impl(vh_accel *a, double dt, double mass, double rcmax) {
     VH_TRY(vh_fill_i32(c, (int32_t *)a->counts, (1 << bits0) * a->nblocks, 0));
     hipLaunchKernelGGL(k_verlet<true>, dim3(grid), dim3(256), 0, c->stream, a->parts[a->cur],
                        a->npad, a->npart, a->force, fa, fh, fx0, pfac, r2max, inv2M, a->dpart,
-                       a->keys[0], a->counts, a->nk, 1 << bits0, a->nblocks);
+                       a->keys[0], a->counts, a->nk, 1 << bits0, a->nblocks, vh_prev_src(a),
+                       vh_prev_perm(a));
     a->keys_fresh = 1;
     a->hist_fresh = 1;
   } else {
     hipLaunchKernelGGL(k_verlet<false>, dim3(grid), dim3(256), 0, c->stream, a->parts[a->cur],
                        a->npad, a->npart, a->force, fa, fh, fx0, pfac, r2max, inv2M, a->dpart,
-                       (uint32_t *)nullptr, (uint32_t *)nullptr, a->nk, 1, (int64_t)1);
+                       (uint32_t *)nullptr, (uint32_t *)nullptr, a->nk, 1, (int64_t)1,
+                       vh_prev_src(a), vh_prev_perm(a));
     a->keys_fresh = 0;
   }
   VH_LAUNCH_CHECK();
+  a->prev_pending = 0;
   hipLaunchKernelGGL(k_reduce_cols, dim3(5), dim3(256), 0, c->stream, a->dpart, grid, 5, a->dscal);
   VH_LAUNCH_CHECK();
   return VH_OK;

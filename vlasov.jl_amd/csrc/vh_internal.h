@@ -242,6 +242,14 @@ struct vh_accel {
   int sorted_once;             // host flag: the lists of a sort exist
   int keys_fresh;              // host flag: the push left keys[0] of the current records
   int hist_fresh;              // host flag: and the first pass's tile histograms in counts
+  // The previous positions after a sort: the sort moves only the current
+  // array, so while prev_pending is set slot s's previous position is record
+  // prev_perm[s] of the other buffer's previous array (in the order before
+  // the sort).  A push gathers it and writes it in slot order; other readers
+  // first call vh_prev_settle.  ident is the identity permutation.
+  int prev_pending;
+  const uint32_t *prev_perm;
+  uint32_t *ident;
   int exact_field;             // VH_OPT_EXACT_FIELD: the reference's FMA order
   int step_graph;              // VH_OPT_STEP_GRAPH
   hipGraphExec_t graph[2];     // vh_step recorded with parts[cur] = parts[0], parts[1]
@@ -329,6 +337,15 @@ int vh_rotate_gemm(vh_context *c, const float *Aop, const float *src, float *out
                    int epi, const float *aux, int nfull);
 
 int vh_sort_impl(vh_accel *a);
+// writes the pending previous positions into the current buffer, in slot order
+int vh_prev_settle(vh_accel *a);
+// where a push reads the previous positions: slot s's is record perm[s] of src
+static inline const int32_t *vh_prev_src(const vh_accel *a) {
+  return a->parts[a->prev_pending ? 1 - a->cur : a->cur] + 6 * a->npad;
+}
+static inline const uint32_t *vh_prev_perm(const vh_accel *a) {
+  return a->prev_pending ? a->prev_perm : a->ident;
+}
 int vh_deposit_fine_impl(vh_accel *a, vh_mesh *fine, double charge);
 int vh_deposit_cic_impl(vh_accel *a, vh_mesh *coarse, double charge);
 int vh_poisson_single_impl(vh_mesh *m);
