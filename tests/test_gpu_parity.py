@@ -360,25 +360,36 @@ def test_verlet_bit_exact(setup, orc):
     assert np.array_equal(occ_d, occ + 1) and np.array_equal(bnd_d, bnd)
 
 
-def test_push_after_sort_gathers_previous_positions(setup, orc):
-    # The sort moves only the current positions; the previous ones stay in
-    # the other buffer until a push (or a second sort) gathers them through
-    # the sort's permutation.  A force upload in between stages its data in
-    # that buffer too.
-    s = setup
-    acc, rec = s["acc"], s["rec"]
-    nk = s["fine"].axis.nknots
-    rng = np.random.default_rng(11)
-    f = (rng.standard_normal((NPART, 3)) * 1e-3).astype(np.float32)
-    for sorts in (1, 2):
-        acc.upload(rec)
-        for _ in range(sorts):
+def test_records_left_in_place_by_a_sort(config1, orc):
+    # After the field kernel's push the sort moves no record: slot s's record
+    # is read through the sort's permutation until the next push moves it or
+    # another reader (download, a second sort, vh_verlet) gathers it first.
+    p, fine, coarse, sm = config1
+    n = p.nparticles
+    w = p.nelectrons / n
+    nk = fine.axis.nknots
+    x0 = float(fine.axis.knots[0])
+    h = float((fine.axis.knots[-1] - fine.axis.knots[0]) / (nk - 1))
+    pos, mom = va.sample_thomas_fermi(va.read_radial_profile(), n, w, seed=-5)
+    sim = va.Simulation(p, None, tables=(fine, coarse, sm), positions=pos, momenta=mom)
+    try:
+        acc = sim.acc
+        rng = np.random.default_rng(11)
+        f = (rng.standard_normal((n, 3)) * 1e-3).astype(np.float32)
+        for case in ("download", "second sort", "verlet"):
+            sim.step()
+            rec = acc.download()
+            want = rec[orc.cellsort(rec, nk)[0]].copy()
             acc.sort()
-        want = rec[orc.cellsort(rec, nk)[0]].copy()
-        acc.upload_forces(f)
-        acc.verlet(0.5, s["w"], 100.0)
-        orc.verlet_f32(want, f, 0.5, s["w"], s["h"], s["x0"], 100.0)
-        assert acc.download().tobytes() == want.tobytes(), f"{sorts} sort(s)"
+            if case == "second sort":
+                acc.sort()
+            if case == "verlet":
+                acc.upload_forces(f)
+                acc.verlet(0.5, w, 100.0)
+                orc.verlet_f32(want, f, 0.5, w, h, x0, 100.0)
+            assert acc.download().tobytes() == want.tobytes(), case
+    finally:
+        sim.close()
 
 
 def test_enclosed_count(setup, orc):

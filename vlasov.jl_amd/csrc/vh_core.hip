@@ -431,6 +431,7 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
 #undef MK
   VH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   a->cur = 0;
+  a->nout_idx = CNT_NOUT;
   *out = a;
   return VH_OK;
 }
@@ -505,7 +506,8 @@ extern "C" int vh_particles_upload(vh_accel *a, const void *records) {
   VH_CHECK_HIP(hipSetDevice(a->ctx->device));
   int32_t *stage = a->parts[1 - a->cur];
   a->keys_fresh = 0;  // new records: the keys of the last push are stale
-  a->prev_pending = 0;
+  a->rec_pending = 0;
+  a->pushed_fused = 0;
   VH_CHECK_HIP(hipMemcpyAsync(stage, records, (size_t)a->npart * 48, hipMemcpyHostToDevice,
                               a->ctx->stream));
   hipLaunchKernelGGL(k_records_in, dim3(2048), dim3(256), 0, a->ctx->stream, stage,
@@ -518,7 +520,7 @@ extern "C" int vh_particles_upload(vh_accel *a, const void *records) {
 extern "C" int vh_particles_download(vh_accel *a, void *records) {
   VH_REQUIRE(a && records, "vh_particles_download: null argument");
   VH_CHECK_HIP(hipSetDevice(a->ctx->device));
-  VH_TRY(vh_prev_settle(a));  // the staging below overwrites the other buffer
+  VH_TRY(vh_rec_settle(a));
   int32_t *stage = a->parts[1 - a->cur];
   hipLaunchKernelGGL(k_records_out, dim3(2048), dim3(256), 0, a->ctx->stream, a->parts[a->cur],
                      stage, a->npart, a->npad);
@@ -621,7 +623,7 @@ struct vh_region {
 // public wrappers of the step's stages
 // ---------------------------------------------------------------------------
 static int read_counters(vh_accel *a, int32_t *h) {
-  VH_CHECK_HIP(hipMemcpyAsync(h, a->counters, sizeof(int32_t) * CNT_COUNT, hipMemcpyDeviceToHost,
+  VH_CHECK_HIP(hipMemcpyAsync(h, a->counters, sizeof(int32_t) * CNT_READ, hipMemcpyDeviceToHost,
                               a->ctx->stream));
   VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
   return VH_OK;
@@ -639,7 +641,7 @@ extern "C" int vh_sort(vh_accel *a, int32_t *n_occupied, int32_t *outcount) {
   VH_CHECK_HIP(hipSetDevice(a->ctx->device));
   VH_TRY(sort_timed(a));
   if (n_occupied || outcount) {
-    int32_t h[CNT_COUNT];
+    int32_t h[CNT_READ];
     VH_TRY(read_counters(a, h));
     if (n_occupied) *n_occupied = h[CNT_NOCC];
     if (outcount) *outcount = h[CNT_OUTCOUNT];
@@ -663,7 +665,7 @@ extern "C" int vh_sort_lists(vh_accel *a, int32_t *occupied, int32_t *bounds, in
   VH_REQUIRE(a && occupied && bounds && outlist, "vh_sort_lists: null argument");
   VH_REQUIRE(a->sorted_once, "vh_sort_lists: no sort has run");
   VH_CHECK_HIP(hipSetDevice(a->ctx->device));
-  int32_t h[CNT_COUNT];
+  int32_t h[CNT_READ];
   VH_TRY(read_counters(a, h));
   int nocc = h[CNT_NOCC], nout = h[CNT_OUTCOUNT];
   int32_t *d = (int32_t *)a->parts[1 - a->cur];
@@ -711,6 +713,7 @@ __global__ void k_columns(const int32_t *__restrict__ P, int64_t npad, int64_t n
 extern "C" int vh_columns(vh_accel *a, int32_t *cols, int32_t *nout) {
   VH_REQUIRE(a && cols, "vh_columns: null argument");
   VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  VH_TRY(vh_rec_settle(a));
   int32_t *dcols = (int32_t *)a->parts[1 - a->cur];
   VH_TRY(vh_fill_i32(a->ctx, a->counters + 32, 1, 0));
   hipLaunchKernelGGL(k_columns, dim3(2048), dim3(256), 0, a->ctx->stream, a->parts[a->cur],
@@ -738,9 +741,9 @@ extern "C" int vh_deposit_smoothed(vh_accel *a, vh_mesh *fine, double charge, in
     VH_TRY(vh_deposit_fine_impl(a, fine, charge));
   }
   if (nout) {
-    int32_t h[CNT_COUNT];
+    int32_t h[CNT_READ];
     VH_TRY(read_counters(a, h));
-    *nout = h[CNT_NOUT];
+    *nout = h[a->nout_idx];
   }
   return VH_OK;
 }
@@ -929,6 +932,28 @@ static void drop_graphs(vh_accel *a) {
 
 static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, double dt,
                       double mass, double rcmax, double jel_nions, double jel_radius);
+
+static vh_step_state host_state(const vh_accel *a) {
+  vh_step_state h;
+  h.cur = a->cur;
+  h.keys_fresh = a->keys_fresh;
+  h.hist_fresh = a->hist_fresh;
+  h.rec_pending = a->rec_pending;
+  h.pushed_fused = a->pushed_fused;
+  h.nout_idx = a->nout_idx;
+  h.rec_perm = a->rec_perm;
+  return h;
+}
+
+static void set_host_state(vh_accel *a, const vh_step_state &h) {
+  a->cur = h.cur;
+  a->keys_fresh = h.keys_fresh;
+  a->hist_fresh = h.hist_fresh;
+  a->rec_pending = h.rec_pending;
+  a->pushed_fused = h.pushed_fused;
+  a->nout_idx = h.nout_idx;
+  a->rec_perm = h.rec_perm;
+}
 // the field and fallback kernels push (the matrix-core field kernel)
 static bool step_fuses_push(const vh_accel *a) { return !a->exact_field; }
 
@@ -944,7 +969,8 @@ extern "C" int vh_step(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weigh
                                     (double)(uintptr_t)fine, (double)(uintptr_t)coarse,
                                     (double)a->has_dproj, (double)a->exact_field, a->proj_charge,
                                     a->proj_scale, (double)a->proj_soft_mode, a->proj_ball_radius,
-                                    (double)(a->keys_fresh + 2 * a->hist_fresh + 4 * a->prev_pending)};
+                                    (double)(a->keys_fresh + 2 * a->hist_fresh + 4 * a->rec_pending +
+                                             8 * a->pushed_fused)};
   if (memcmp(key, a->graph_key, sizeof key) != 0) {
     drop_graphs(a);
     memcpy(a->graph_key, key, sizeof key);
@@ -972,18 +998,14 @@ extern "C" int vh_step(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weigh
       vh_set_error("vh_step: graph instantiation: %s", hipGetErrorString(ei));
       return VH_EHIP;
     }
-    // recording left the host state of a step behind (current buffer
-    // swapped); the device work runs now
+    // recording left the host state of a step behind (current buffer,
+    // fresh keys, ...), kept for the replays; the device work runs now
+    a->graph_after[b] = host_state(a);
     VH_CHECK_HIP(hipGraphLaunch(a->graph[b], st));
     return VH_OK;
   }
   VH_CHECK_HIP(hipGraphLaunch(a->graph[b], st));
-  a->cur = 1 - b;  // what the recorded sort and push did on the host
-  // what the recorded push left: keys, and the histograms too when
-  // vh_verlet pushed
-  a->keys_fresh = step_fuses_push(a) || vh_verlet_tiles(a) ? 1 : 0;
-  a->hist_fresh = !step_fuses_push(a) && vh_verlet_tiles(a) ? 1 : 0;
-  a->prev_pending = 0;
+  set_host_state(a, a->graph_after[b]);  // what the recorded step did on the host
   return VH_OK;
 }
 
@@ -1122,6 +1144,7 @@ __global__ void k_enclosed(const int32_t *__restrict__ P, int64_t npad, int64_t 
 extern "C" int vh_enclosed_count(vh_accel *a, const double center[3], double radius, int64_t *count) {
   VH_REQUIRE(a && center && count, "vh_enclosed_count: null argument");
   VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  VH_TRY(vh_rec_settle(a));
   unsigned long long *d = (unsigned long long *)(a->dscal + 40);
   VH_TRY(vh_fill_u64(a->ctx, (uint64_t *)d, 1, 0));
   hipLaunchKernelGGL(k_enclosed, dim3(2048), dim3(256), 0, a->ctx->stream, a->parts[a->cur],

@@ -38,7 +38,8 @@ typedef float vh_f4d __attribute__((ext_vector_type(4)));
 // particles zero, so that every operand read is unconditional; the next
 // chunk's offsets are loaded while the current chunk is multiplied.
 __global__ void __launch_bounds__(64)
-    k_dep_tiles(const int32_t *__restrict__ P, int64_t npad, const float *__restrict__ nodes,
+    k_dep_tiles(const int32_t *__restrict__ P, const uint32_t *__restrict__ perm,
+                const float *__restrict__ nodes,
                 const vh_unit *__restrict__ units, const int32_t *__restrict__ cnt, int nk, int n,
                 float spacing, float nbdt, int ncol, float *__restrict__ tiles) {
   __shared__ float vals[DEP_ROW * DEP_STAGE];
@@ -58,8 +59,11 @@ __global__ void __launch_bounds__(64)
     vh_f4d acc[4];
 #pragma unroll
     for (int tt = 0; tt < 4; ++tt) acc[tt] = (vh_f4d){0.f, 0.f, 0.f, 0.f};
+    // slot s's record is record perm[s]; the index is loaded a chunk ahead
+    // of the record
     float dx, dy, dz;
-    vh_load_delta(P, lo + l < hi ? lo + l : lo, dx, dy, dz);
+    vh_load_delta(P, perm[lo + l < hi ? lo + l : lo], dx, dy, dz);
+    uint32_t inext = perm[lo + DEP_STAGE + l < hi ? lo + DEP_STAGE + l : lo];
     for (int cur = lo; cur < hi; cur += DEP_STAGE) {
       const int chunk = min(DEP_STAGE, hi - cur);
       const float live = l < chunk ? 1.f : 0.f;
@@ -70,8 +74,9 @@ __global__ void __launch_bounds__(64)
                    *n3 = (const float4 *)(nodes + c3 * 8);
       const float4 g[6] = {n1[0], n1[1], n2[0], n2[1], n3[0], n3[1]};
       {  // the next chunk's offsets, in flight during this chunk's products
-        const int sn = cur + DEP_STAGE + l;
-        vh_load_delta(P, sn < hi ? sn : lo, dx, dy, dz);
+        const int sn = cur + 2 * DEP_STAGE + l;
+        vh_load_delta(P, inext, dx, dy, dz);
+        inext = perm[sn < hi ? sn : lo];
       }
       __syncthreads();   // the previous chunk's products have read vals
       float *v = vals + DEP_ROW * l;
@@ -261,9 +266,12 @@ __global__ void k_reduce_partials(const double *__restrict__ part, int nparts, i
 }
 
 // rho *= E((N - nout) * charge / q)  (src/accelerator.jl:557)
+// nout: the rejection count; clear: the count the next push makes, zeroed
 __global__ void k_dep_scale(float *__restrict__ rho, int64_t len, const double *__restrict__ q,
-                            const int32_t *__restrict__ cnt, int64_t npart, double charge) {
-  const float sc = (float)((double)(npart - cnt[CNT_NOUT]) * charge / q[0]);
+                            const int32_t *__restrict__ nout, int32_t *__restrict__ clear,
+                            int64_t npart, double charge) {
+  const float sc = (float)((double)(npart - *nout) * charge / q[0]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *clear = 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len;
        i += (int64_t)gridDim.x * blockDim.x)
     rho[i] *= sc;
@@ -276,7 +284,7 @@ int vh_deposit_fine_impl(vh_accel *a, vh_mesh *fine, double charge) {
   // one wave per work unit up to 65536 units, then a stride over units
   const int grid = (int)(a->maxunits < 65536 ? a->maxunits : 65536);
   int tok = vh_prof_begin(c, VH_ST_DEPOSIT_KERNEL);
-  hipLaunchKernelGGL(k_dep_tiles, dim3(grid), dim3(64), 0, st, a->parts[a->cur], a->npad, a->nodes,
+  hipLaunchKernelGGL(k_dep_tiles, dim3(grid), dim3(64), 0, st, a->parts[a->cur], vh_rec_perm(a), a->nodes,
                      a->units, a->counters, a->nk, a->n, a->spacing, (float)a->nbdt, a->ncol,
                      a->tiles);
   vh_prof_end(c, tok);
@@ -298,7 +306,8 @@ int vh_deposit_fine_impl(vh_accel *a, vh_mesh *fine, double charge) {
   VH_LAUNCH_CHECK();
   int64_t len = (int64_t)a->n * a->n * a->n;
   hipLaunchKernelGGL(k_dep_scale, dim3(2048), dim3(256), 0, st, fine->rho, len, a->dscal + 16,
-                     a->counters, a->npart, charge);
+                     a->counters + a->nout_idx, a->counters + CNT_PUSH_NOUT + 1 - a->cur,
+                     a->npart, charge);
   VH_LAUNCH_CHECK();
   return VH_OK;
 }
@@ -360,7 +369,8 @@ static inline size_t cic_lds_bytes(int n, int nbin) {
 }
 
 __global__ void __launch_bounds__(64 * CIC_WAVES)
-    k_cic_tiles(const int32_t *__restrict__ P, int64_t npad, const vh_unit *__restrict__ units,
+    k_cic_tiles(const int32_t *__restrict__ P, const uint32_t *__restrict__ perm,
+                const vh_unit *__restrict__ units,
                 const int32_t *__restrict__ unitstart, const int32_t *__restrict__ cnt, int nkf,
                 float x0f, float hf, const float *__restrict__ gt_g, int n, float x0t, float iwt,
                 const int32_t *__restrict__ cells_g, int nbin, double scale,
@@ -390,13 +400,17 @@ __global__ void __launch_bounds__(64 * CIC_WAVES)
     }
     const bool anchored = anc[0] > 0 && anc[1] > 0 && anc[2] > 0;
     long long sum[4] = {0, 0, 0, 0};
-    vh_pos rn = vh_load_pos(P, un.lo + l < un.hi ? un.lo + l : un.lo);
+    // slot s's record is record perm[s]; the index is loaded a chunk ahead
+    // of the record
+    vh_pos rn = vh_load_pos(P, perm[un.lo + l < un.hi ? un.lo + l : un.lo]);
+    uint32_t inext = perm[un.lo + 64 + l < un.hi ? un.lo + 64 + l : un.lo];
     for (int c0 = un.lo; c0 < un.hi; c0 += 64) {
       const bool live = c0 + l < un.hi;
       const vh_pos r = rn;
       {  // the next chunk's records, in flight during this chunk's products
-        const int sn = c0 + 64 + l;
-        rn = vh_load_pos(P, sn < un.hi ? sn : un.lo);
+        const int sn = c0 + 128 + l;
+        rn = vh_load_pos(P, inext);
+        inext = perm[sn < un.hi ? sn : un.lo];
       }
       float X[4] = {0.f, 0.f, 0.f, 0.f}, Y[4] = {0.f, 0.f, 0.f, 0.f}, Z[4] = {0.f, 0.f, 0.f, 0.f};
       int ix[3];
@@ -543,7 +557,7 @@ int vh_deposit_cic_impl(vh_accel *a, vh_mesh *m, double charge) {
   VH_REQUIRE(lds <= 65536, "vh_deposit_cic: coarse locate table too large (%d bins)", m->nbin);
   const int grid = (int)((a->maxunits + CIC_WAVES - 1) / CIC_WAVES < 16384
                              ? (a->maxunits + CIC_WAVES - 1) / CIC_WAVES : 16384);
-  hipLaunchKernelGGL(k_cic_tiles, dim3(grid), dim3(64 * CIC_WAVES), lds, st, a->parts[a->cur], a->npad,
+  hipLaunchKernelGGL(k_cic_tiles, dim3(grid), dim3(64 * CIC_WAVES), lds, st, a->parts[a->cur], vh_rec_perm(a),
                      a->units, a->unitstart, a->counters, a->nk, (float)a->x0, (float)a->h,
                      m->colloc, m->n, m->loc_x0, m->loc_iw, m->loc_cells, m->nbin, scale,
                      (unsigned long long *)m->acc64, wins);

@@ -126,9 +126,21 @@ struct vh_push_args {
   float a, h;          // dt^2 / mass, the cell width
   int nk;              // cells per axis (the key)
   uint32_t *keys;      // new cell key per slot
-  const int32_t *qsrc;    // the previous positions: slot s's is record qperm[s] of qsrc
-  const uint32_t *qperm;
+  int32_t *dst;        // the buffer the pushed records go to, in slot order
+  float x0, lo, hi;    // the deposit's domain: pushed records outside it are
+  int32_t *nout;       // counted here
 };
+
+// the deposit's rejection test of a record (k_permute's, same arithmetic)
+__device__ __forceinline__ bool push_outside(const vh_pos &r, const vh_push_args &pu) {
+  bool inside = true;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float p = pu.x0 + (float)(r.k[d] - 1) * pu.h + r.d[d];
+    inside = inside && (pu.lo <= p) && (p <= pu.hi);
+  }
+  return !inside;
+}
 
 __device__ __forceinline__ uint32_t push_record(const vh_pos &cur, const vh_pos &old, float f0,
                                                float f1, float f2, const vh_push_args &pu,
@@ -429,13 +441,18 @@ __device__ __forceinline__ void ft_contract(const float A[7][3], const float *ta
 }
 
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
-    k_field_mfma(int32_t *__restrict__ P, int64_t npad, const float *__restrict__ csol, int n,
+    k_field_mfma(const int32_t *__restrict__ P, const uint32_t *__restrict__ perm, int64_t npad,
+                 const float *__restrict__ csol, int n,
                  const float4 *__restrict__ og, const float *__restrict__ zero,
                  const vh_unit *__restrict__ units, const int32_t *__restrict__ cnt, int nk,
                  float x0, float h, float spacing, float nbdt, float w, int ncol,
                  vh_proj_args pa, vh_push_args pu, float *__restrict__ force,
                  double *__restrict__ part) {
-  int32_t *Q = vh_prev_half(P, npad);
+  // slot s's record is record perm[s] of P (both arrays); the push writes
+  // slot s of pu.dst
+  const int32_t *Q = vh_prev_half(P, npad);
+  int32_t *Pd = pu.dst, *Qd = vh_prev_half(pu.dst, npad);
+  int nout = 0;
   typedef ft_f4 f4;
   if (pa.dev) {
     pa.px = (float)pa.dev[0];
@@ -481,8 +498,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     }
     // the first chunk's records; each later chunk's are loaded during the
     // previous chunk's contraction
-    vh_pos rnext = vh_load_pos(P, lo + l < hi ? lo + l : lo);
-    uint32_t qnext = pu.qperm[lo + l < hi ? lo + l : lo];
+    // record indices are loaded a chunk ahead of the records
+    uint32_t inow = perm[lo + l < hi ? lo + l : lo];
+    vh_pos rnext = vh_load_pos(P, inow);
+    uint32_t inext = perm[lo + 64 + l < hi ? lo + 64 + l : lo];
     for (int base = lo; base < hi; base += 64) {
       const int nlive = min(64, hi - base);
       // phase A, one lane per particle: offsets, the six table columns of the
@@ -492,7 +511,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
       const bool mine = l < nlive;
       const vh_pos rp = rnext;
       // the previous positions, for the push after the contraction
-      const vh_pos qp = vh_load_pos(pu.qsrc, qnext);
+      const vh_pos qp = vh_load_pos(Q, inow);
       const float dx0 = rp.d[0], dy0 = rp.d[1], dz0 = rp.d[2];
       // everything that reads the record comes before the first branch (a
       // load still in flight at a branch is waited for with every later one)
@@ -562,8 +581,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
       __syncthreads();
       {
         const int sn = base + 64 + l;
-        rnext = vh_load_pos(P, sn < hi ? sn : lo);
-        qnext = pu.qperm[sn < hi ? sn : lo];
+        inow = inext;
+        rnext = vh_load_pos(P, inow);
+        inext = perm[sn + 64 < hi ? sn + 64 : lo];
       }
       // phase B: the contraction, one sub-group of 16 particles at a time
       // (two at a time, ft_contract<2>, measured no faster at twice the
@@ -587,10 +607,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
         const int np = pu.on && ok ? nlive : 0;
         vh_pos nw;
         const uint32_t key = push_record(rp, qp, f0, f1, f2, pu, nw);
-        vh_bstore_pos(vh_rsrc(P + 6 * (int64_t)base, 24 * np), l, nw);
-        vh_bstore_pos(vh_rsrc(Q + 6 * (int64_t)base, 24 * np), l, rp);
+        vh_bstore_pos(vh_rsrc(Pd + 6 * (int64_t)base, 24 * np), l, nw);
+        vh_bstore_pos(vh_rsrc(Qd + 6 * (int64_t)base, 24 * np), l, rp);
         const uint32_t *kb = pu.on ? pu.keys + base : (const uint32_t *)P;
         __builtin_amdgcn_raw_buffer_store_b32((int)key, vh_rsrc(kb, 4 * np), 4 * l, 0, 0);
+        nout += l < np && push_outside(nw, pu) ? 1 : 0;
       }
     }
   }
@@ -598,6 +619,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
   say = vh_wave_sum(say);
   saz = vh_wave_sum(saz);
   sae = vh_wave_sum(sae);
+  nout = vh_wave_sum(nout);
+  if (l == 0 && nout) atomicAdd(pu.nout, nout);
   if (l == 0) {
     part[blockIdx.x] = sax;
     part[gridDim.x + blockIdx.x] = say;
@@ -660,12 +683,16 @@ struct vh_fallback_args {
   const double *dev;  // device-resident projectile state (position at 0..2), or null
 };
 
-__global__ void k_fallback(int32_t *__restrict__ P, int64_t npad, const int32_t *__restrict__ outlist,
+__global__ void k_fallback(const int32_t *__restrict__ P, const uint32_t *__restrict__ perm,
+                           int64_t npad, const int32_t *__restrict__ outlist,
                            const int32_t *__restrict__ cnt, double x0, double h,
                            const float *__restrict__ csolc, const double *__restrict__ kc, int nkc,
                            int nc, vh_fallback_args fa, vh_push_args pu, float *__restrict__ force) {
-  int32_t *Q = vh_prev_half(P, npad);
+  // slot s's record is record perm[s] of P; the push writes slot s of pu.dst
+  const int32_t *Q = vh_prev_half(P, npad);
+  int32_t *Qd = vh_prev_half(pu.dst, npad);
   const int nout = cnt[CNT_OUTCOUNT];
+  int rejected = 0;
   if (fa.dev) {
     fa.px = fa.dev[0];
     fa.py = fa.dev[1];
@@ -674,8 +701,9 @@ __global__ void k_fallback(int32_t *__restrict__ P, int64_t npad, const int32_t 
   for (int64_t u = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; u < nout;
        u += (int64_t)gridDim.x * blockDim.x) {
     const int s = outlist[u];
+    const uint32_t i = perm[s];
     double p[3];
-    const vh_pos rp = vh_load_pos(P, s);
+    const vh_pos rp = vh_load_pos(P, i);
     for (int d = 0; d < 3; ++d) p[d] = x0 + (double)(rp.k[d] - 1) * h + (double)rp.d[d];
     int cell[3];
     bool out = false;
@@ -729,13 +757,16 @@ __global__ void k_fallback(int32_t *__restrict__ P, int64_t npad, const int32_t 
     force[npad + s] = (float)f1;
     force[2 * npad + s] = (float)f2;
     if (pu.on) {
-      const vh_pos old = vh_load_pos(pu.qsrc, pu.qperm[s]);
+      const vh_pos old = vh_load_pos(Q, i);
       vh_pos nw;
       pu.keys[s] = push_record(rp, old, (float)f0, (float)f1, (float)f2, pu, nw);
-      vh_store_pos(P, s, nw);
-      vh_store_pos(Q, s, rp);
+      vh_store_pos(pu.dst, s, nw);
+      vh_store_pos(Qd, s, rp);
+      rejected += push_outside(nw, pu) ? 1 : 0;
     }
   }
+  rejected = vh_wave_sum(rejected);
+  if (vh_lane() == 0 && rejected) atomicAdd(pu.nout, rejected);
 }
 
 // Sums each of `nvals` columns of `nparts` partials in a fixed order; one
@@ -870,6 +901,7 @@ int vh_energy_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, d
   ea.nkc = coarse->nk;
   ea.nc = coarse->n;
   ea.ncol = a->ncol;
+  VH_TRY(vh_rec_settle(a));
   hipLaunchKernelGGL(k_energy, dim3(ENERGY_BLOCKS), dim3(256), 0, c->stream, a->parts[a->cur],
                      a->npad, a->npart, fine->csol, a->knots64, a->ovl64, coarse->csol,
                      coarse->knots64, ea, a->dpart);
@@ -884,19 +916,28 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
                    int soft_mode, double ball_radius, const double *dproj, double push_a) {
   vh_context *c = a->ctx;
   hipStream_t st = c->stream;
+  // the exact-order kernel reads the records in slot order
+  if (a->exact_field) VH_TRY(vh_rec_settle(a));
   vh_push_args pu;
   pu.on = 0;
   pu.a = pu.h = 0.f;
   pu.nk = a->nk;
   pu.keys = nullptr;
-  pu.qsrc = vh_prev_src(a);
-  pu.qperm = vh_prev_perm(a);
+  pu.dst = a->parts[a->cur];
+  pu.x0 = (float)a->x0;
+  pu.lo = a->lo;
+  pu.hi = a->hi;
+  pu.nout = nullptr;
   if (push_a > 0.0) {
     VH_REQUIRE(!a->exact_field, "forces with the push: the matrix-core field kernel only");
     pu.on = 1;
     pu.a = (float)push_a;
     pu.h = (float)a->h;
     pu.keys = a->keys[0];
+    // the moved records go to the other buffer; the deposit that follows
+    // the next sort reads their rejection count from the slot of that buffer
+    pu.dst = a->parts[1 - a->cur];
+    pu.nout = a->counters + CNT_PUSH_NOUT + (1 - a->cur);
   }
   vh_proj_args pa;
   pa.dev = dproj;
@@ -926,8 +967,8 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
                        a->nk, (float)a->x0, (float)a->h, a->spacing, (float)a->nbdt,
                        (float)weight, a->ncol, pa, a->force, a->dpart);
   else
-    hipLaunchKernelGGL(k_field_mfma, dim3(grid), dim3(64), 0, st, a->parts[a->cur], a->npad,
-                       fine->csol, a->n, (const float4 *)a->og, a->zero, a->units, a->counters,
+    hipLaunchKernelGGL(k_field_mfma, dim3(grid), dim3(64), 0, st, a->parts[a->cur],
+                       vh_rec_perm(a), a->npad, fine->csol, a->n, (const float4 *)a->og, a->zero, a->units, a->counters,
                        a->nk, (float)a->x0, (float)a->h, a->spacing, (float)a->nbdt,
                        (float)weight, a->ncol, pa, pu, a->force, a->dpart);
   vh_prof_end(c, tok);
@@ -945,14 +986,16 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
   fa.w = weight;
   fa.escaped = 0.0;
   fa.dev = dproj;
-  hipLaunchKernelGGL(k_fallback, dim3(1024), dim3(256), 0, st, a->parts[a->cur], a->npad,
+  hipLaunchKernelGGL(k_fallback, dim3(1024), dim3(256), 0, st, a->parts[a->cur], vh_rec_perm(a), a->npad,
                      a->outlist, a->counters, a->x0, a->h, coarse->csol, coarse->knots64, coarse->nk,
                      coarse->n, fa, pu, a->force);
   VH_LAUNCH_CHECK();
   if (pu.on) {
-    a->keys_fresh = 1;   // keys of the pushed records; the histograms are the sort's
+    a->cur = 1 - a->cur;  // the pushed records, in slot order
+    a->rec_pending = 0;
+    a->keys_fresh = 1;    // keys of the pushed records; the histograms are the sort's
     a->hist_fresh = 0;
-    a->prev_pending = 0;  // the push wrote the previous array in slot order
+    a->pushed_fused = 1;
   }
   return VH_OK;
 }
@@ -1001,8 +1044,7 @@ __global__ void __launch_bounds__(256)
     k_verlet(int32_t *__restrict__ P, int64_t npad, int64_t n, const float *__restrict__ force,
              float a, float h, float x0, float pfac, float r2max, float inv2M,
              double *__restrict__ part, uint32_t *__restrict__ keys, uint32_t *__restrict__ counts,
-             int nk, int nbins0, int64_t nblocks, const int32_t *__restrict__ qsrc,
-             const uint32_t *__restrict__ qperm) {
+             int nk, int nbins0, int64_t nblocks) {
   __shared__ double lds[8];
   int32_t *Q = vh_prev_half(P, npad);
   double ek = 0.0, eo = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
@@ -1014,7 +1056,7 @@ __global__ void __launch_bounds__(256)
     const bool live = i < n;
     uint32_t key = 0u;
     if (live) {
-      const vh_pos cur = vh_load_pos(P, i), old = vh_load_pos(qsrc, qperm[i]);
+      const vh_pos cur = vh_load_pos(P, i), old = vh_load_pos(Q, i);
       const int k1 = cur.k[0], k2 = cur.k[1], k3 = cur.k[2];
       const float d1 = cur.d[0], d2 = cur.d[1], d3 = cur.d[2];
       const int o1 = old.k[0], o2 = old.k[1], o3 = old.k[2];
@@ -1071,6 +1113,7 @@ __global__ void __launch_bounds__(256)
 
 int vh_verlet_impl(vh_accel *a, double dt, double mass, double rcmax) {
   vh_context *c = a->ctx;
+  VH_TRY(vh_rec_settle(a));  // pushed in place, in slot order
   const float fa = (float)(dt * dt / mass);
   const float fh = (float)a->h, fx0 = (float)a->x0;
   const float pfac = (float)(mass / (2.0 * dt));
@@ -1084,19 +1127,17 @@ int vh_verlet_impl(vh_accel *a, double dt, double mass, double rcmax) {
     VH_TRY(vh_fill_i32(c, (int32_t *)a->counts, (1 << bits0) * a->nblocks, 0));
     hipLaunchKernelGGL(k_verlet<true>, dim3(grid), dim3(256), 0, c->stream, a->parts[a->cur],
                        a->npad, a->npart, a->force, fa, fh, fx0, pfac, r2max, inv2M, a->dpart,
-                       a->keys[0], a->counts, a->nk, 1 << bits0, a->nblocks, vh_prev_src(a),
-                       vh_prev_perm(a));
+                       a->keys[0], a->counts, a->nk, 1 << bits0, a->nblocks);
     a->keys_fresh = 1;
     a->hist_fresh = 1;
   } else {
     hipLaunchKernelGGL(k_verlet<false>, dim3(grid), dim3(256), 0, c->stream, a->parts[a->cur],
                        a->npad, a->npart, a->force, fa, fh, fx0, pfac, r2max, inv2M, a->dpart,
-                       (uint32_t *)nullptr, (uint32_t *)nullptr, a->nk, 1, (int64_t)1,
-                       vh_prev_src(a), vh_prev_perm(a));
+                       (uint32_t *)nullptr, (uint32_t *)nullptr, a->nk, 1, (int64_t)1);
     a->keys_fresh = 0;
   }
   VH_LAUNCH_CHECK();
-  a->prev_pending = 0;
+  a->pushed_fused = 0;
   hipLaunchKernelGGL(k_reduce_cols, dim3(5), dim3(256), 0, c->stream, a->dpart, grid, 5, a->dscal);
   VH_LAUNCH_CHECK();
   return VH_OK;

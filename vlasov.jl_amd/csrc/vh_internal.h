@@ -177,7 +177,12 @@ enum {
   CNT_NNONFIT,    // occupied cells that do not fit
   CNT_SORTED,     // 1 once a sort has run
   CNT_NUNITS,     // work units (see VH_UNIT)
-  CNT_COUNT
+  CNT_COUNT,
+  // not cleared by the sort: the rejections among the records a push wrote
+  // into parts[b] (slot CNT_PUSH_NOUT + b), counted by the push for the
+  // deposit of the next step
+  CNT_PUSH_NOUT = CNT_COUNT,
+  CNT_READ = CNT_PUSH_NOUT + 2  // the counters a host read copies
 };
 
 // Work units of the per-cell kernels: an occupied cell's particles in runs of
@@ -192,6 +197,13 @@ struct vh_unit {
 };
 
 #define VH_GRAPH_KEY 15
+
+// the host side of an accelerator that a step changes (vh_step's replays
+// restore what the recorded step left)
+struct vh_step_state {
+  int cur, keys_fresh, hist_fresh, rec_pending, pushed_fused, nout_idx;
+  const uint32_t *rec_perm;
+};
 
 struct vh_accel {
   vh_context *ctx;
@@ -242,18 +254,23 @@ struct vh_accel {
   int sorted_once;             // host flag: the lists of a sort exist
   int keys_fresh;              // host flag: the push left keys[0] of the current records
   int hist_fresh;              // host flag: and the first pass's tile histograms in counts
-  // The previous positions after a sort: the sort moves only the current
-  // array, so while prev_pending is set slot s's previous position is record
-  // prev_perm[s] of the other buffer's previous array (in the order before
-  // the sort).  A push gathers it and writes it in slot order; other readers
-  // first call vh_prev_settle.  ident is the identity permutation.
-  int prev_pending;
-  const uint32_t *prev_perm;
+  // Records in place after a sort: when the sort follows a push of the
+  // step (fresh keys, matrix-core field) it moves nothing; while rec_pending
+  // is set slot s's record (both arrays) is record rec_perm[s] of parts[cur]
+  // in the order before the sort.  The deposits and the field and fallback
+  // kernels read through the permutation, and the push writes the moved
+  // records in slot order into the other buffer; other readers first call
+  // vh_rec_settle.  ident is the identity permutation.
+  int rec_pending;
+  const uint32_t *rec_perm;
   uint32_t *ident;
+  int nout_idx;                // counters[nout_idx]: the deposit's rejections of this sort
+  int pushed_fused;            // host flag: the field kernel's push wrote the records
   int exact_field;             // VH_OPT_EXACT_FIELD: the reference's FMA order
   int step_graph;              // VH_OPT_STEP_GRAPH
   hipGraphExec_t graph[2];     // vh_step recorded with parts[cur] = parts[0], parts[1]
   double graph_key[15];        // the arguments they were recorded with (VH_GRAPH_KEY)
+  vh_step_state graph_after[2];  // the host state each recorded step left
   // the projectile kept on the device (vh_projectile_set)
   double *dproj;               // 16 doubles, see k_projectile_step
   int has_dproj;
@@ -337,14 +354,12 @@ int vh_rotate_gemm(vh_context *c, const float *Aop, const float *src, float *out
                    int epi, const float *aux, int nfull);
 
 int vh_sort_impl(vh_accel *a);
-// writes the pending previous positions into the current buffer, in slot order
-int vh_prev_settle(vh_accel *a);
-// where a push reads the previous positions: slot s's is record perm[s] of src
-static inline const int32_t *vh_prev_src(const vh_accel *a) {
-  return a->parts[a->prev_pending ? 1 - a->cur : a->cur] + 6 * a->npad;
-}
-static inline const uint32_t *vh_prev_perm(const vh_accel *a) {
-  return a->prev_pending ? a->prev_perm : a->ident;
+// moves the pending records into the other buffer in slot order (the
+// sort's gather) and makes it current
+int vh_rec_settle(vh_accel *a);
+// slot s's record is record vh_rec_perm(a)[s] of parts[cur]
+static inline const uint32_t *vh_rec_perm(const vh_accel *a) {
+  return a->rec_pending ? a->rec_perm : a->ident;
 }
 int vh_deposit_fine_impl(vh_accel *a, vh_mesh *fine, double charge);
 int vh_deposit_cic_impl(vh_accel *a, vh_mesh *coarse, double charge);

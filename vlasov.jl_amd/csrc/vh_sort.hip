@@ -262,15 +262,15 @@ __global__ void __launch_bounds__(SORT_THREADS)
 // record gather + deposit "nout" (columns test of src/kernels.jl:754-779)
 // ---------------------------------------------------------------------------
 __global__ void k_permute(const int32_t *__restrict__ src, int32_t *__restrict__ dst,
-                          const uint32_t *__restrict__ perm, int64_t n, float x0,
+                          const uint32_t *__restrict__ perm, int64_t n, int64_t npad, float x0,
                           float h, float lo, float hi, int32_t *cnt) {
   int local = 0;
   for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
        s += (int64_t)gridDim.x * blockDim.x) {
-    // the current array only: the previous positions are gathered through
-    // perm by the push that reads them
-    const vh_pos cur = vh_load_pos(src, perm[s]);
+    const int64_t i = perm[s];
+    const vh_pos cur = vh_load_pos(src, i), old = vh_load_pos(vh_prev_half(src, npad), i);
     vh_store_pos(dst, s, cur);
+    vh_store_pos(vh_prev_half(dst, npad), s, old);
     bool inside = true;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -404,20 +404,16 @@ __global__ void k_outlist(const int32_t *__restrict__ nonfit, const int32_t *__r
   }
 }
 
-// the previous array of the current buffer from the pending gather
-__global__ void k_prev_gather(const int32_t *__restrict__ src, const uint32_t *__restrict__ perm,
-                              int32_t *__restrict__ dst, int64_t n) {
-  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
-       s += (int64_t)gridDim.x * blockDim.x)
-    vh_store_pos(dst, s, vh_load_pos(src, perm[s]));
-}
-
-int vh_prev_settle(vh_accel *a) {
-  if (!a->prev_pending) return VH_OK;
-  hipLaunchKernelGGL(k_prev_gather, dim3(4096), dim3(256), 0, a->ctx->stream, vh_prev_src(a),
-                     a->prev_perm, a->parts[a->cur] + 6 * a->npad, a->npart);
+// the gather of a sort whose records stayed in place
+int vh_rec_settle(vh_accel *a) {
+  if (!a->rec_pending) return VH_OK;
+  hipLaunchKernelGGL(k_permute, dim3(4096), dim3(256), 0, a->ctx->stream, a->parts[a->cur],
+                     a->parts[1 - a->cur], a->rec_perm, a->npart, a->npad, (float)a->x0,
+                     (float)a->h, a->lo, a->hi, a->counters);
   VH_LAUNCH_CHECK();
-  a->prev_pending = 0;
+  a->cur = 1 - a->cur;
+  a->rec_pending = 0;
+  a->nout_idx = CNT_NOUT;
   return VH_OK;
 }
 
@@ -425,10 +421,16 @@ int vh_sort_impl(vh_accel *a) {
   vh_context *c = a->ctx;
   hipStream_t st = c->stream;
   const int64_t n = a->npart;
-  // the radix passes reuse the permutation buffers
-  VH_TRY(vh_prev_settle(a));
+  // records left in place by the last sort, whose permutation the radix
+  // passes overwrite
+  VH_TRY(vh_rec_settle(a));
   // counters of this step
   VH_TRY(vh_fill_i32(c, a->counters, CNT_COUNT, 0));
+  // after the field kernel's push the records stay where they are: the
+  // deposits and that kernel read them through the permutation, and the push
+  // moves them; it has counted the deposit's rejections among them
+  const bool in_place = a->pushed_fused;
+  a->pushed_fused = 0;
   // --- radix passes over (key, index) ---
   int src = 0;
   // after a push the keys of the current records are in keys[0], and after
@@ -461,13 +463,19 @@ int vh_sort_impl(vh_accel *a) {
   }
   const uint32_t *skeys = a->keys[src];
   const uint32_t *perm = a->vals[src];
-  // --- move the records; count the deposit's rejections on the way ---
-  hipLaunchKernelGGL(k_permute, dim3(4096), dim3(256), 0, st, a->parts[a->cur], a->parts[1 - a->cur],
-                     perm, n, (float)a->x0, (float)a->h, a->lo, a->hi, a->counters);
-  VH_LAUNCH_CHECK();
-  a->cur = 1 - a->cur;
-  a->prev_pending = 1;
-  a->prev_perm = perm;
+  if (in_place) {
+    a->rec_pending = 1;
+    a->rec_perm = perm;
+    a->nout_idx = CNT_PUSH_NOUT + a->cur;
+  } else {
+    // --- move the records; count the deposit's rejections on the way ---
+    hipLaunchKernelGGL(k_permute, dim3(4096), dim3(256), 0, st, a->parts[a->cur],
+                       a->parts[1 - a->cur], perm, n, a->npad, (float)a->x0, (float)a->h, a->lo,
+                       a->hi, a->counters);
+    VH_LAUNCH_CHECK();
+    a->cur = 1 - a->cur;
+    a->nout_idx = CNT_NOUT;
+  }
   // --- occupied cells, bounds, classification, lists ---
   hipLaunchKernelGGL(k_run_mark, dim3(4096), dim3(256), 0, st, skeys, n, a->cellrun);
   VH_LAUNCH_CHECK();
