@@ -31,10 +31,10 @@ FP32_PEAK_TFLOPS = 157.3
 FIELD_FLOP_PER_PARTICLE = 4900.0      # 2300 FMA + 300 MUL of contract_spline_10
 # HBM bytes per k_field_mfma launch of the step (forces and push) at 8e6 particles
 # on 130^3, from two rocprofv3 counter passes over `bench.py --steps 20 --warmup 5`
-# (FETCH_SIZE 204 381 KiB x 2.000 + WRITE_SIZE 494 788 KiB x 1.000, the factors that
-# make a 1 GiB device copy read and write 1 GiB); recorded with the command in
-# profiles/hbm_traffic_xenon_8M_130.md
-FIELD_TRAFFIC_BYTES_8M = (204381 * 2.0 + 494788 * 1.0) * 1024
+# (the average of the launches that push: FETCH_SIZE 321 636 KiB x 2.000 + WRITE_SIZE
+# 542 865 KiB x 1.000, the factors that make a 1 GiB device copy read and write 1 GiB);
+# recorded with the command in profiles/hbm_traffic_xenon_8M_130.md
+FIELD_TRAFFIC_BYTES_8M = (321636 * 2.0 + 542865 * 1.0) * 1024
 METRIC = "ms/step & particle-steps/s @8M,130^3; capture-q err vs 80M plain run"
 
 
@@ -222,7 +222,7 @@ def main():
                                f"dt=0.5, steps {args.warmup + 1}-{args.warmup + args.steps} of 700",
                    "particles": args.particles, "grid": 2 * args.nfine + 2, "softening": args.mode},
         "roofline": {"bound": "mfma",
-                     "engine": "FP32 MFMA 16x16x4 (table contraction) + VALU FP32 (second stage); the launch also pushes its particles (Verlet, keys of the next sort)",
+                     "engine": "FP32 MFMA 16x16x4 (table contraction) + VALU FP32 (second stage); the launch also pushes its particles (Verlet, keys of the next sort), reading the records through the sort's permutation and writing them in slot order",
                      "kernel": "k_field_mfma",
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,

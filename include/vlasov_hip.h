@@ -158,7 +158,11 @@ int vh_forces_upload(vh_accel *a, const float *host);
  * (src/kernels.jl:902-908), occupied cells and bounds, and the list of
  * particles whose 10^3 stencil leaves the fine grid (src/kernels.jl:1041-1058).
  * The records are moved; the order within a cell is the previous order
- * (stable, as cellsort!, src/sorting.jl:163-204).  Outputs may be NULL. */
+ * (stable, as cellsort!, src/sorting.jl:163-204).  After a vh_step the move
+ * is deferred to the next reader (the deposits and forces read through the
+ * sort's permutation, the next vh_step's push moves the records); every
+ * entry point that returns or reads records sees them in sorted order.
+ * Outputs may be NULL. */
 int vh_sort(vh_accel *a, int32_t *n_occupied, int32_t *outcount);
 /* The occupied-cell list (1-based keys, increasing), bounds (length n_occ+1,
  * exclusive prefix) and outlist (1-based sorted slots, increasing) of the
