@@ -407,6 +407,7 @@ __global__ void k_outlist(const int32_t *__restrict__ nonfit, const int32_t *__r
 // the gather of a sort whose records stayed in place
 int vh_rec_settle(vh_accel *a) {
   if (!a->rec_pending) return VH_OK;
+  VH_TRY(vh_fill_i32(a->ctx, a->counters + CNT_NOUT, 1, 0));
   hipLaunchKernelGGL(k_permute, dim3(4096), dim3(256), 0, a->ctx->stream, a->parts[a->cur],
                      a->parts[1 - a->cur], a->rec_perm, a->npart, a->npad, (float)a->x0,
                      (float)a->h, a->lo, a->hi, a->counters);
@@ -424,8 +425,8 @@ int vh_sort_impl(vh_accel *a) {
   // records left in place by the last sort, whose permutation the radix
   // passes overwrite
   VH_TRY(vh_rec_settle(a));
-  // counters of this step
-  VH_TRY(vh_fill_i32(c, a->counters, CNT_COUNT, 0));
+  // (the counters of the sort are written by k_cell_lists; only the gather's
+  // rejection count accumulates, and is cleared before it)
   // after the field kernel's push the records stay where they are: the
   // deposits and that kernel read them through the permutation, and the push
   // moves them; it has counted the deposit's rejections among them
@@ -469,6 +470,7 @@ int vh_sort_impl(vh_accel *a) {
     a->nout_idx = CNT_PUSH_NOUT + a->cur;
   } else {
     // --- move the records; count the deposit's rejections on the way ---
+    VH_TRY(vh_fill_i32(c, a->counters + CNT_NOUT, 1, 0));
     hipLaunchKernelGGL(k_permute, dim3(4096), dim3(256), 0, st, a->parts[a->cur],
                        a->parts[1 - a->cur], perm, n, a->npad, (float)a->x0, (float)a->h, a->lo,
                        a->hi, a->counters);
