@@ -1026,17 +1026,22 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
     VH_REQUIRE(fine->n == a->n && fine->ctx == c && coarse->ctx == c && jel_radius > 0,
                "vh_step: meshes do not match the accelerator, or jellium radius not positive");
     VH_TRY(sort_timed(a));
-    // Chains that share no buffer run side by side on the two streams: the
-    // coarse level (CIC, solve, coefficients) beside the fine deposit; then
-    // the coarse mean field beside the fine solve and coefficients, forked
-    // after the fine boundary has read the coarse coefficients it adds to.
-    // The rotating products and small kernels of the side chains are
+    // Chains that share no buffer run side by side on the two streams: on
+    // the side stream the CIC, the coarse level (solve, coefficients), the
+    // fine boundary read from the coarse coefficients and then the coarse
+    // mean field added to them; on the main stream the fine deposit, then,
+    // once the boundary is there, the fine solve and coefficients.  The
+    // rotating products and small kernels of the side chain are
     // latency-bound launches that fill room the main chain leaves.  (With
     // the CIC on the main stream before the fork, the fine deposit starved
-    // a product of the coarse solve: 20 us slower a step.)
-    // (the main chain's few launches are submitted before the side chain's
-    // many: the host takes a few microseconds a launch, and the main stream
-    // would idle meanwhile)
+    // a product of the coarse solve: 20 us slower a step.)  The host runs
+    // ahead of the device until a stream's queue is full and then submits at
+    // the device's pace, so launches go in the order they are wanted: the
+    // fine deposit, the whole side chain, then the fine level, which waits
+    // for the boundary anyway (submitted after the fine level, the coarse
+    // mean field started 90 us late).  A stream wait binds to the event's
+    // last record at the time of the call, so the fork event serves again
+    // for the second join.
     VH_CHECK_HIP(hipEventRecord(c->fork_ev, c->stream));
     VH_TRY(vh_deposit_fine_impl(a, fine, weight));
     VH_CHECK_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
@@ -1044,20 +1049,15 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
       vh_on_side on(c);
       VH_TRY(vh_deposit_cic_impl(a, coarse, weight));
       VH_TRY(vh_poisson_coarse_level_impl(coarse));
+      VH_TRY(vh_poisson_fine_boundary_impl(fine, coarse));
+      VH_CHECK_HIP(hipEventRecord(c->join_ev, c->side));
+      VH_TRY(vh_effective_potential_impl(coarse, jel_nions, jel_radius));
+      VH_CHECK_HIP(hipEventRecord(c->fork_ev, c->side));
     }
-    VH_CHECK_HIP(hipEventRecord(c->join_ev, c->side));
     VH_CHECK_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
-    VH_TRY(vh_poisson_fine_boundary_impl(fine, coarse));
-    VH_CHECK_HIP(hipEventRecord(c->fork_ev, c->stream));
     VH_TRY(vh_poisson_fine_part_impl(fine));
     VH_TRY(vh_potential_coefficients_impl(fine, jel_nions, jel_radius));
-    VH_CHECK_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
-    {
-      vh_on_side on(c);
-      VH_TRY(vh_effective_potential_impl(coarse, jel_nions, jel_radius));
-    }
-    VH_CHECK_HIP(hipEventRecord(c->join_ev, c->side));
-    VH_CHECK_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
+    VH_CHECK_HIP(hipStreamWaitEvent(c->stream, c->fork_ev, 0));
   }
   if (step_fuses_push(a)) {
     // the field and fallback kernels push the particles they computed the
