@@ -106,9 +106,10 @@ int vh_spline_coefficients(vh_mesh *m);
  * V_jel(r)) (src/devicemesh.jl:171-183; src/meanfield.jl:116-147). */
 int vh_effective_potential(vh_mesh *m, double nions, double radius);
 /* spline_coefficients!(c, phi, dm) followed by effective_potential!(c, rho,
- * dm, jel) on the same mesh (src/devicemesh.jl:126-127, 171-183) as one
- * transform, csol = S^-1 (phi + V_xc(rho) + V_jel): three rotating products
- * instead of six (the sums are rounded in another order: FP32 agreement). */
+ * dm, jel) on the same mesh (src/devicemesh.jl:126-127, 171-183) in one call:
+ * csol = S^-1 (V_xc(rho) + V_jel), then csol += S^-1 phi (the terms added
+ * after their transforms, as the two calls add them; vh_step runs the first
+ * half before the fine potential is solved). */
 int vh_potential_coefficients(vh_mesh *m, double nions, double radius);
 /* total_charge(rho, dm) in FP64 (src/devicemesh.jl:112-113). */
 int vh_total_charge(vh_mesh *m, double *q);
@@ -226,7 +227,7 @@ int vh_projectile_step(vh_accel *a, double dt);
  * of _update_forces_resident! and step! (src/simulation.jl:472-541; the
  * driver's nine calls): vh_deposit_smoothed, vh_deposit_cic,
  * vh_poisson_nested, vh_potential_coefficients (fine: the spline
- * coefficients and the mean field in one transform), vh_effective_potential
+ * coefficients and the mean field in one call), vh_effective_potential
  * (coarse; jellium of jel_nions ions and radius jel_radius), the forces
  * with the device projectile (vh_forces_resident + vh_projectile_step) or,
  * without vh_projectile_set, without projectile, and vh_verlet.  One call

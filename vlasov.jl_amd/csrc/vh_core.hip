@@ -1044,6 +1044,9 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
     // for the second join.
     VH_CHECK_HIP(hipEventRecord(c->fork_ev, c->stream));
     VH_TRY(vh_deposit_fine_impl(a, fine, weight));
+    // the fine mean field's coefficients need only the density: they fill
+    // the main stream's wait for the boundary
+    VH_TRY(vh_mean_field_coefficients_impl(fine, jel_nions, jel_radius));
     VH_CHECK_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
     {
       vh_on_side on(c);
@@ -1056,7 +1059,7 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
     }
     VH_CHECK_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
     VH_TRY(vh_poisson_fine_part_impl(fine));
-    VH_TRY(vh_potential_coefficients_impl(fine, jel_nions, jel_radius));
+    VH_TRY(vh_hartree_coefficients_add_impl(fine));
     VH_CHECK_HIP(hipStreamWaitEvent(c->stream, c->fork_ev, 0));
   }
   if (step_fuses_push(a)) {

@@ -381,6 +381,9 @@ struct vh_on_side {
 int vh_spline_coefficients_impl(vh_mesh *m, const float *src, float *dst);
 int vh_effective_potential_impl(vh_mesh *m, double nions, double radius);
 int vh_potential_coefficients_impl(vh_mesh *m, double nions, double radius);
+// its two halves: csol = S^-1 V(rho) (needs the density only), csol += S^-1 phi
+int vh_mean_field_coefficients_impl(vh_mesh *m, double nions, double radius);
+int vh_hartree_coefficients_add_impl(vh_mesh *m);
 int vh_total_charge_impl(vh_mesh *m, const float *rho, double *dq);
 // push_a > 0: the field and fallback kernels also push their particles
 // (Verlet factor dt^2 / mass) and leave the next sort's keys; requires the

@@ -745,13 +745,29 @@ int vh_effective_potential_impl(vh_mesh *m, double nions, double radius) {
   return VH_OK;
 }
 
-// spline_coefficients! then effective_potential! as one transform:
-// csol = S^-1 (phi + V_xc(rho) + V_jel), three products instead of six
-int vh_potential_coefficients_impl(vh_mesh *m, double nions, double radius) {
+// spline_coefficients! then effective_potential!, in the other order:
+// csol = S^-1 (V_xc(rho) + V_jel), which needs only the density, then
+// csol += S^-1 phi once the potential is solved (the same sum as the two
+// calls, whose terms are added after their transforms too)
+int vh_mean_field_coefficients_impl(vh_mesh *m, double nions, double radius) {
   vh_context *c = m->ctx;
   hipLaunchKernelGGL(k_effective, dim3(2048), dim3(256), 0, c->stream, m->extra, m->rho, m->colloc,
-                     m->n, (float)nions, (float)radius, (const float *)m->phi);
+                     m->n, (float)nions, (float)radius, (const float *)nullptr);
   VH_LAUNCH_CHECK();
+  // extra -> work -> extra -> csol
   VH_TRY(apply_all(c, m->Sinv_op, m->n, m->extra, m->work, m->extra, m->csol, EPI_STORE, nullptr, 0));
+  return VH_OK;
+}
+
+int vh_hartree_coefficients_add_impl(vh_mesh *m) {
+  // phi -> extra -> work -> csol (+=)
+  VH_TRY(apply_all(m->ctx, m->Sinv_op, m->n, m->phi, m->extra, m->work, m->csol, EPI_ACCUMULATE,
+                   nullptr, 0));
+  return VH_OK;
+}
+
+int vh_potential_coefficients_impl(vh_mesh *m, double nions, double radius) {
+  VH_TRY(vh_mean_field_coefficients_impl(m, nions, radius));
+  VH_TRY(vh_hartree_coefficients_add_impl(m));
   return VH_OK;
 }
