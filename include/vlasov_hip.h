@@ -91,6 +91,10 @@ int vh_mesh_destroy(vh_mesh *m);
 /* `Array(dev.rho[l])` etc. (src/simulation.jl:544-552): n^3 floats. */
 int vh_mesh_grid_download(vh_mesh *m, int which, float *host);
 int vh_mesh_grid_upload(vh_mesh *m, int which, const float *host);
+/* The frame of a film step, Float32.((rho[:, :, iz1] .+ rho[:, :, iz2]) ./ 2)
+ * of the mesh's density (scripts/xenon.jl:396-410, after sync_host!),
+ * reduced on the device: n^2 floats, i fastest; iz1, iz2 1-based planes. */
+int vh_density_slice(vh_mesh *m, int iz1, int iz2, float *host);
 
 /* poisson!(phi, rho, dm) single level with multipole faces
  * (src/devicemesh.jl:148-151). */

@@ -181,8 +181,10 @@ def test_poisson_chain(setup, orc):
 
 
 def test_potential_coefficients_fused(setup, orc):
-    """csol = S^-1 (phi + V) in one transform against spline_coefficients
-    followed by effective_potential, and against the FP64 oracle."""
+    """vh_potential_coefficients (csol = S^-1 V, then S^-1 phi added) against
+    spline_coefficients followed by effective_potential (S^-1 phi, then
+    S^-1 V added: the same two FP32 terms, added once), and against the FP64
+    oracle."""
     s = setup
     acc, dmf, fine = s["acc"], s["dmf"], s["fine"]
     acc.upload(s["rec"])
@@ -200,7 +202,7 @@ def test_potential_coefficients_fused(setup, orc):
     want = orc.spline_coefficients(dmf.phi.astype(np.float64), fine) + extra
     scale = max(np.max(np.abs(hartree)), np.max(np.abs(extra)))
     assert np.max(np.abs(fused - want)) / scale < REL
-    assert np.max(np.abs(fused - separate)) / scale < REL
+    assert fused.tobytes() == separate.tobytes()
 
 
 def test_spline_coefficients_mfma_layout(setup, orc):
@@ -398,6 +400,45 @@ def test_enclosed_count(setup, orc):
     acc.upload(s["rec"])
     c = np.array([3.0, -2.0, 1.0])
     assert acc.enclosed_count(c, 8.0) == orc.enclosed(s["rec"], s["x0"], s["h"], c, 8.0)
+
+
+def test_density_slice_frame(setup):
+    # the film frame of scripts/xenon.jl:396-410: Float32 of the FP64 mean of
+    # two z planes of the fine density, to the bit
+    s = setup
+    acc, dmf = s["acc"], s["dmf"]
+    acc.upload(s["rec"])
+    acc.deposit_smoothed(dmf, s["w"])
+    n = dmf.n
+    rho = dmf.rho.reshape(n, n, n).astype(np.float64)   # [k, j, i]
+    for iz1, iz2 in ((n // 2, n // 2 + 1), (1, n), (7, 7)):
+        want = ((rho[iz1 - 1] + rho[iz2 - 1]) / 2).astype(np.float32)
+        got = dmf.density_slice(iz1, iz2)
+        assert got.tobytes() == want.tobytes(), (iz1, iz2)
+    for bad in ((0, 1), (1, n + 1)):
+        with pytest.raises(va.VlasovHIPError, match="outside"):
+            dmf.density_slice(*bad)
+
+
+def test_film_observables(config1):
+    # frame and q_net of a film step read from the device against the same
+    # quantities computed from the downloaded density and cloud
+    p, fine, coarse, sm = config1
+    n = p.nparticles
+    w = p.nelectrons / n
+    pos, mom = va.sample_thomas_fermi(va.read_radial_profile(), n, w, seed=-3)
+    sim = va.Simulation(p, None, tables=(fine, coarse, sm), positions=pos, momenta=mom)
+    try:
+        sim.step()
+        nf, ng = p.nfine, sim.dmf.n
+        rho = sim.dmf.rho.reshape(ng, ng, ng).astype(np.float64)
+        want = ((rho[nf - 1] + rho[nf]) / 2).astype(np.float32)
+        assert sim.frame().tobytes() == want.tobytes()
+        r = sim.positions()
+        inside = int(np.count_nonzero(np.sum(r.astype(np.float64) ** 2, axis=1) < 35.0 ** 2))
+        assert sim.net_charge(35.0) == p.nelectrons - sim.weight * inside
+    finally:
+        sim.close()
 
 
 def test_bad_arguments_refused_before_launch(setup):

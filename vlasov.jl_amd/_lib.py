@@ -41,6 +41,7 @@ _SIGNATURES = {
     "vh_mesh_destroy": [_vp],
     "vh_mesh_grid_download": [_vp, C.c_int, _f32p],
     "vh_mesh_grid_upload": [_vp, C.c_int, _f32p],
+    "vh_density_slice": [_vp, C.c_int, C.c_int, _f32p],
     "vh_poisson": [_vp],
     "vh_poisson_nested": [_vp, _vp],
     "vh_spline_coefficients": [_vp],

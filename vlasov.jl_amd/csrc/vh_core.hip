@@ -286,6 +286,31 @@ extern "C" int vh_mesh_grid_download(vh_mesh *m, int which, float *host) {
   return VH_OK;
 }
 
+// (a + b) / 2 in FP32 is the FP32 rounding of the exact mean (the halving is
+// exact), the value the script's Float32 of its FP64 mean gives
+__global__ void k_density_slice(const float *__restrict__ rho, int n, int z1, int z2,
+                                float *__restrict__ out) {
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nn;
+       t += (int64_t)gridDim.x * blockDim.x)
+    out[t] = (rho[t + nn * z1] + rho[t + nn * z2]) * 0.5f;
+}
+
+extern "C" int vh_density_slice(vh_mesh *m, int iz1, int iz2, float *host) {
+  VH_REQUIRE(m && host, "vh_density_slice: null argument");
+  VH_REQUIRE(iz1 >= 1 && iz1 <= m->n && iz2 >= 1 && iz2 <= m->n,
+             "vh_density_slice: planes %d, %d outside 1..%d", iz1, iz2, m->n);
+  VH_CHECK_HIP(hipSetDevice(m->ctx->device));
+  // staged in the mesh's transform scratch (n^3 floats), free between calls
+  hipLaunchKernelGGL(k_density_slice, dim3(64), dim3(256), 0, m->ctx->stream, m->rho, m->n,
+                     iz1 - 1, iz2 - 1, m->work);
+  VH_LAUNCH_CHECK();
+  VH_CHECK_HIP(hipMemcpyAsync(host, m->work, (size_t)m->n * m->n * 4, hipMemcpyDeviceToHost,
+                              m->ctx->stream));
+  VH_CHECK_HIP(hipStreamSynchronize(m->ctx->stream));
+  return VH_OK;
+}
+
 extern "C" int vh_mesh_grid_upload(vh_mesh *m, int which, const float *host) {
   VH_REQUIRE(m && host, "vh_mesh_grid_upload: null argument");
   float *g = mesh_grid(m, which);

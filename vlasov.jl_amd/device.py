@@ -84,6 +84,14 @@ class DeviceMesh:
             raise ValueError(f"grid must hold {self.n}^3 values, got {a.size}")
         _lib.call("vh_mesh_grid_upload", self.handle, int(which), a)
 
+    def density_slice(self, iz1: int, iz2: int) -> np.ndarray:
+        """A film frame, Float32.((rho[:, :, iz1] .+ rho[:, :, iz2]) ./ 2) with
+        1-based planes (scripts/xenon.jl:396-410), reduced on the device:
+        shape (n, n), indexed [j, i] (i fastest in memory)."""
+        out = np.empty(self.n * self.n, dtype=np.float32)
+        _lib.call("vh_density_slice", self.handle, int(iz1), int(iz2), out)
+        return out.reshape(self.n, self.n)
+
     @property
     def rho(self):
         return self.download(VH_GRID_RHO)

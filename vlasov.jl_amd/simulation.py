@@ -233,6 +233,17 @@ class Simulation:
             raise ValueError("no projectile")
         return self.weight * self.acc.enclosed_count(self.projectile.position, radius)
 
+    def net_charge(self, radius: float = 35.0) -> float:
+        """The film's q_net: electrons minus those within `radius` of the
+        cluster's centre (scripts/xenon.jl:405-406), counted on the device."""
+        return self.params.nelectrons - self.weight * self.acc.enclosed_count(np.zeros(3), radius)
+
+    def frame(self) -> np.ndarray:
+        """The film frame: the fine density's mean over the planes nfine and
+        nfine + 1, the reaction plane z = 0 (scripts/xenon.jl:370, 403),
+        reduced on the device (n^2 floats read back instead of n^3)."""
+        return self.dmf.density_slice(self.params.nfine, self.params.nfine + 1)
+
     def positions(self) -> np.ndarray:
         return record_positions(self.acc.download(), self.x0, self.h)
 
