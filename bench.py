@@ -31,10 +31,10 @@ FP32_PEAK_TFLOPS = 157.3
 FIELD_FLOP_PER_PARTICLE = 4900.0      # 2300 FMA + 300 MUL of contract_spline_10
 # HBM bytes per k_field_mfma launch of the step (forces and push) at 8e6 particles
 # on 130^3, from two rocprofv3 counter passes over `bench.py --steps 20 --warmup 5`
-# (the average of the launches that push: FETCH_SIZE 321 636 KiB x 2.000 + WRITE_SIZE
-# 542 865 KiB x 1.000, the factors that make a 1 GiB device copy read and write 1 GiB);
+# (the average of the launches that push: FETCH_SIZE 325 821 KiB x 2.000 + WRITE_SIZE
+# 537 051 KiB x 1.000, the factors that make a 1 GiB device copy read and write 1 GiB);
 # recorded with the command in profiles/hbm_traffic_xenon_8M_130.md
-FIELD_TRAFFIC_BYTES_8M = (321636 * 2.0 + 542865 * 1.0) * 1024
+FIELD_TRAFFIC_BYTES_8M = (325821 * 2.0 + 537051 * 1.0) * 1024
 METRIC = "ms/step & particle-steps/s @8M,130^3; capture-q err vs 80M plain run"
 
 
