@@ -1,0 +1,88 @@
+"""Properties that do not depend on the size, checked at the benchmark's
+full size: 8e6 pseudo-particles of the Na196 cloud on the 130^3 grids of
+nfine = 64, with the Xe25+ projectile on the device.
+
+Bars: the sort bit-exact against the oracle's stable counting sort (records,
+occupied cells, bounds); the charge of the fine deposit equal to
+(N - nout) w to FP32 summation accuracy, and every deposit reproducible to
+the bit; one vh_step (forces and push fused) bit-identical to the separate
+entry points (forces, projectile step, vh_verlet), records, forces and
+projectile alike, over two steps."""
+import numpy as np
+import pytest
+
+from conftest import load_package
+
+pytestmark = pytest.mark.gpu
+va = load_package()
+
+NPART = 8_000_000
+NFINE = 64
+
+
+@pytest.fixture(scope="module")
+def full():
+    p = va.xenon_parameters(NPART, NFINE)
+    sim = va.Simulation(p, va.read_radial_profile(), projectile=va.xenon_projectile())
+    yield p, sim
+    sim.close()
+
+
+def test_full_size_sort_bit_exact(full, orc):
+    p, sim = full
+    acc = sim.acc
+    rec = acc.download()
+    # an unsorted cloud: the sorted one shuffled
+    shuffled = rec[np.random.default_rng(5).permutation(NPART)]
+    acc.upload(shuffled)
+    nocc, nout = acc.sort()
+    perm, occ, bnd = orc.cellsort(shuffled, sim.fine.axis.nknots)
+    assert acc.download().tobytes() == shuffled[perm].tobytes()
+    assert nocc == len(occ)
+    occ_d, bnd_d, _ = acc.sort_lists(nocc, nout)
+    assert np.array_equal(occ_d, occ + 1) and np.array_equal(bnd_d, bnd)
+    acc.upload(rec)
+
+
+def test_full_size_deposits_conserve_and_repeat(full):
+    p, sim = full
+    acc, dmf, dmc = sim.acc, sim.dmf, sim.dmc
+    nout = acc.deposit_smoothed(dmf, sim.weight)
+    rho = dmf.rho
+    q = dmf.total_charge()
+    want = (NPART - nout) * sim.weight
+    assert abs(q - want) < 1e-5 * want
+    acc.deposit_smoothed(dmf, sim.weight)
+    assert dmf.rho.tobytes() == rho.tobytes()
+    acc.deposit_cic(dmc, sim.weight)
+    rc = dmc.rho
+    acc.deposit_cic(dmc, sim.weight)
+    assert dmc.rho.tobytes() == rc.tobytes()
+
+
+def test_full_size_step_matches_separate_calls(full):
+    p, sim = full
+    pos = sim.positions()
+    # the momenta that give the device's previous positions back are not
+    # needed: both runs start from the same records
+    rec = sim.acc.download()
+    runs = []
+    try:
+        for _ in range(2):
+            s = va.Simulation(p, None, projectile=va.xenon_projectile(),
+                              tables=(sim.fine, sim.coarse, sim.smoothing),
+                              positions=pos, momenta=np.zeros_like(pos))
+            s.acc.upload(rec)
+            runs.append(s)
+        a, b = runs
+        for _ in range(2):
+            a.step()
+            b.update_forces(advance=True)
+            b.acc.verlet(p.dt, b.mass, p.rcmax, diagnostics=False)
+        assert a.acc.download().tobytes() == b.acc.download().tobytes()
+        assert a.acc.forces().tobytes() == b.acc.forces().tobytes()
+        assert np.array_equal(a.projectile.position, b.projectile.position)
+        assert np.array_equal(a.projectile.velocity, b.projectile.velocity)
+    finally:
+        for s in runs:
+            s.close()
