@@ -7,7 +7,8 @@ occupied cells, bounds); the charge of the fine deposit equal to
 (N - nout) w to FP32 summation accuracy, and every deposit reproducible to
 the bit; one vh_step (forces and push fused) bit-identical to the separate
 entry points (forces, projectile step, vh_verlet), records, forces and
-projectile alike, over two steps."""
+projectile alike, over two steps; the matrix-core field within 1e-6 of the
+exact-order kernel on every particle."""
 import numpy as np
 import pytest
 
@@ -86,3 +87,26 @@ def test_full_size_step_matches_separate_calls(full):
     finally:
         for s in runs:
             s.close()
+
+
+def test_full_size_field_matrix_cores_against_exact_order(full):
+    # the dominant kernel on the full cloud and grid, against the kernel
+    # that runs the reference's FMA order (pinned to it bit for bit at the
+    # parity size): a rough zero-mean fine potential, every particle whose
+    # stencil fits, 1e-6 relative (the two orders' FP32 rounding)
+    p, sim = full
+    acc, dmf, dmc = sim.acc, sim.dmf, sim.dmc
+    acc.sort()
+    rng = np.random.default_rng(19)
+    csol = (rng.standard_normal(sim.fine.n ** 3) * 0.01).astype(np.float32)
+    dmf.upload(va._lib.VH_GRID_CSOL, csol)
+    nout, _ = acc.compute_forces(dmf, dmc, sim.weight, None)
+    f = acc.forces().astype(np.float64)
+    acc.set_exact_field(True)
+    try:
+        acc.compute_forces(dmf, dmc, sim.weight, None)
+        fe = acc.forces().astype(np.float64)
+    finally:
+        acc.set_exact_field(False)
+    assert np.all(np.isfinite(f)) and nout < NPART // 10
+    assert np.max(np.abs(f - fe)) / np.max(np.abs(fe)) < 1e-6
