@@ -769,6 +769,20 @@ __global__ void k_fallback(const int32_t *__restrict__ P, const uint32_t *__rest
   if (vh_lane() == 0 && rejected) atomicAdd(pu.nout, rejected);
 }
 
+// First pass of a column sum over many blocks: block (s, v) sums slice s
+// (of gridDim.x equal slices) of column v into out[v gridDim.x + s]; a
+// k_reduce_cols over those partials ends it (a fixed order either way).
+__global__ void k_reduce_slices(const double *__restrict__ part, int nparts,
+                                double *__restrict__ out) {
+  __shared__ double lds[16];
+  const int s = blockIdx.x, v = blockIdx.y, ns = gridDim.x;
+  const int len = (nparts + ns - 1) / ns;
+  const int lo = s * len < nparts ? s * len : nparts;
+  const int hi = lo + len < nparts ? lo + len : nparts;
+  const double r = vh_block_sum_d(vh_thread_sum(part + (int64_t)v * nparts + lo, hi - lo), lds);
+  if (threadIdx.x == 0) out[(int64_t)v * ns + s] = r;
+}
+
 // Sums each of `nvals` columns of `nparts` partials in a fixed order; one
 // block per column (any grid works, columns are strided over the blocks).
 __global__ void k_reduce_cols(const double *__restrict__ part, int nparts, int nvals,
@@ -973,8 +987,17 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
                        (float)weight, a->ncol, pa, pu, a->force, a->dpart);
   vh_prof_end(c, tok);
   VH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_reduce_cols, dim3(4), dim3(1024), 0, st, a->dpart, grid, 4, a->dscal + 8);
-  VH_LAUNCH_CHECK();
+  {
+    // 65536 partials of 4 columns: slices over 32 blocks, then their sums
+    // (4 blocks alone took 13 us); the slices' sums go after the partials
+    constexpr int nsl = 8;
+    double *sl = a->dpart + 4 * (int64_t)grid;
+    VH_REQUIRE(4 * (int64_t)grid + 4 * nsl <= 8 * a->dpart_len, "forces: partials beyond the buffer");
+    hipLaunchKernelGGL(k_reduce_slices, dim3(nsl, 4), dim3(1024), 0, st, a->dpart, grid, sl);
+    VH_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_reduce_cols, dim3(4), dim3(64), 0, st, sl, nsl, 4, a->dscal + 8);
+    VH_LAUNCH_CHECK();
+  }
   vh_fallback_args fa;
   fa.px = proj ? proj[0] : 0.0;
   fa.py = proj ? proj[1] : 0.0;
