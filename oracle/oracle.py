@@ -75,6 +75,10 @@ def lib():
 
 
 def _p(a: np.ndarray):
+    # the C loops read the buffer densely: a strided view (rec[::k]) would
+    # hand them its base pointer and they would read the wrong records
+    if not a.flags.c_contiguous:
+        raise ValueError("oracle arrays must be C-contiguous (use np.ascontiguousarray)")
     return a.ctypes.data
 
 

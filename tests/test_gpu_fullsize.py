@@ -8,7 +8,9 @@ occupied cells, bounds); the charge of the fine deposit equal to
 the bit; one vh_step (forces and push fused) bit-identical to the separate
 entry points (forces, projectile step, vh_verlet), records, forces and
 projectile alike, over two steps; the matrix-core field within 1e-6 of the
-exact-order kernel on every particle."""
+exact-order kernel on every particle, and the exact-order kernel equal to the
+oracle's FP32 emulation of the reference's FMA chain, bit for bit, and within
+1e-5 of its FP64 contraction, on every 97th particle."""
 import numpy as np
 import pytest
 
@@ -110,3 +112,40 @@ def test_full_size_field_matrix_cores_against_exact_order(full):
         acc.set_exact_field(False)
     assert np.all(np.isfinite(f)) and nout < NPART // 10
     assert np.max(np.abs(f - fe)) / np.max(np.abs(fe)) < 1e-6
+
+
+def test_full_size_field_against_oracle(full, orc):
+    # the field on the full cloud and grid against the oracle on every 97th
+    # particle (the oracle's field of a particle depends only on its own
+    # record): a rough zero-mean fine potential, the exact-order kernel bit
+    # for bit against the FP32 emulation, the matrix-core kernel at 1e-5
+    # relative against the FP64 contraction
+    p, sim = full
+    acc, dmf, dmc = sim.acc, sim.dmf, sim.dmc
+    fine, sm = sim.fine, sim.smoothing
+    acc.sort()
+    srec = acc.download()
+    rng = np.random.default_rng(19)
+    csol = (rng.standard_normal(fine.n ** 3) * 0.01).astype(np.float32)
+    dmf.upload(va._lib.VH_GRID_CSOL, csol)
+    acc.compute_forces(dmf, dmc, sim.weight, None)
+    f = acc.forces()[::97]
+    acc.set_exact_field(True)
+    try:
+        acc.compute_forces(dmf, dmc, sim.weight, None)
+        fe = acc.forces()[::97]
+    finally:
+        acc.set_exact_field(False)
+    sub = np.ascontiguousarray(srec[::97])
+    _, occ, bnd = orc.cellsort(sub, fine.axis.nknots)
+    ost = orc.OracleStep(fine, sim.coarse, sm, va.Jellium.of(p.nions), sim.weight, sim.weight,
+                         p.dt, p.rcmax)
+    args = (sub, occ, bnd, ost.nk, fine.n, csol.astype(np.float64), sm.overlap, sm.gradient,
+            ost.x0, ost.h, ost.spacing, sm.nbdt, sm.nbdt + 1, sim.weight, None, 0, 0.0)
+    w32, _ = orc.field(*args, f32=True)
+    w64, _ = orc.field(*args, f32=False)
+    inside = ~np.isnan(w64[:, 0])
+    assert inside.sum() > 0.9 * len(sub)
+    assert fe[inside].tobytes() == w32[inside].astype(np.float32).tobytes()
+    ref = w64[inside]
+    assert np.max(np.abs(f[inside] - ref)) / np.max(np.abs(ref)) < 1e-5
