@@ -10,7 +10,8 @@ entry points (forces, projectile step, vh_verlet), records, forces and
 projectile alike, over two steps; the matrix-core field within 1e-6 of the
 exact-order kernel on every particle, and the exact-order kernel equal to the
 oracle's FP32 emulation of the reference's FMA chain, bit for bit, and within
-1e-5 of its FP64 contraction, on every 97th particle."""
+1e-5 of its FP64 contraction, on every 97th particle; both densities within
+1e-5 of the oracle's FP64 deposits."""
 import numpy as np
 import pytest
 
@@ -149,3 +150,23 @@ def test_full_size_field_against_oracle(full, orc):
     assert fe[inside].tobytes() == w32[inside].astype(np.float32).tobytes()
     ref = w64[inside]
     assert np.max(np.abs(f[inside] - ref)) / np.max(np.abs(ref)) < 1e-5
+
+
+def test_full_size_deposits_against_oracle(full, orc):
+    # both densities of the full cloud against the oracle's FP64 deposits,
+    # 1e-5 relative (max |diff| / max |ref|)
+    p, sim = full
+    acc, dmf, dmc = sim.acc, sim.dmf, sim.dmc
+    nout = acc.deposit_smoothed(dmf, sim.weight)
+    acc.deposit_cic(dmc, sim.weight)
+    srec = acc.download()
+    _, occ, bnd = orc.cellsort(srec, sim.fine.axis.nknots)
+    ost = orc.OracleStep(sim.fine, sim.coarse, sim.smoothing, va.Jellium.of(p.nions), sim.weight,
+                         sim.weight, p.dt, p.rcmax)
+    want_f, wn = ost.deposit_fine(srec, occ, bnd)
+    assert nout == wn
+    got_f = dmf.rho.astype(np.float64)
+    assert np.max(np.abs(got_f - want_f)) / np.max(np.abs(want_f)) < 1e-5
+    want_c = ost.deposit_coarse(srec)
+    got_c = dmc.rho.astype(np.float64)
+    assert np.max(np.abs(got_c - want_c)) / np.max(np.abs(want_c)) < 1e-5
