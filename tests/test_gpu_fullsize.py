@@ -11,7 +11,8 @@ projectile alike, over two steps; the matrix-core field within 1e-6 of the
 exact-order kernel on every particle, and the exact-order kernel equal to the
 oracle's FP32 emulation of the reference's FMA chain, bit for bit, and within
 1e-5 of its FP64 contraction, on every 97th particle; both densities within
-1e-5 of the oracle's FP64 deposits."""
+1e-5 of the oracle's FP64 deposits; the nested Poisson solve and the coarse
+coefficients within 1e-5 of the oracle's FP64 chain."""
 import numpy as np
 import pytest
 
@@ -170,3 +171,20 @@ def test_full_size_deposits_against_oracle(full, orc):
     want_c = ost.deposit_coarse(srec)
     got_c = dmc.rho.astype(np.float64)
     assert np.max(np.abs(got_c - want_c)) / np.max(np.abs(want_c)) < 1e-5
+
+
+def test_full_size_nested_poisson_against_oracle(full, orc):
+    # the two-level solve on the full grids from the device's own densities
+    # (narrowed to FP32 on both sides), against the oracle's FP64 chain at
+    # 1e-5 relative; the coarse coefficients likewise
+    p, sim = full
+    acc, dmf, dmc = sim.acc, sim.dmf, sim.dmc
+    acc.deposit_smoothed(dmf, sim.weight)
+    acc.deposit_cic(dmc, sim.weight)
+    rho_f, rho_c = dmf.rho.astype(np.float64), dmc.rho.astype(np.float64)
+    va.poisson_nested(dmf, dmc)
+    wf, wc, _ = orc.poisson_nested(rho_f, rho_c, sim.fine, sim.coarse)
+    for got, want in ((dmc.phi, wc), (dmf.phi, wf),
+                      (dmc.csol, orc.spline_coefficients(wc, sim.coarse))):
+        got = got.astype(np.float64)
+        assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < 1e-5
