@@ -12,7 +12,8 @@ exact-order kernel on every particle, and the exact-order kernel equal to the
 oracle's FP32 emulation of the reference's FMA chain, bit for bit, and within
 1e-5 of its FP64 contraction, on every 97th particle; both densities within
 1e-5 of the oracle's FP64 deposits; the nested Poisson solve and the coarse
-coefficients within 1e-5 of the oracle's FP64 chain."""
+coefficients within 1e-5 of the oracle's FP64 chain; vh_verlet bit-exact
+against the oracle's FP32 restatement."""
 import numpy as np
 import pytest
 
@@ -188,3 +189,23 @@ def test_full_size_nested_poisson_against_oracle(full, orc):
                       (dmc.csol, orc.spline_coefficients(wc, sim.coarse))):
         got = got.astype(np.float64)
         assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < 1e-5
+
+
+def test_full_size_verlet_bit_exact(full, orc):
+    # vh_verlet on the full cloud with random forces against the oracle's
+    # FP32 restatement of _verlet_packed, to the bit; its FP64 diagnostics
+    p, sim = full
+    acc = sim.acc
+    acc.sort()
+    rec = acc.download()
+    knots = sim.fine.axis.knots
+    x0 = float(knots[0])
+    h = float((knots[-1] - knots[0]) / (sim.fine.axis.nknots - 1))
+    rng = np.random.default_rng(23)
+    f = (rng.standard_normal((NPART, 3)) * 1e-3).astype(np.float32)
+    acc.upload_forces(f)
+    d = acc.verlet(p.dt, sim.mass, p.rcmax)
+    want = rec.copy()
+    wd = orc.verlet_f32(want, f, p.dt, sim.mass, h, x0, p.rcmax)
+    assert acc.download().tobytes() == want.tobytes()
+    assert np.allclose([d["kinetic"], d["escaped"], *d["angular"]], wd, rtol=1e-9, atol=1e-12)
