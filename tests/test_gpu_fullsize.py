@@ -13,7 +13,8 @@ oracle's FP32 emulation of the reference's FMA chain, bit for bit, and within
 1e-5 of its FP64 contraction, on every 97th particle; both densities within
 1e-5 of the oracle's FP64 deposits; the nested Poisson solve and the coarse
 coefficients within 1e-5 of the oracle's FP64 chain; vh_verlet bit-exact
-against the oracle's FP32 restatement."""
+against the oracle's FP32 restatement; the interaction energy within 1e-11
+and the enclosed counts exact against the oracle."""
 import numpy as np
 import pytest
 
@@ -209,3 +210,25 @@ def test_full_size_verlet_bit_exact(full, orc):
     wd = orc.verlet_f32(want, f, p.dt, sim.mass, h, x0, p.rcmax)
     assert acc.download().tobytes() == want.tobytes()
     assert np.allclose([d["kinetic"], d["escaped"], *d["angular"]], wd, rtol=1e-9, atol=1e-12)
+
+
+def test_full_size_energy_and_enclosed(full, orc):
+    # the interaction energy of the full cloud on the step's own coefficients
+    # against the oracle's FP64 sum (same per-particle arithmetic, another
+    # summation order: 1e-11 relative), and the enclosed count, exact
+    p, sim = full
+    acc, dmf, dmc = sim.acc, sim.dmf, sim.dmc
+    sim.step()
+    rec = acc.download()
+    fine, coarse, sm = sim.fine, sim.coarse, sim.smoothing
+    knots = fine.axis.knots
+    x0 = float(knots[0])
+    h = float((knots[-1] - knots[0]) / (fine.axis.nknots - 1))
+    csol_f, csol_c = dmf.csol.astype(np.float64), dmc.csol.astype(np.float64)
+    got = acc.interaction_energy(dmf, dmc, sim.weight, 3.0)
+    want = orc.interaction_energy(rec, x0, h, knots, csol_f, sm.overlap, float(sm.spacing),
+                                  sm.nbdt, coarse.axis.knots, csol_c, sim.weight, 3.0)
+    assert abs(got - want) <= 1e-11 * abs(want)
+    c = np.array(sim.projectile.position, dtype=np.float64)
+    for r in (8.0, 35.0):
+        assert acc.enclosed_count(c, r) == orc.enclosed(rec, x0, h, c, r)
