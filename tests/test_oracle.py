@@ -37,6 +37,20 @@ def test_sort_invariants_and_stability(orc):
     assert np.array_equal(p2, np.arange(500)) and np.array_equal(o2, occ)
 
 
+def test_oracle_refuses_strided_views(orc):
+    # the C loops read records densely: a strided view must be refused, not
+    # read from its base pointer (which would give the wrong records); its
+    # contiguous copy sorts like the records it holds
+    ax = G.uniform_axis(-10.0, 10.0, 8)
+    rec, _ = cloud(400, ax.knots[0], ax.knots[1] - ax.knots[0])
+    with pytest.raises(ValueError, match="contiguous"):
+        orc.cellsort(rec[::3], ax.nknots)
+    sub = np.ascontiguousarray(rec[::3])
+    perm, _, _ = orc.cellsort(sub, ax.nknots)
+    keys = np.array([orc_key(r, ax.nknots) for r in sub])
+    assert np.array_equal(perm, np.argsort(keys, kind="stable"))
+
+
 def orc_key(r, nk):
     k = np.clip(r["knode"], 1, nk) - 1
     return k[0] + nk * (k[1] + nk * k[2])
