@@ -14,6 +14,7 @@ Grids are flat float64 arrays in Julia column-major order ([i + n j + n^2 k]).
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import math
 import os
@@ -32,46 +33,75 @@ REC = np.dtype([("knode", "<i4", (3,)), ("delta", "<f4", (3,)),
 _lib = None
 
 
-def build(force: bool = False) -> Path:
-    """Compiles the C restatement with gcc (OpenMP, no FP contraction)."""
+PLANTS = ("COLUMN_SLIP", "STENCIL_SHIFT", "CIC_SWAP")
+
+
+def build(force: bool = False, plant: str | None = None) -> Path:
+    """Compiles the C restatement with gcc (OpenMP, no FP contraction).
+    `plant` names one ORACLE_PLANT_* mistake: that copy goes to its own file
+    and is only ever loaded by `planted()`."""
     LIB.parent.mkdir(exist_ok=True)
-    if force or not LIB.exists() or LIB.stat().st_mtime < SRC.stat().st_mtime:
-        tmp = LIB.with_suffix(".so.tmp")
+    if plant is not None and plant not in PLANTS:
+        raise ValueError(f"unknown planted mistake {plant!r}")
+    out = LIB if plant is None else LIB.with_name(f"liboracle_plant_{plant.lower()}.so")
+    if force or not out.exists() or out.stat().st_mtime < SRC.stat().st_mtime:
+        tmp = out.with_suffix(".so.tmp")
         cmd = ["gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",
                "-std=c11", str(SRC), "-o", str(tmp), "-lm"]
+        if plant is not None:
+            cmd.insert(1, f"-DORACLE_PLANT_{plant}")
         subprocess.run(cmd, check=True)
-        os.replace(tmp, LIB)
-    return LIB
+        os.replace(tmp, out)
+    return out
+
+
+@contextlib.contextmanager
+def planted(mistake: str):
+    """Within the block, every oracle function runs a copy of the C
+    restatement built with the planted mistake `mistake` (one of PLANTS)."""
+    global _lib
+    saved = lib()
+    _lib = _bind(C.CDLL(str(build(plant=mistake))))
+    try:
+        yield
+    finally:
+        _lib = saved
 
 
 def lib():
     global _lib
     if _lib is None:
         build()
-        L = C.CDLL(str(LIB))
-        i64, i32, f32, f64 = C.c_int64, C.c_int, C.c_float, C.c_double
-        vp = C.c_void_p
-        L.or_cellsort.argtypes = [vp, i64, i32, vp, vp, vp]
-        L.or_cellsort.restype = i64
-        L.or_outside.argtypes = [vp, i64, i32, vp]
-        L.or_outside.restype = i64
-        L.or_columns.argtypes = [vp, i64, f32, f32, f32, f32, f32, f32, i32, vp]
-        L.or_columns.restype = i64
-        L.or_deposit_smoothed.argtypes = [vp, vp, vp, i64, i32, i32, vp, f32, f32, i32, vp]
-        L.or_deposit_cic.argtypes = [vp, i64, f32, f32, vp, i32, f32, f32, vp, i32, vp]
-        L.or_field.argtypes = [vp, vp, vp, i64, i32, i32, vp, vp, vp, f64, f64, f32, f32, i32, f64,
-                               vp, i32, f64, vp, vp, i32]
-        L.or_spline_potential.argtypes = [vp, i32, vp, i32, vp, i64, vp]
-        L.or_fallback.argtypes = [vp, vp, i64, f64, f64, vp, i32, vp, i32, f64, vp, i32, f64, f64, vp]
-        L.or_verlet_f32.argtypes = [vp, vp, i64, f32, f32, f32, f32, f32, f32, vp]
-        L.or_interaction_energy.argtypes = [vp, i64, f64, f64, vp, i32, vp, i32, vp, f64, i32, i32,
-                                            vp, i32, vp, i32, f64, f64, vp]
-        L.or_interaction_energy.restype = f64
-        L.or_enclosed.argtypes = [vp, i64, f64, f64, vp, f64]
-        L.or_enclosed.restype = i64
-        L.or_num_threads.restype = i32
-        _lib = L
+        _lib = _bind(C.CDLL(str(LIB)))
     return _lib
+
+
+def _bind(L):
+    """Declares the argument and result types of the C restatement's functions."""
+    i64, i32, f32, f64 = C.c_int64, C.c_int, C.c_float, C.c_double
+    vp = C.c_void_p
+    L.or_cellsort.argtypes = [vp, i64, i32, vp, vp, vp]
+    L.or_cellsort.restype = i64
+    L.or_outside.argtypes = [vp, i64, i32, vp]
+    L.or_outside.restype = i64
+    L.or_columns.argtypes = [vp, i64, f32, f32, f32, f32, f32, f32, i32, vp]
+    L.or_columns.restype = i64
+    L.or_deposit_smoothed.argtypes = [vp, vp, vp, i64, i32, i32, vp, f32, f32, i32, vp]
+    L.or_deposit_cic.argtypes = [vp, i64, f32, f32, vp, i32, f32, f32, vp, i32, vp]
+    L.or_field.argtypes = [vp, vp, vp, i64, i32, i32, vp, vp, vp, f64, f64, f32, f32, i32, f64,
+                           vp, i32, f64, vp, vp, i32]
+    L.or_spline_potential.argtypes = [vp, i32, vp, i32, vp, i64, vp]
+    L.or_fallback.argtypes = [vp, vp, i64, f64, f64, vp, i32, vp, i32, f64, vp, i32, f64, f64, vp]
+    L.or_verlet_f32.argtypes = [vp, vp, i64, f32, f32, f32, f32, f32, f32, vp]
+    L.or_interaction_energy.argtypes = [vp, i64, f64, f64, vp, i32, vp, i32, vp, f64, i32, i32,
+                                        vp, i32, vp, i32, f64, f64, vp]
+    L.or_interaction_energy.restype = f64
+    L.or_enclosed.argtypes = [vp, i64, f64, f64, vp, f64]
+    L.or_enclosed.restype = i64
+    L.or_num_threads.restype = i32
+    L.or_locate.argtypes = [vp, i32, f32, f32, vp, i32, f32, C.POINTER(f32)]
+    L.or_locate.restype = i32
+    return L
 
 
 def _p(a: np.ndarray):
@@ -114,6 +144,17 @@ def deposit_smoothed_raw(srec, occ, bnd, nk, ngrid, nodes, spacing, nbdt, ncol):
     lib().or_deposit_smoothed(_p(srec), _p(occ), _p(bnd), len(occ), nk, ngrid, _p(nodes),
                               spacing, float(nbdt), ncol, _p(rho))
     return rho
+
+
+def locate(x, colloc32, x0t, iwt, cells):
+    """_locate_cell in Float32 (src/kernels.jl:1069-1081): (1-based left
+    collocation point, weight of that point), (0, 0) outside."""
+    g = np.ascontiguousarray(colloc32, dtype=np.float32)
+    c = np.ascontiguousarray(cells, dtype=np.int32)
+    w = C.c_float()
+    cell = lib().or_locate(_p(g), len(g), np.float32(x0t), np.float32(iwt), _p(c), len(c),
+                           np.float32(x), C.byref(w))
+    return int(cell), np.float32(w.value)
 
 
 def deposit_cic_raw(rec, x0f, hf, colloc32, x0t, iwt, cells):

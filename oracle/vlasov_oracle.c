@@ -13,6 +13,11 @@
  *     the reference's CPU precision, and are compared within tolerance.
  * `or_field(..., f32=1)` and `or_verlet_f32` are additionally exact Float32
  * emulations (explicit fmaf, no contraction: build with -ffp-contract=off).
+ *
+ * The ORACLE_PLANT_* macros are never defined in the oracle the tests check
+ * against.  A test builds a separate copy with one of them, a plausible
+ * mistake in a placement convention, to show that its known-answer check
+ * fails on that mistake.
  */
 #include <math.h>
 #include <stdint.h>
@@ -83,6 +88,9 @@ static inline int col_f32(float d, float spacing, float nbdt, int ncol) {
   float half = spacing * 0.5f;
   float t = (d + half) / spacing * nbdt + 0.5f;
   int c = (int)floorf(t) + 1;
+#ifdef ORACLE_PLANT_COLUMN_SLIP
+  c += 1; /* planted mistake: one table column off */
+#endif
   return clampi(c, 1, ncol);
 }
 
@@ -113,6 +121,9 @@ static inline int cell_fits8(int64_t key, int nk, int ngrid, int *b) {
   b[0] = 2 * kx - 5;
   b[1] = 2 * ky - 5;
   b[2] = 2 * kz - 5;
+#ifdef ORACLE_PLANT_STENCIL_SHIFT
+  b[0] += 1; /* planted mistake: the x stencil one collocation point right */
+#endif
   return b[0] >= 0 && b[1] >= 0 && b[2] >= 0 && b[0] + 8 <= ngrid && b[1] + 8 <= ngrid &&
          b[2] + 8 <= ngrid;
 }
@@ -156,6 +167,13 @@ static inline int locate_f32(const float *gt, int ng, float x0t, float iwt, cons
   return c;
 }
 
+/* locate through the table (src/kernels.jl:1069-1081) for one position:
+ * 1-based left collocation point (0 outside) and the weight of that point */
+int or_locate(const float *gt, int ng, float x0t, float iwt, const int32_t *cells, int nbin,
+              float x, float *w) {
+  return locate_f32(gt, ng, x0t, iwt, cells, nbin, x, w);
+}
+
 /* coarse cloud-in-cell of every particle (src/kernels.jl:1207-1273), cell and
  * weights in Float32 as the device computes them, products and sums in FP64.
  * rho raw (not divided by the dual volumes). */
@@ -175,6 +193,9 @@ void or_deposit_cic(const rec_t *parts, int64_t n, float x0f, float hf, const fl
     if (!in) continue;
     double ax = a[0], ay = a[1], az = a[2];
     double bx = 1.0 - ax, by = 1.0 - ay, bz = 1.0 - az;
+#ifdef ORACLE_PLANT_CIC_SWAP
+    { double t = ax; ax = bx; bx = t; } /* planted mistake: x weights swapped */
+#endif
     int64_t i = ix[0] - 1, j = ix[1] - 1, k = ix[2] - 1;
     rho[i + nn * (j + nn * k)] += ax * ay * az;
     rho[i + nn * (j + nn * (k + 1))] += ax * ay * bz;

@@ -216,12 +216,13 @@ def test_spline_coefficients_mfma_layout(setup, orc):
     assert relerr(s["dmf"].csol, want) < REL
 
 
-@pytest.mark.parametrize("nfine", [16, 17, 33, 64, 80, 99])
+@pytest.mark.parametrize("nfine", [16, 17, 33, 64, 80, 99, 128])
 def test_rotating_product_shapes(setup, orc, nfine):
     """Every epilogue of the rotating product at the launch shapes it takes:
     one panel per block (n = 34, 36), several (n = 68, 130), a last row tile
     with two rows (n = 34, 66, 130, 162), two block rows with four source
-    pairs per thread (n = 162, 200), against the FP64 oracle.  S^-1 on a random grid (plain stores), the single-level
+    pairs per thread (n = 162, 200, 256, 258; at 256 and 258 the 65-step K
+    slices and three panels in flight), against the FP64 oracle.  S^-1 on a random grid (plain stores), the single-level
     Poisson solve (Hadamard and interior-fill epilogues, at n - 2) and the
     mean field added to a random csol (accumulating epilogue).  The mean
     field is not added to S^-1 of the solved potential here: that potential
