@@ -234,7 +234,7 @@ def main():
                          "charges give the run-to-run spread (0: skip)")
     ap.add_argument("--cpu-particles", type=int, default=100_000)
     ap.add_argument("--cpu-nfine", type=int, default=28)
-    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB, five times "
