@@ -121,6 +121,20 @@ int vh_total_charge(vh_mesh *m, double *q);
  * (src/devicemesh.jl:115-117; src/poisson.jl:152-189). */
 int vh_multipole(vh_mesh *m, double out10[10]);
 
+/* multipole(rho, dm) as vh_multipole, without allocating: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  Inputs and output are the
+ * caller's device memory on `device`: rho n^3 floats (i fastest), dual the
+ * 3 x n FP64 dual moments S^-T m_k (order k at k*n), out10 the 10 doubles
+ * {q, center[3], quadrupole[6]}.  Launched on `stream` (a hipStream_t, NULL
+ * for the null stream) without synchronising.  n <= 288.  The FP64 partials
+ * go to a static scratch of 320 KiB per GPU, held until the process ends and
+ * shared by every call on that GPU: calls that overlap on one GPU race (on
+ * one stream they are ordered).  Calls no hipMalloc, hipFree or allocating
+ * routine. */
+int vh_multipole_noalloc(int device, void *stream, const float *rho, const double *dual, int n,
+                         double *out10);
+
 /* --- DeviceAccelerator (src/accelerator.jl:126-185, 281-331) -------------
  *   npart      particles;  nk knots of the uniform fine axis, n = 2 nk
  *   knots      the nk knots (x0 = knots[0], h = (knots[nk-1]-knots[0])/(nk-1),

@@ -48,6 +48,7 @@ _SIGNATURES = {
     "vh_effective_potential": [_vp, C.c_double, C.c_double],
     "vh_total_charge": [_vp, C.POINTER(C.c_double)],
     "vh_multipole": [_vp, _f64p],
+    "vh_multipole_noalloc": [C.c_int, _vp, _vp, _vp, C.c_int, _vp],
     "vh_accel_create": [_vp, C.c_int64, C.c_int, _f64p, C.c_double, C.c_int, _f64p, _f64p, _f64p,
                         C.POINTER(_vp)],
     "vh_accel_destroy": [_vp],

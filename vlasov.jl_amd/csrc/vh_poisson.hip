@@ -14,6 +14,7 @@
 #include "vh_internal.h"
 
 #include <cmath>
+#include <mutex>
 
 typedef float vh_f4 __attribute__((ext_vector_type(4)));
 
@@ -439,26 +440,77 @@ __global__ void __launch_bounds__(640)
 }
 
 #define MOM_BLOCKS 256
+// blocks of the moments kernel: one wave per four columns, about one pass per
+// wave, at most MOM_MAXBLOCKS (partials: 10 per block)
+#define MOM_MAXBLOCKS 4096
+static int moments_blocks(int n) {
+  return (n * n + 15) / 16 < MOM_MAXBLOCKS ? (n * n + 15) / 16 : MOM_MAXBLOCKS;
+}
+
+// the ten moments of rho into mom[0..9] and the multipole into mp[0..9],
+// through 10 x moments_blocks(n) FP64 partials
+static int multipole_launch(hipStream_t st, const float *rho, const double *dual, int n,
+                            double *partials, double *mom, double *mp) {
+  VH_REQUIRE(n >= 2 && n <= MOM_MAXN, "multipole: n = %d outside [2, %d]", n, MOM_MAXN);
+  const int nb = moments_blocks(n);
+  if (n <= 128)
+    hipLaunchKernelGGL(k_moments<2>, dim3(nb), dim3(256), 0, st, rho, dual, n, partials, nb);
+  else if (n <= 192)
+    hipLaunchKernelGGL(k_moments<3>, dim3(nb), dim3(256), 0, st, rho, dual, n, partials, nb);
+  else
+    hipLaunchKernelGGL(k_moments<5>, dim3(nb), dim3(256), 0, st, rho, dual, n, partials, nb);
+  VH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_moments_finish, dim3(1), dim3(640), 0, st, partials, nb, mom, mp);
+  VH_LAUNCH_CHECK();
+  return VH_OK;
+}
 
 int vh_multipole_impl(vh_mesh *m) {
-  vh_context *c = m->ctx;
-  const int n = m->n;
-  VH_REQUIRE(n <= MOM_MAXN, "multipole: n = %d beyond %d", n, MOM_MAXN);
-  // one wave per four columns: about one pass per wave (partials: 10 n^2)
-  const int nb = (n * n + 15) / 16 < 4096 ? (n * n + 15) / 16 : 4096;
-  if (n <= 128)
-    hipLaunchKernelGGL(k_moments<2>, dim3(nb), dim3(256), 0, c->stream, m->rho, m->dual, n,
-                       m->partials, nb);
-  else if (n <= 192)
-    hipLaunchKernelGGL(k_moments<3>, dim3(nb), dim3(256), 0, c->stream, m->rho, m->dual, n,
-                       m->partials, nb);
-  else
-    hipLaunchKernelGGL(k_moments<5>, dim3(nb), dim3(256), 0, c->stream, m->rho, m->dual, n,
-                       m->partials, nb);
-  VH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_moments_finish, dim3(1), dim3(640), 0, c->stream, m->partials, nb,
-                     m->scal, m->scal + 10);
-  VH_LAUNCH_CHECK();
+  return multipole_launch(m->ctx->stream, m->rho, m->dual, m->n, m->partials, m->scal,
+                          m->scal + 10);
+}
+
+// --- the multipole without allocation --------------------------------------
+// The partials, moments and multipole of vh_multipole_noalloc live in this
+// file-scope array, one copy per GPU, held until the process ends.  Its host
+// name is only a stand-in for the device symbol: host code uses the address
+// hipGetSymbolAddress returns for the current device, never the name.
+#define NOALLOC_DOUBLES (10 * MOM_MAXBLOCKS + 32)
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+#define NOALLOC_MAX_DEVICES 64
+static double *scratch_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex scratch_mu;
+
+// the device address of `scratch` on `device` (current), looked up once
+static int scratch_of(int device, double **out) {
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (!scratch_ptr[device]) {
+    void *p = nullptr;
+    VH_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    VH_REQUIRE(p != nullptr, "multipole without allocation: no scratch on device %d", device);
+    scratch_ptr[device] = (double *)p;
+  }
+  *out = scratch_ptr[device];
+  return VH_OK;
+}
+
+extern "C" int vh_multipole_noalloc(int device, void *stream, const float *rho, const double *dual,
+                                    int n, double *out10) {
+  VH_REQUIRE(rho && dual && out10, "vh_multipole_noalloc: null argument");
+  VH_REQUIRE(device >= 0 && device < NOALLOC_MAX_DEVICES,
+             "vh_multipole_noalloc: device %d outside [0, %d)", device, NOALLOC_MAX_DEVICES);
+  VH_REQUIRE(n >= 2 && n <= MOM_MAXN,
+             "vh_multipole_noalloc: n = %d outside [2, %d] (the scratch holds the partials of "
+             "at most %d^3 grids)", n, MOM_MAXN, MOM_MAXN);
+  VH_REQUIRE(20 + 10 * moments_blocks(n) <= NOALLOC_DOUBLES,
+             "vh_multipole_noalloc: partials of n = %d beyond the scratch", n);
+  VH_CHECK_HIP(hipSetDevice(device));
+  double *sc = nullptr;
+  VH_TRY(scratch_of(device, &sc));
+  // every partial the finishing kernel reads is written by the moments
+  // kernel of the same call: nothing to clear
+  VH_TRY(multipole_launch((hipStream_t)stream, rho, dual, n, sc + 20, sc, out10));
   return VH_OK;
 }
 
