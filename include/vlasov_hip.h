@@ -91,6 +91,10 @@ int vh_mesh_destroy(vh_mesh *m);
 /* `Array(dev.rho[l])` etc. (src/simulation.jl:544-552): n^3 floats. */
 int vh_mesh_grid_download(vh_mesh *m, int which, float *host);
 int vh_mesh_grid_upload(vh_mesh *m, int which, const float *host);
+/* copyto!(dev.phi[l], csol) at the end of _update_forces_resident!
+ * (src/simulation.jl:527-530): grid `dst` of the mesh := grid `src`, on the
+ * device, asynchronously. */
+int vh_mesh_grid_copy(vh_mesh *m, int dst, int src);
 /* The frame of a film step, Float32.((rho[:, :, iz1] .+ rho[:, :, iz2]) ./ 2)
  * of the mesh's density (scripts/xenon.jl:396-410, after sync_host!),
  * reduced on the device: n^2 floats, i fastest; iz1, iz2 1-based planes. */
@@ -214,6 +218,38 @@ int vh_deposit_cic(vh_accel *a, vh_mesh *coarse, double charge);
 int vh_forces(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight,
               const double *proj, int soft_mode, double ball_radius,
               double red[4], int32_t *nout);
+
+/* --- the legacy route: a cloud held on the host --------------------------
+ * ForceAccelerator(MtlArray, fine, sm, npart, n) and update_forces!'s
+ * accelerator branch (ext/VlasovMetalExt.jl:51-54; src/simulation.jl:318-345;
+ * scripts/film_xenon_80M.jl:83-120): the cloud, the CIC deposit, the FP64
+ * Poisson chain and the Verlet stay with the caller; the smoothed deposit and
+ * the field run here.
+ *
+ * _pack!(acc, positions) (src/accelerator.jl:342-349): pos = npart x 3 FP64
+ * absolute positions in the cloud's own order (particle-major, (3, N)
+ * column-major), packed on the device in FP64 as _pack_kd_kernel! does
+ * (src/kernels.jl:134-145: nearest knot clamped to [1, nk], offset subtracted
+ * in FP64 then narrowed) and kept in FP64.  The next sort (vh_deposit_smoothed
+ * makes one) records which cloud particle each sorted slot holds; the density
+ * is then read with vh_mesh_grid_download (deposit_smoothed!(ρ::Array, acc,
+ * ...), src/accelerator.jl:531-578). */
+int vh_positions_upload(vh_accel *a, const double *pos);
+/* forces!(cloud, acc, fine, csol_f, coarse, csol_c, sm; escaped, projectile,
+ * packed) for that cloud (src/accelerator.jl:607-733): csol_f (n^3 FP64, the
+ * fine level) narrowed into the fine mesh's csol; the field kernel with the
+ * device law of the projectile on the cells whose 10^3 stencil fits; every
+ * other particle by the coarse spline_field of csol_c (nc^3 FP64 on knots_c,
+ * nc = 2 nkc) at its FP64 position, or the monopole of `escaped` charges
+ * beyond both grids, plus the projectile's own law (ball of ball_radius > 0,
+ * else the Gaussian of proj's scale).  forces: npart x 3 FP64 in the cloud's
+ * own order (the in-stencil FP32 forces widened, as T(f[s]) does).  proj =
+ * {x, y, z, charge, scale} or NULL; red[4] and outcount as vh_forces.
+ * Needs a sort of a cloud from vh_positions_upload (VH_ESTATE otherwise). */
+int vh_forces_host(vh_accel *a, vh_mesh *fine, const double *csol_f, const double *knots_c,
+                   int nkc, const double *csol_c, int nc, double weight, const double *proj,
+                   double ball_radius, double escaped, double *forces, double red[4],
+                   int32_t *outcount);
 
 /* step!(cloud, dt, acc; rcmax) -> (kinetic, escaped, angular[3])
  * (src/accelerator.jl:580-605): packed position Verlet, diagnostics summed

@@ -99,6 +99,7 @@ def _bind(L):
     L.or_enclosed.argtypes = [vp, i64, f64, f64, vp, f64]
     L.or_enclosed.restype = i64
     L.or_num_threads.restype = i32
+    L.or_fallback_points.argtypes = [vp, i64, vp, i32, vp, i32, f64, vp, i32, f64, f64, vp]
     L.or_locate.argtypes = [vp, i32, f32, f32, vp, i32, f32, C.POINTER(f32)]
     L.or_locate.restype = i32
     return L
@@ -192,6 +193,34 @@ def fallback(srec, outl, x0, h, knots_c, csolc, w, proj=None, ball=True, radius=
     lib().or_fallback(_p(srec), _p(ol), len(ol), x0, h, _p(kc), len(kc), _p(cs), nc, w,
                       None if pj is None else _p(pj), 1 if ball else 0, radius, escaped, _p(f))
     return f.reshape(-1, 3)
+
+
+def fallback_points(pts, knots_c, csolc, w, proj=None, ball=True, radius=5.0, escaped=0.0):
+    """The out-of-stencil force at FP64 points (a host-held cloud's own
+    positions), src/accelerator.jl:705-730.  proj = (x, y, z, charge)."""
+    p = np.ascontiguousarray(pts, dtype=np.float64)
+    kc = np.ascontiguousarray(knots_c, dtype=np.float64)
+    cs = np.ascontiguousarray(csolc, dtype=np.float64)
+    nc = round(len(cs) ** (1 / 3))
+    pj = None if proj is None else np.ascontiguousarray(proj, dtype=np.float64)
+    f = np.zeros(3 * len(p))
+    lib().or_fallback_points(_p(p), len(p), _p(kc), len(kc), _p(cs), nc, w,
+                             None if pj is None else _p(pj), 1 if ball else 0, radius, escaped, _p(f))
+    return f.reshape(-1, 3)
+
+
+def pack_kd(pos, x0, h, nk):
+    """_pack_kd_kernel! (src/kernels.jl:134-145): FP64 absolute positions ->
+    PackedParticle{Float32} records, knode = round((p - x0)/h) + 1 (half to
+    even) clamped to [1, nk], delta = Float32(p - (x0 + (k - 1) h)) with the
+    subtraction in FP64; the previous half set equal to the current one."""
+    p = np.asarray(pos, dtype=np.float64)
+    k = np.clip(np.rint((p - x0) / h) + 1, 1, nk).astype(np.int32)
+    d = (p - (x0 + (k.astype(np.float64) - 1) * h)).astype(np.float32)
+    rec = np.zeros(len(p), dtype=REC)
+    rec["knode"] = rec["pknode"] = k
+    rec["delta"] = rec["pdelta"] = d
+    return rec
 
 
 def spline_potential(knots, csol, pts):

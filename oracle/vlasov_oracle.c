@@ -506,6 +506,34 @@ static int spline_field(const double *knots, int nk, const double *csol, int nc,
   return 1;
 }
 
+/* the host fallback at given FP64 points (the positions of a host-held
+ * cloud, src/accelerator.jl:705-730 with p = cloud.positions[i]): coarse
+ * spline field, or the monopole of `escaped` beyond both grids, plus the
+ * projectile's own law.  proj = {x, y, z, charge} or NULL; force n x 3. */
+void or_fallback_points(const double *pts, int64_t n, const double *knots, int nkc,
+                        const double *csolc, int nc, double w, const double *proj, int ball,
+                        double radius, double escaped, double *force) {
+#pragma omp parallel for
+  for (int64_t u = 0; u < n; ++u) {
+    const double *p = pts + 3 * u;
+    double f[3], e[3];
+    if (spline_field(knots, nkc, csolc, nc, p, e)) {
+      for (int d = 0; d < 3; ++d) f[d] = w * e[d];
+    } else {
+      double r3 = pow(p[0] * p[0] + p[1] * p[1] + p[2] * p[2], 1.5);
+      double cf = -(w * w) * escaped / r3;
+      for (int d = 0; d < 3; ++d) f[d] = cf * p[d];
+    }
+    if (proj) {
+      double dd[3] = {proj[0] - p[0], proj[1] - p[1], proj[2] - p[2]};
+      double r2 = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
+      double m = -w * proj[3] * (ball ? ball_fk(r2, radius) : gaussian_fk(r2, radius));
+      for (int d = 0; d < 3; ++d) f[d] = f[d] - m * dd[d];
+    }
+    for (int d = 0; d < 3; ++d) force[3 * u + d] = f[d];
+  }
+}
+
 /* the host fallback of the device forces! (src/accelerator.jl:705-730):
  * coarse spline field, or the enclosed-charge monopole beyond both grids, plus
  * the projectile's own law.  proj = {x, y, z, charge}. */

@@ -9,14 +9,14 @@ reference (DeviceMesh, DeviceAccelerator) and the simulation driver.
 """
 from . import _lib, grids, physics
 from ._lib import VlasovHIPError, device_count
-from .device import Context, DeviceAccelerator, DeviceMesh, poisson_nested
+from .device import Context, DeviceAccelerator, DeviceMesh, ForceAccelerator, poisson_nested
 from .grids import GaussianSmoothing, SplineAxis, SplineMesh, stretched_axis, uniform_axis
 from .physics import (BallSoftening, GaussianSoftening, Jellium, PACKED_DTYPE, Projectile,
                       RadialProfile, read_radial_profile, sample_thomas_fermi)
 from .simulation import Simulation, SimulationParameters, xenon_parameters, xenon_projectile
 
 __all__ = [
-    "VlasovHIPError", "device_count", "Context", "DeviceAccelerator", "DeviceMesh",
+    "VlasovHIPError", "device_count", "Context", "DeviceAccelerator", "DeviceMesh", "ForceAccelerator",
     "poisson_nested", "GaussianSmoothing", "SplineAxis", "SplineMesh", "stretched_axis",
     "uniform_axis", "BallSoftening", "GaussianSoftening", "Jellium", "PACKED_DTYPE",
     "Projectile", "RadialProfile", "read_radial_profile", "sample_thomas_fermi", "Simulation",

@@ -41,6 +41,7 @@ _SIGNATURES = {
     "vh_mesh_destroy": [_vp],
     "vh_mesh_grid_download": [_vp, C.c_int, _f32p],
     "vh_mesh_grid_upload": [_vp, C.c_int, _f32p],
+    "vh_mesh_grid_copy": [_vp, C.c_int, C.c_int],
     "vh_density_slice": [_vp, C.c_int, C.c_int, _f32p],
     "vh_poisson": [_vp],
     "vh_poisson_nested": [_vp, _vp],
@@ -64,6 +65,9 @@ _SIGNATURES = {
     "vh_deposit_cic": [_vp, _vp, C.c_double],
     "vh_forces": [_vp, _vp, _vp, C.c_double, _vp, C.c_int, C.c_double, _vp, C.POINTER(C.c_int32)],
     "vh_verlet": [_vp, C.c_double, C.c_double, C.c_double, _vp],
+    "vh_positions_upload": [_vp, _f64p],
+    "vh_forces_host": [_vp, _vp, _f64p, _f64p, C.c_int, _f64p, C.c_int, C.c_double, _vp, C.c_double,
+                       C.c_double, _f64p, _f64p, C.POINTER(C.c_int32)],
     "vh_enclosed_count": [_vp, _f64p, C.c_double, C.POINTER(C.c_int64)],
     "vh_projectile_set": [_vp, _f64p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double,
                           C.c_double, C.c_double],

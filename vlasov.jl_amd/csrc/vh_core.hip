@@ -322,6 +322,17 @@ extern "C" int vh_mesh_grid_upload(vh_mesh *m, int which, const float *host) {
   return VH_OK;
 }
 
+extern "C" int vh_mesh_grid_copy(vh_mesh *m, int dst, int src) {
+  VH_REQUIRE(m != nullptr, "vh_mesh_grid_copy: null mesh");
+  float *d = mesh_grid(m, dst), *s = mesh_grid(m, src);
+  VH_REQUIRE(d && s, "vh_mesh_grid_copy: unknown grid %d or %d", dst, src);
+  if (d == s) return VH_OK;
+  VH_CHECK_HIP(hipSetDevice(m->ctx->device));
+  const size_t bytes = (size_t)m->n * m->n * m->n * 4;
+  VH_CHECK_HIP(hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, m->ctx->stream));
+  return VH_OK;
+}
+
 // ---------------------------------------------------------------------------
 // accelerator
 // ---------------------------------------------------------------------------
@@ -336,7 +347,8 @@ extern "C" int vh_accel_destroy(vh_accel *a) {
                   a->keys[0], a->keys[1], a->vals[0], a->vals[1], a->counts, a->scan_tmp,
                   a->occupied, a->bounds, a->cellmap, a->fitlist, a->nonfit, a->nonfit_prefix,
                   a->outlist, a->counters, a->cellrun, a->tiles, a->cicwins, a->dpart, a->dscal, a->cellscratch,
-                  a->dproj, a->units, a->unitstart, a->zero, a->ident};
+                  a->dproj, a->units, a->unitstart, a->zero, a->ident, a->pos64, a->cloud_perm,
+                  a->host_csol64, a->host_knots64, a->force64};
   for (void *p : ptrs) vh_free(p);
   delete a;
   return VH_OK;
@@ -533,6 +545,7 @@ extern "C" int vh_particles_upload(vh_accel *a, const void *records) {
   a->keys_fresh = 0;  // new records: the keys of the last push are stale
   a->rec_pending = 0;
   a->pushed_fused = 0;
+  a->track_cloud = a->cloud_valid = 0;  // no host-held cloud order any more
   VH_CHECK_HIP(hipMemcpyAsync(stage, records, (size_t)a->npart * 48, hipMemcpyHostToDevice,
                               a->ctx->stream));
   hipLaunchKernelGGL(k_records_in, dim3(2048), dim3(256), 0, a->ctx->stream, stage,

@@ -927,7 +927,8 @@ int vh_energy_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, d
 }
 
 int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, const double *proj,
-                   int soft_mode, double ball_radius, const double *dproj, double push_a) {
+                   int soft_mode, double ball_radius, const double *dproj, double push_a,
+                   bool fallback) {
   vh_context *c = a->ctx;
   hipStream_t st = c->stream;
   // the exact-order kernel reads the records in slot order
@@ -1009,10 +1010,14 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
   fa.w = weight;
   fa.escaped = 0.0;
   fa.dev = dproj;
-  hipLaunchKernelGGL(k_fallback, dim3(1024), dim3(256), 0, st, a->parts[a->cur], vh_rec_perm(a), a->npad,
-                     a->outlist, a->counters, a->x0, a->h, coarse->csol, coarse->knots64, coarse->nk,
-                     coarse->n, fa, pu, a->force);
-  VH_LAUNCH_CHECK();
+  if (fallback) {
+    hipLaunchKernelGGL(k_fallback, dim3(1024), dim3(256), 0, st, a->parts[a->cur], vh_rec_perm(a),
+                       a->npad, a->outlist, a->counters, a->x0, a->h, coarse->csol, coarse->knots64,
+                       coarse->nk, coarse->n, fa, pu, a->force);
+    VH_LAUNCH_CHECK();
+  } else {
+    VH_REQUIRE(!pu.on, "forces: the push needs the fallback");
+  }
   if (pu.on) {
     a->cur = 1 - a->cur;  // the pushed records, in slot order
     a->rec_pending = 0;
@@ -1163,5 +1168,208 @@ int vh_verlet_impl(vh_accel *a, double dt, double mass, double rcmax) {
   a->pushed_fused = 0;
   hipLaunchKernelGGL(k_reduce_cols, dim3(5), dim3(256), 0, c->stream, a->dpart, grid, 5, a->dscal);
   VH_LAUNCH_CHECK();
+  return VH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// the legacy route: a cloud held on the host (ForceAccelerator(MtlArray, ...),
+// update_forces!'s accelerator branch, src/simulation.jl:318-345)
+// ---------------------------------------------------------------------------
+// _pack_kd_kernel! (src/kernels.jl:134-145), in FP64 on the device: the
+// nearest knot round((p - x0)/h) + 1 (half to even) clamped to [1, nk], the
+// offset p - (x0 + (k - 1) h) subtracted in FP64 and only then narrowed.  The
+// previous half gets the same record (the legacy route never reads it).
+__global__ void k_pack_kd(const double *__restrict__ pos, int32_t *__restrict__ P, int64_t n,
+                          int64_t npad, double x0, double h, int nk) {
+  int32_t *Q = vh_prev_half(P, npad);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    vh_pos r;
+    for (int d = 0; d < 3; ++d) {
+      const double p = pos[3 * i + d];
+      int k = (int)rint((p - x0) / h) + 1;
+      k = k < 1 ? 1 : (k > nk ? nk : k);
+      r.k[d] = k;
+      r.d[d] = (float)(p - (x0 + (double)(k - 1) * h));
+    }
+    vh_store_pos(P, i, r);
+    vh_store_pos(Q, i, r);
+  }
+}
+
+// The forces of every particle in the cloud's own order, FP64
+// (src/accelerator.jl:660-731): a particle of a cell whose 10^3 stencil fits
+// takes the field kernel's FP32 force widened; any other one the coarse
+// spline_field of the host's FP64 coefficients at its FP64 position (or the
+// escaped-charge monopole beyond both grids), plus the projectile's own law.
+__global__ void k_forces_host(const int32_t *__restrict__ P, int64_t npart, int64_t npad,
+                              const uint32_t *__restrict__ cperm, const double *__restrict__ pos,
+                              const float *__restrict__ force, int nk, int n,
+                              const double *__restrict__ csolc, const double *__restrict__ kc,
+                              int nkc, int nc, vh_fallback_args fa, double *__restrict__ out) {
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < npart;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    const vh_pos r = vh_load_pos(P, s);
+    bool fits = true;
+    for (int d = 0; d < 3; ++d) {
+      const int k = r.k[d] < 1 ? 1 : (r.k[d] > nk ? nk : r.k[d]);
+      const int b = 2 * k - 5;
+      fits = fits && b >= 1 && b + 9 <= n;
+    }
+    const int64_t i = cperm[s];
+    if (fits) {
+      out[3 * i] = (double)force[s];
+      out[3 * i + 1] = (double)force[npad + s];
+      out[3 * i + 2] = (double)force[2 * npad + s];
+      continue;
+    }
+    const double p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    int cell[3];
+    bool outside = false;
+    for (int d = 0; d < 3; ++d) {
+      cell[d] = cell_index_d(kc, nkc, p[d]);
+      outside = outside || cell[d] < 0;
+    }
+    double f0, f1, f2;
+    if (outside) {
+      const double r3 = pow(p[0] * p[0] + p[1] * p[1] + p[2] * p[2], 1.5);
+      const double cf = -(fa.w * fa.w) * fa.escaped / r3;
+      f0 = cf * p[0];
+      f1 = cf * p[1];
+      f2 = cf * p[2];
+    } else {
+      double v[3][4], dv[3][4];
+      for (int d = 0; d < 3; ++d)
+        for (int a = 0; a < 4; ++a) hermite_vd(kc, nkc, 2 * cell[d] + a, p[d], v[d][a], dv[d][a]);
+      double ex = 0.0, ey = 0.0, ez = 0.0;
+      for (int ak = 0; ak < 4; ++ak) {
+        const int k = 2 * cell[2] + ak;
+        for (int aj = 0; aj < 4; ++aj) {
+          const int j = 2 * cell[1] + aj;
+          for (int ai = 0; ai < 4; ++ai) {
+            const int ii = 2 * cell[0] + ai;
+            const double c = csolc[ii + (int64_t)nc * (j + (int64_t)nc * k)];
+            ex -= c * dv[0][ai] * v[1][aj] * v[2][ak];
+            ey -= c * v[0][ai] * dv[1][aj] * v[2][ak];
+            ez -= c * v[0][ai] * v[1][aj] * dv[2][ak];
+          }
+        }
+      }
+      f0 = fa.w * ex;
+      f1 = fa.w * ey;
+      f2 = fa.w * ez;
+    }
+    if (fa.has_proj) {
+      const double d0 = fa.px - p[0], d1 = fa.py - p[1], d2 = fa.pz - p[2];
+      const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      const double fk = fa.ball ? ball_force_kernel_d(r2, fa.radius) : gaussian_force_kernel_d(r2, fa.radius);
+      const double m = -fa.w * fa.q * fk;
+      f0 = f0 - m * d0;
+      f1 = f1 - m * d1;
+      f2 = f2 - m * d2;
+    }
+    out[3 * i] = f0;
+    out[3 * i + 1] = f1;
+    out[3 * i + 2] = f2;
+  }
+}
+
+extern "C" int vh_positions_upload(vh_accel *a, const double *pos) {
+  VH_REQUIRE(a && pos, "vh_positions_upload: null argument");
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  if (!a->pos64) VH_TRY(vh_alloc((void **)&a->pos64, sizeof(double) * 3 * (size_t)a->npart));
+  if (!a->cloud_perm) VH_TRY(vh_alloc((void **)&a->cloud_perm, sizeof(uint32_t) * (size_t)a->npart));
+  hipStream_t st = a->ctx->stream;
+  VH_CHECK_HIP(hipMemcpyAsync(a->pos64, pos, sizeof(double) * 3 * (size_t)a->npart,
+                              hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_pack_kd, dim3(2048), dim3(256), 0, st, a->pos64, a->parts[a->cur], a->npart,
+                     a->npad, a->x0, a->h, a->nk);
+  VH_LAUNCH_CHECK();
+  // new records in the cloud's order: no push state, and the next sort
+  // records slot -> cloud index
+  a->keys_fresh = a->hist_fresh = 0;
+  a->rec_pending = 0;
+  a->pushed_fused = 0;
+  a->track_cloud = 1;
+  a->cloud_valid = 0;
+  VH_CHECK_HIP(hipStreamSynchronize(st));
+  return VH_OK;
+}
+
+static int grow(double **p, int64_t *cap, int64_t need) {
+  if (*cap >= need) return VH_OK;
+  vh_free(*p);
+  *p = nullptr;
+  *cap = 0;
+  VH_TRY(vh_alloc((void **)p, sizeof(double) * (size_t)need));
+  *cap = need;
+  return VH_OK;
+}
+
+extern "C" int vh_forces_host(vh_accel *a, vh_mesh *fine, const double *csol_f,
+                              const double *knots_c, int nkc, const double *csol_c, int nc,
+                              double weight, const double *proj, double ball_radius, double escaped,
+                              double *forces, double red[4], int32_t *outcount) {
+  VH_REQUIRE(a && fine && csol_f && knots_c && csol_c && forces, "vh_forces_host: null argument");
+  VH_REQUIRE(fine->n == a->n && fine->ctx == a->ctx,
+             "vh_forces_host: fine mesh n=%d (accelerator n=%d) or context differ", fine->n, a->n);
+  VH_REQUIRE(nkc >= 2 && nc == 2 * nkc, "vh_forces_host: coarse grid of %d points for %d knots",
+             nc, nkc);
+  VH_REQUIRE(std::isfinite(weight) && std::isfinite(escaped),
+             "vh_forces_host: weight and escaped must be finite");
+  if (proj) {
+    VH_REQUIRE(proj[4] > 0, "vh_forces_host: projectile scale must be positive");
+    VH_REQUIRE(std::isfinite(proj[0]) && std::isfinite(proj[1]) && std::isfinite(proj[2]) &&
+                   std::isfinite(proj[3]), "vh_forces_host: projectile state must be finite");
+  }
+  for (int j = 0; j + 1 < nkc; ++j)
+    VH_REQUIRE(knots_c[j + 1] > knots_c[j], "vh_forces_host: coarse knots must increase");
+  if (!a->cloud_valid) {
+    vh_set_error("vh_forces_host: no sort of a cloud uploaded by vh_positions_upload "
+                 "(deposit it first, as update_forces! does)");
+    return VH_ESTATE;
+  }
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  hipStream_t st = a->ctx->stream;
+  // the fine coefficients narrowed to the device's precision
+  // (acc.csol.host .= csol_fine, src/accelerator.jl:624-626)
+  const size_t g = (size_t)fine->n * fine->n * fine->n;
+  std::vector<float> cs(g);
+  for (size_t t = 0; t < g; ++t) cs[t] = (float)csol_f[t];
+  VH_CHECK_HIP(hipMemcpyAsync(fine->csol, cs.data(), g * 4, hipMemcpyHostToDevice, st));
+  // the coarse ones stay FP64, as the host fallback reads them
+  const int64_t gc = (int64_t)nc * nc * nc;
+  VH_TRY(grow(&a->host_csol64, &a->host_csol_cap, gc));
+  VH_TRY(grow(&a->host_knots64, &a->host_knots_cap, nkc));
+  VH_CHECK_HIP(hipMemcpyAsync(a->host_csol64, csol_c, (size_t)gc * 8, hipMemcpyHostToDevice, st));
+  VH_CHECK_HIP(hipMemcpyAsync(a->host_knots64, knots_c, (size_t)nkc * 8, hipMemcpyHostToDevice, st));
+  if (!a->force64) VH_TRY(vh_alloc((void **)&a->force64, sizeof(double) * 3 * (size_t)a->npart));
+  VH_TRY(vh_rec_settle(a));
+  const int tok = vh_prof_begin(a->ctx, VH_ST_FIELD);
+  const int rc = vh_forces_impl(a, fine, fine, weight, proj, VH_SOFT_DEVICE, ball_radius, nullptr,
+                                0.0, false);
+  vh_prof_end(a->ctx, tok);
+  if (rc != VH_OK) return rc;
+  vh_fallback_args fa;
+  fa.px = proj ? proj[0] : 0.0;
+  fa.py = proj ? proj[1] : 0.0;
+  fa.pz = proj ? proj[2] : 0.0;
+  fa.q = proj ? proj[3] : 0.0;
+  fa.ball = ball_radius > 0;
+  fa.radius = ball_radius > 0 ? ball_radius : (proj ? proj[4] : 1.0);
+  fa.has_proj = proj != nullptr;
+  fa.w = weight;
+  fa.escaped = escaped;
+  fa.dev = nullptr;
+  hipLaunchKernelGGL(k_forces_host, dim3(2048), dim3(256), 0, st, a->parts[a->cur], a->npart,
+                     a->npad, a->cloud_perm, a->pos64, a->force, a->nk, a->n, a->host_csol64,
+                     a->host_knots64, nkc, nc, fa, a->force64);
+  VH_LAUNCH_CHECK();
+  VH_CHECK_HIP(hipMemcpyAsync(forces, a->force64, sizeof(double) * 3 * (size_t)a->npart,
+                              hipMemcpyDeviceToHost, st));
+  if (red) VH_CHECK_HIP(hipMemcpyAsync(red, a->dscal + 8, 32, hipMemcpyDeviceToHost, st));
+  if (outcount)
+    VH_CHECK_HIP(hipMemcpyAsync(outcount, a->counters + CNT_OUTCOUNT, 4, hipMemcpyDeviceToHost, st));
+  VH_CHECK_HIP(hipStreamSynchronize(st));
   return VH_OK;
 }

@@ -276,6 +276,16 @@ struct vh_accel {
   int has_dproj;
   double proj_charge, proj_scale, proj_ball_radius;
   int proj_soft_mode;
+  // a cloud held on the host (vh_positions_upload, vh_forces_host): its FP64
+  // positions in its own order, and slot -> cloud index after each sort
+  double *pos64;
+  uint32_t *cloud_perm;
+  int track_cloud;             // the next sort starts from the cloud's order
+  int cloud_valid;             // cloud_perm holds the current slots' indices
+  double *host_csol64;         // FP64 coarse coefficients and knots of vh_forces_host
+  double *host_knots64;
+  int64_t host_csol_cap, host_knots_cap;
+  double *force64;             // npart x 3, the cloud's order
 };
 
 // ---------------------------------------------------------------------------
@@ -388,8 +398,11 @@ int vh_total_charge_impl(vh_mesh *m, const float *rho, double *dq);
 // push_a > 0: the field and fallback kernels also push their particles
 // (Verlet factor dt^2 / mass) and leave the next sort's keys; requires the
 // matrix-core field kernel and vh_verlet_tiles
+// fallback = false leaves the out-of-stencil slots of the force buffer to the
+// caller (vh_forces_host resolves them in FP64 from the host positions)
 int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, const double *proj,
-                   int soft_mode, double ball_radius, const double *dproj, double push_a = 0.0);
+                   int soft_mode, double ball_radius, const double *dproj, double push_a = 0.0,
+                   bool fallback = true);
 int vh_projectile_step_impl(vh_accel *a, double dt);
 int vh_verlet_impl(vh_accel *a, double dt, double mass, double rcmax);
 // the push leaves the next sort's keys and first-digit histograms

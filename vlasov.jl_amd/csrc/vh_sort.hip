@@ -404,6 +404,14 @@ __global__ void k_outlist(const int32_t *__restrict__ nonfit, const int32_t *__r
   }
 }
 
+// out[s] = prev[perm[s]]: slot -> cloud index across two sorts
+__global__ void k_compose_perm(const uint32_t *__restrict__ perm, const uint32_t *__restrict__ prev,
+                               int64_t n, uint32_t *__restrict__ out) {
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
+       s += (int64_t)gridDim.x * blockDim.x)
+    out[s] = prev[perm[s]];
+}
+
 // the gather of a sort whose records stayed in place
 int vh_rec_settle(vh_accel *a) {
   if (!a->rec_pending) return VH_OK;
@@ -464,6 +472,21 @@ int vh_sort_impl(vh_accel *a) {
   }
   const uint32_t *skeys = a->keys[src];
   const uint32_t *perm = a->vals[src];
+  // a cloud held on the host: slot -> the cloud's own index
+  if (a->cloud_perm && !in_place && (a->track_cloud || a->cloud_valid)) {
+    if (a->track_cloud) {
+      VH_CHECK_HIP(hipMemcpyAsync(a->cloud_perm, perm, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    } else {   // compose with the slots of the previous sort
+      uint32_t *tmp = a->keys[1 - src];
+      hipLaunchKernelGGL(k_compose_perm, dim3(1024), dim3(256), 0, st, perm, a->cloud_perm, n, tmp);
+      VH_LAUNCH_CHECK();
+      VH_CHECK_HIP(hipMemcpyAsync(a->cloud_perm, tmp, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    }
+    a->cloud_valid = 1;
+  } else {
+    a->cloud_valid = 0;
+  }
+  a->track_cloud = 0;
   if (in_place) {
     a->rec_pending = 1;
     a->rec_perm = perm;

@@ -84,6 +84,10 @@ class DeviceMesh:
             raise ValueError(f"grid must hold {self.n}^3 values, got {a.size}")
         _lib.call("vh_mesh_grid_upload", self.handle, int(which), a)
 
+    def copy_grid(self, dst: int, src: int):
+        """copyto!(dst grid, src grid) on the device (src/simulation.jl:527-530)."""
+        _lib.call("vh_mesh_grid_copy", self.handle, int(dst), int(src))
+
     def density_slice(self, iz1: int, iz2: int) -> np.ndarray:
         """A film frame, Float32.((rho[:, :, iz1] .+ rho[:, :, iz2]) ./ 2) with
         1-based planes (scripts/xenon.jl:396-410), reduced on the device:
@@ -338,3 +342,83 @@ class DeviceAccelerator:
             self.close()
         except Exception:
             pass
+
+
+class ForceAccelerator:
+    """The legacy accelerator route, `ForceAccelerator(ArrayT, fine, sm, npart,
+    n)` (ext/VlasovMetalExt.jl:51-54) as `update_forces!`'s accelerator branch
+    uses it (src/simulation.jl:318-345; scripts/film_xenon_80M.jl:83-120):
+    the cloud stays on the host as FP64 positions, and so do the CIC deposit,
+    the FP64 Poisson chain and the Verlet, which are the caller's; the
+    smoothed deposit and the field run on the device.  Methods mirror
+    `deposit_smoothed!(ρ::Array, acc, mesh, sm, positions; charge)`,
+    `forces!(cloud, acc, fine, csol_f, coarse, csol_c, sm; escaped,
+    projectile, packed)` and `projectile_forces!(cloud, acc, proj, jel)`
+    (src/accelerator.jl:531, 607, 746)."""
+
+    def __init__(self, ctx: Context, fine: SplineMesh, sm: GaussianSmoothing, npart: int):
+        self.mesh = DeviceMesh(ctx, fine)
+        self.acc = DeviceAccelerator(ctx, fine.axis, sm, npart)
+        self.npart = int(npart)
+        self.n = self.acc.n
+        self.reduction = np.zeros(4)
+
+    def _positions(self, positions) -> np.ndarray:
+        pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64))
+        if pos.shape != (self.npart, 3):
+            raise ValueError(f"accelerator sized for {self.npart} particles")
+        return pos
+
+    def deposit_smoothed(self, rho: np.ndarray, positions, charge: float) -> int:
+        """deposit_smoothed!(ρ, acc, mesh, sm, positions; charge) -> nout:
+        packs and sorts the host cloud on the device, deposits, renormalises,
+        and writes the density into the host array `rho` (n^3, FP64)."""
+        if rho.size != self.n ** 3:
+            raise ValueError("ρ must cover the whole collocation grid")
+        _lib.call("vh_positions_upload", self.acc.handle, self._positions(positions))
+        nout = self.acc.deposit_smoothed(self.mesh, charge)
+        rho.reshape(-1)[:] = self.mesh.rho
+        return nout
+
+    def forces(self, forces: np.ndarray, positions, csol_fine, coarse_knots, csol_coarse,
+               weight: float, projectile: Projectile | None = None, escaped: float = 0.0,
+               packed: bool = True) -> int:
+        """forces!(cloud, acc, fine, csol_f, coarse, csol_c, sm; escaped,
+        projectile, packed) -> outcount: the forces of the cloud, in its own
+        order and FP64, written into `forces` (N x 3).  `packed` as the
+        driver passes it: the deposit of the same step has already packed the
+        cloud (otherwise `positions` are packed again first)."""
+        if not packed:
+            _lib.call("vh_positions_upload", self.acc.handle, self._positions(positions))
+        f = np.zeros((self.npart, 3))
+        kc = np.ascontiguousarray(coarse_knots, dtype=np.float64)
+        cc = np.ascontiguousarray(np.asarray(csol_coarse, dtype=np.float64).ravel())
+        cf = np.ascontiguousarray(np.asarray(csol_fine, dtype=np.float64).ravel())
+        if cf.size != self.n ** 3:
+            raise ValueError(f"fine coefficients must hold {self.n}^3 values")
+        nc = 2 * len(kc)
+        if cc.size != nc ** 3:
+            raise ValueError(f"coarse coefficients must hold {nc}^3 values")
+        if projectile is None:
+            pp, radius = None, 0.0
+        else:
+            pp = np.array([*projectile.position, projectile.charge, projectile.scale])
+            radius = projectile.softening.radius if isinstance(projectile.softening, BallSoftening) else 0.0
+        out = C.c_int32()
+        _lib.call("vh_forces_host", self.acc.handle, self.mesh.handle, cf, kc, len(kc), cc, nc,
+                  float(weight), None if pp is None else pp.ctypes.data, float(radius),
+                  float(escaped), f, self.reduction, C.byref(out))
+        forces[...] = f.reshape(forces.shape)
+        return int(out.value)
+
+    def projectile_forces(self, weight: float, proj: Projectile, jel: Jellium):
+        """projectile_forces!(cloud, acc, proj, jel) on the reduction of the
+        last forces (src/accelerator.jl:746-756)."""
+        from .physics import jellium_force
+        fjel, e_jel = jellium_force(proj, jel)
+        force = tuple(a + b for a, b in zip(fjel, self.reduction[:3]))
+        return force, weight * proj.charge * self.reduction[3], e_jel
+
+    def close(self):
+        self.acc.close()
+        self.mesh.close()
