@@ -172,6 +172,17 @@ int vh_accel_set_option(vh_accel *a, int option, int value);
  * order. */
 int vh_particles_upload(vh_accel *a, const void *records);
 int vh_particles_download(vh_accel *a, void *records);
+/* upload! of slots [offset, offset + count) only: a cloud built in slices
+ * (no counterpart in the reference, whose construction holds the whole
+ * cloud on the host as FP64 positions and momenta, docs/src/device.md:196-200).
+ * The other slots keep their records. */
+int vh_particles_upload_range(vh_accel *a, int64_t offset, int64_t count, const void *records);
+/* prime_leapfrog!'s second stage (src/simulation.jl:265-273,
+ * full_step_back src/particles.jl:466-469) on the device: every record holding
+ * q(-dt/2) (current) and q(0) (previous), with the forces of the last forces
+ * call at q(-dt/2), becomes q(0) and -q0 + 2 q(-dt/2) + coef f, in FP64.
+ * coef = dt/M (the thesis's coefficient) or dt^2/(4M) (consistent). */
+int vh_prime(vh_accel *a, double coef);
 /* download!(acc.force): (3, npart) floats, slot order (src/accelerator.jl:647). */
 int vh_forces_download(vh_accel *a, float *host);
 int vh_forces_upload(vh_accel *a, const float *host);
@@ -335,6 +346,15 @@ int vh_profile_read(vh_context *ctx, double *ms, int64_t *count);
  * sample_thomas_fermi(::PotentialProfile) (src/initial.jl:204-229).
  * pos, mom: npart x 3 doubles, particle-major ((3, N) column-major). */
 int vh_ran2_sequence(int seed, int consistent, int64_t count, float *out);
+/* sample_thomas_fermi(::RadialProfile, n, w; rng) in slices: a sampler holds
+ * the profile and the generator (seeded as Ran2(seed)); each vh_sampler_next
+ * draws the next `count` particles of the sequence a single vh_sample_radial
+ * call would draw (pos, mom: count x 3 doubles). */
+typedef struct vh_sampler vh_sampler;
+int vh_sampler_create(const double *quantiles, int nq, const double *density, int nrho,
+                      double rmax, double weight, int seed, vh_sampler **out);
+int vh_sampler_next(vh_sampler *s, int64_t count, double *pos, double *mom);
+int vh_sampler_destroy(vh_sampler *s);
 int vh_sample_radial(const double *quantiles, int nq, const double *density, int nrho,
                      double rmax, int64_t npart, double weight, int seed,
                      double *pos, double *mom);

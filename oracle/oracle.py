@@ -223,6 +223,44 @@ def pack_kd(pos, x0, h, nk):
     return rec
 
 
+def split_positions(x, x0, h):
+    """_split (src/particles.jl:135-138): k = round((x - x0)/h) + 1 (half to
+    even, unclamped), delta = Float32(x - (x0 + (k - 1) h))."""
+    x = np.asarray(x, dtype=np.float64)
+    k = np.rint((x - x0) / h) + 1
+    return k.astype(np.int32), (x - (x0 + (k - 1) * h)).astype(np.float32)
+
+
+def unsplit_positions(k, d, x0, h):
+    """getindex of PackedPositions (src/particles.jl:128-132): x0 + (k - 1) h + T(delta)."""
+    return (x0 + (k.astype(np.float64) - 1) * h) + d.astype(np.float64)
+
+
+def half_step_back(positions, momenta, M, dt):
+    """half_step_back (src/particles.jl:441-442): q .- (dt / 2M) .* p."""
+    return np.asarray(positions, np.float64) - (dt / (2 * M)) * np.asarray(momenta, np.float64)
+
+
+def full_step_back(q0, hp, f, M, dt, consistent=False):
+    """full_step_back (src/particles.jl:466-469): .-q .+ 2 .* h .+ coef .* f,
+    coef = dt/M (the thesis's) or dt^2/4M."""
+    coef = dt ** 2 / (4 * M) if consistent else dt / M
+    return (-q0 + 2 * hp) + coef * f
+
+
+def prime_records(srec, force, x0, h, M, dt, consistent=False):
+    """The loop of prime_leapfrog! (src/simulation.jl:266-273) over records
+    in the sort's order holding q(-dt/2) (current) and q(0) (previous), with
+    the forces at q(-dt/2) in the same order (widened to FP64)."""
+    q0 = unsplit_positions(srec["pknode"], srec["pdelta"], x0, h)
+    hp = unsplit_positions(srec["knode"], srec["delta"], x0, h)
+    back = full_step_back(q0, hp, np.asarray(force, np.float32).astype(np.float64), M, dt, consistent)
+    out = srec.copy()
+    out["knode"], out["delta"] = split_positions(q0, x0, h)
+    out["pknode"], out["pdelta"] = split_positions(back, x0, h)
+    return out
+
+
 def spline_potential(knots, csol, pts):
     kc = np.ascontiguousarray(knots, dtype=np.float64)
     cs = np.ascontiguousarray(csol, dtype=np.float64)

@@ -130,3 +130,18 @@ def test_packed_split_round_trip():
     # ties round to even, as Julia's round
     k2, _ = va.physics.split(np.array([x0 + 0.5 * h, x0 + 1.5 * h]), x0, h)
     assert list(k2) == [1, 3]
+
+
+def test_sampler_in_slices_is_the_single_call_sequence():
+    """sample_thomas_fermi in slices (vh_sampler_*): the slices, put end to
+    end, are bit for bit the cloud one call draws from the same Ran2 seed."""
+    prof = va.read_radial_profile()
+    w = 196.0 / 10_000
+    pos, mom = va.sample_thomas_fermi(prof, 10_000, w, seed=-1)
+    s = va.physics.RadialSampler(prof, w, seed=-1)
+    try:
+        parts = [s.next(c) for c in (1000, 3333, 0, 5667)]
+    finally:
+        s.close()
+    assert np.concatenate([a for a, _ in parts]).tobytes() == pos.tobytes()
+    assert np.concatenate([b for _, b in parts]).tobytes() == mom.tobytes()

@@ -57,6 +57,12 @@ _SIGNATURES = {
     "vh_particles_upload": [_vp, _vp],
     "vh_particles_download": [_vp, _vp],
     "vh_forces_download": [_vp, _f32p],
+    "vh_particles_upload_range": [_vp, C.c_int64, C.c_int64, _vp],
+    "vh_prime": [_vp, C.c_double],
+    "vh_sampler_create": [_f64p, C.c_int, _f64p, C.c_int, C.c_double, C.c_double, C.c_int,
+                          C.POINTER(_vp)],
+    "vh_sampler_next": [_vp, C.c_int64, _f64p, _f64p],
+    "vh_sampler_destroy": [_vp],
     "vh_forces_upload": [_vp, _f32p],
     "vh_sort": [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
     "vh_sort_lists": [_vp, _i32p, _i32p, _i32p],

@@ -506,6 +506,23 @@ __global__ void k_records_in(const int32_t *__restrict__ rec, int32_t *__restric
   }
 }
 
+// records of slots [offset, offset + n)
+__global__ void k_records_in_at(const int32_t *__restrict__ rec, int32_t *__restrict__ P, int64_t n,
+                                int64_t npad, int64_t offset) {
+  int32_t *prev = vh_prev_half(P, npad);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int2 *r = reinterpret_cast<const int2 *>(rec + 12 * i);
+    int2 *c = reinterpret_cast<int2 *>(P + 6 * (offset + i)),
+         *q = reinterpret_cast<int2 *>(prev + 6 * (offset + i));
+#pragma unroll
+    for (int w = 0; w < 3; ++w) {
+      c[w] = r[w];
+      q[w] = r[3 + w];
+    }
+  }
+}
+
 __global__ void k_records_out(const int32_t *__restrict__ P, int32_t *__restrict__ rec, int64_t n,
                               int64_t npad) {
   const int32_t *prev = vh_prev_half(P, npad);
@@ -550,6 +567,32 @@ extern "C" int vh_particles_upload(vh_accel *a, const void *records) {
                               a->ctx->stream));
   hipLaunchKernelGGL(k_records_in, dim3(2048), dim3(256), 0, a->ctx->stream, stage,
                      a->parts[a->cur], a->npart, a->npad);
+  VH_LAUNCH_CHECK();
+  VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
+  return VH_OK;
+}
+
+extern "C" int vh_particles_upload_range(vh_accel *a, int64_t offset, int64_t count,
+                                         const void *records) {
+  VH_REQUIRE(a && records, "vh_particles_upload_range: null argument");
+  VH_REQUIRE(offset >= 0 && count >= 0 && offset + count <= a->npart,
+             "vh_particles_upload_range: [%lld, %lld) outside the %lld particles",
+             (long long)offset, (long long)(offset + count), (long long)a->npart);
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  if (count == 0) return VH_OK;
+  // the slots outside the range keep their records: settle a deferred move
+  // first, and the new records replace slots [offset, offset + count)
+  VH_TRY(vh_rec_settle(a));
+  int32_t *stage = a->parts[1 - a->cur];
+  a->keys_fresh = 0;
+  a->pushed_fused = 0;
+  a->track_cloud = a->cloud_valid = 0;
+  VH_CHECK_HIP(hipMemcpyAsync(stage, records, (size_t)count * 48, hipMemcpyHostToDevice,
+                              a->ctx->stream));
+  int32_t *P = a->parts[a->cur];
+  // the current and previous arrays of slots [offset, ...): shift both halves
+  hipLaunchKernelGGL(k_records_in_at, dim3(2048), dim3(256), 0, a->ctx->stream, stage, P, count,
+                     a->npad, offset);
   VH_LAUNCH_CHECK();
   VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
   return VH_OK;

@@ -1373,3 +1373,54 @@ extern "C" int vh_forces_host(vh_accel *a, vh_mesh *fine, const double *csol_f,
   VH_CHECK_HIP(hipStreamSynchronize(st));
   return VH_OK;
 }
+
+// ---------------------------------------------------------------------------
+// prime_leapfrog!'s second stage (src/simulation.jl:265-273; full_step_back,
+// src/particles.jl:466-469) on the device: with the records in slot order
+// holding q(-dt/2) (current) and q(0) (previous) and the forces at q(-dt/2),
+// each particle becomes q(0) (current) and -q0 + 2 hp + coef f (previous),
+// positions rebuilt and split again as the host loop's getindex / setindex!
+// do (src/particles.jl:128-145), in FP64 with the same operation order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void split_d(double x, double x0, double h, int &k, float &d) {
+  k = (int)rint((x - x0) / h) + 1;
+  d = (float)(x - (x0 + (double)(k - 1) * h));
+}
+
+__global__ void k_prime(int32_t *__restrict__ P, int64_t npad, int64_t n,
+                        const float *__restrict__ force, double x0, double h, double coef) {
+  int32_t *Q = vh_prev_half(P, npad);
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < n;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    const vh_pos hp = vh_load_pos(P, s), q0 = vh_load_pos(Q, s);
+    vh_pos cur, prev;
+    for (int d = 0; d < 3; ++d) {
+      const double q = (x0 + (double)(q0.k[d] - 1) * h) + (double)q0.d[d];
+      const double m = (x0 + (double)(hp.k[d] - 1) * h) + (double)hp.d[d];
+      const double f = (double)force[d * npad + s];
+      const double back = (-q + 2.0 * m) + coef * f;
+      split_d(q, x0, h, cur.k[d], cur.d[d]);
+      split_d(back, x0, h, prev.k[d], prev.d[d]);
+    }
+    vh_store_pos(P, s, cur);
+    vh_store_pos(Q, s, prev);
+  }
+}
+
+extern "C" int vh_prime(vh_accel *a, double coef) {
+  VH_REQUIRE(a != nullptr, "vh_prime: null accelerator");
+  VH_REQUIRE(std::isfinite(coef), "vh_prime: coefficient must be finite");
+  if (!a->sorted_once) {
+    vh_set_error("vh_prime: the forces of the half step have not been computed");
+    return VH_ESTATE;
+  }
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  VH_TRY(vh_rec_settle(a));   // records in slot order, as the forces are
+  hipLaunchKernelGGL(k_prime, dim3(2048), dim3(256), 0, a->ctx->stream, a->parts[a->cur], a->npad,
+                     a->npart, a->force, a->x0, a->h, coef);
+  VH_LAUNCH_CHECK();
+  a->keys_fresh = a->hist_fresh = 0;   // new positions
+  a->pushed_fused = 0;
+  a->track_cloud = a->cloud_valid = 0;
+  return VH_OK;
+}

@@ -116,6 +116,54 @@ extern "C" int vh_sample_radial(const double *quantiles, int nq, const double *d
   return VH_OK;
 }
 
+// sample_thomas_fermi(::RadialProfile, n, w; rng) in slices: the generator's
+// state carries over from one call to the next, so consecutive slices are
+// the particles a single call would have drawn, in order
+struct vh_sampler {
+  Ran2 g;
+  std::vector<double> quantiles, density;
+  double rmax, weight;
+  vh_sampler(int seed) : g(seed, false) {}
+};
+
+extern "C" int vh_sampler_create(const double *quantiles, int nq, const double *density, int nrho,
+                                 double rmax, double weight, int seed, vh_sampler **out) {
+  if (!quantiles || !density || !out || nq < 2 || nrho < 2 || !(rmax > 0)) {
+    vh_set_error("vh_sampler_create: bad argument");
+    return VH_EINVAL;
+  }
+  vh_sampler *s = new vh_sampler(seed);
+  s->quantiles.assign(quantiles, quantiles + nq);
+  s->density.assign(density, density + nrho);
+  s->rmax = rmax;
+  s->weight = weight;
+  *out = s;
+  return VH_OK;
+}
+
+extern "C" int vh_sampler_next(vh_sampler *s, int64_t count, double *pos, double *mom) {
+  if (!s || !pos || !mom || count < 0) {
+    vh_set_error("vh_sampler_next: bad argument");
+    return VH_EINVAL;
+  }
+  const int nq = (int)s->quantiles.size(), nrho = (int)s->density.size();
+  for (int64_t i = 0; i < count; ++i) {
+    double x[6];
+    for (int k = 0; k < 6; ++k) x[k] = (double)s->g.next();
+    const double r = interp_regular(s->quantiles.data(), x[0], nq);
+    on_sphere(r, x[1], x[2], pos + 3 * i);
+    const double rho = interp_regular(s->density.data(), r / s->rmax, nrho);
+    const double p = std::cbrt(x[3]) * FERMI_COEFFICIENT * std::cbrt(rho);
+    on_sphere(p * s->weight, x[4], x[5], mom + 3 * i);
+  }
+  return VH_OK;
+}
+
+extern "C" int vh_sampler_destroy(vh_sampler *s) {
+  delete s;
+  return VH_OK;
+}
+
 extern "C" int vh_sample_rejection(const double *grid, const double *potential, int ngrid, double rmax,
                                    double pmax, double fermi, int64_t npart, double weight, int seed,
                                    double *pos, double *mom) {

@@ -6,6 +6,7 @@ src/particles.jl:135-138, src/initial.jl).
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 from dataclasses import dataclass, field
 from pathlib import Path
@@ -275,6 +276,39 @@ def sample_thomas_fermi(profile, npart: int, weight: float, seed: int = -1):
         _lib.call("vh_sample_rejection", g, v, len(g), float(profile.rmax), float(profile.pmax),
                   float(profile.fermi), int(npart), float(weight), int(seed), pos, mom)
     return pos, mom
+
+
+class RadialSampler:
+    """sample_thomas_fermi(::RadialProfile, npart, weight; rng = Ran2(seed))
+    (src/initial.jl:94-113) drawn in slices: `next(count)` returns the next
+    `count` (positions, momenta) of the sequence one call would draw, so that
+    a cloud is built without holding all of its FP64 positions and momenta on
+    the host at once (docs/src/device.md:196-200)."""
+
+    def __init__(self, profile: "RadialProfile", weight: float, seed: int = -1):
+        q = np.ascontiguousarray(profile.quantiles, dtype=np.float64)
+        r = np.ascontiguousarray(profile.density, dtype=np.float64)
+        h = C.c_void_p()
+        _lib.call("vh_sampler_create", q, len(q), r, len(r), float(profile.rmax), float(weight),
+                  int(seed), C.byref(h))
+        self.handle = h
+
+    def next(self, count: int):
+        pos = np.zeros((count, 3))
+        mom = np.zeros((count, 3))
+        _lib.call("vh_sampler_next", self.handle, int(count), pos, mom)
+        return pos, mom
+
+    def close(self):
+        if self.handle:
+            _lib.call("vh_sampler_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def ran2_sequence(count: int, seed: int = -1, consistent: bool = False) -> np.ndarray:

@@ -20,8 +20,10 @@ import numpy as np
 
 from . import grids
 from .device import Context, DeviceAccelerator, DeviceMesh, poisson_nested
-from .physics import (ELECTRON_MASS, Jellium, Projectile, energy_budget, jellium_force,
-                      packed_records, record_positions, sample_thomas_fermi, split)
+from . import _lib
+from .physics import (ELECTRON_MASS, Jellium, Projectile, RadialProfile, RadialSampler,
+                      energy_budget, jellium_force, packed_records, record_positions,
+                      sample_thomas_fermi)
 
 
 @dataclass
@@ -75,7 +77,7 @@ class Simulation:
     def __init__(self, params: SimulationParameters, profile, *, projectile: Projectile | None = None,
                  seed: int = -1, softening_mode: str = "device", consistent_startup: bool = False,
                  device: int = 0, tables=None, positions=None, momenta=None,
-                 resident_projectile: bool = True):
+                 resident_projectile: bool = True, slice_size: int = 1 << 20):
         self.params = params
         self._projectile = projectile
         self._proj_on_device = False
@@ -90,7 +92,7 @@ class Simulation:
         self.jellium = Jellium.of(params.nions)
         self.weight = params.nelectrons / params.nparticles
         self.mass = ELECTRON_MASS * self.weight
-        if positions is None:
+        if positions is None and not isinstance(profile, RadialProfile):
             positions, momenta = sample_thomas_fermi(profile, params.nparticles, self.weight, seed)
         self.ctx = Context(device)
         self.dmf = DeviceMesh(self.ctx, fine)
@@ -102,7 +104,20 @@ class Simulation:
         self.red = np.zeros(4)
         self.step_count = 0
         self.diagnostics = None
-        self._prime_leapfrog(positions, momenta, consistent_startup)
+        if positions is None:
+            # the Thomas-Fermi sample drawn and uploaded slice by slice
+            sampler = RadialSampler(profile, self.weight, seed)
+            try:
+                slices = (sampler.next(min(slice_size, params.nparticles - i0))
+                          for i0 in range(0, params.nparticles, slice_size))
+                self._prime_leapfrog(slices, consistent_startup, slice_size)
+            finally:
+                sampler.close()
+        else:
+            n = params.nparticles
+            slices = ((positions[i0:i0 + slice_size], momenta[i0:i0 + slice_size])
+                      for i0 in range(0, n, slice_size))
+            self._prime_leapfrog(slices, consistent_startup, slice_size)
         if projectile is not None and resident_projectile:
             # from here the projectile is integrated on the device, and a step
             # waits for nothing on the host
@@ -121,23 +136,27 @@ class Simulation:
         return p
 
     # --- construction ------------------------------------------------------------
-    def _prime_leapfrog(self, positions, momenta, consistent):
-        """src/simulation.jl:233-277: q(-dt/2) from the momenta, forces there,
-        then q(-dt) by the thesis's (dimensionally dubious) coefficient dt/M."""
+    def _prime_leapfrog(self, slices, consistent, slice_size):
+        """prime_leapfrog! (src/simulation.jl:233-277): q(-dt/2) from the
+        momenta (half_step_back, src/particles.jl:441-442), with q(0) carried
+        in the previous half through the sort; the forces there; then q(-dt)
+        by the thesis's (dimensionally dubious) coefficient dt/M, or dt^2/4M
+        with `consistent` (full_step_back, src/particles.jl:466-469).  The
+        cloud goes up slice by slice (each slice's half step on the host in
+        FP64, as the reference's map does), and the second stage runs on the
+        device (vh_prime), so the whole cloud never crosses back."""
         M, dt = self.mass, self.params.dt
-        half = positions - (dt / (2 * M)) * momenta
-        rec = packed_records(half, positions, self.x0, self.h)
-        self.acc.upload(rec)
+        i0 = 0
+        for pos, mom in slices:
+            half = pos - (dt / (2 * M)) * mom
+            rec = packed_records(half, pos, self.x0, self.h)
+            _lib.call("vh_particles_upload_range", self.acc.handle, i0, len(rec), rec.ctypes.data)
+            i0 += len(rec)
+        if i0 != self.params.nparticles:
+            raise ValueError(f"{i0} particles for an accelerator of {self.params.nparticles}")
         self.update_forces(advance=False)
-        rec = self.acc.download()
-        f = self.acc.forces().astype(np.float64)
-        q0 = record_positions(rec, self.x0, self.h, previous=True)
-        hp = record_positions(rec, self.x0, self.h)
         coef = dt ** 2 / (4 * M) if consistent else dt / M
-        back = (-q0 + 2 * hp) + coef * f
-        rec["knode"], rec["delta"] = split(q0, self.x0, self.h)
-        rec["pknode"], rec["pdelta"] = split(back, self.x0, self.h)
-        self.acc.upload(rec)
+        _lib.call("vh_prime", self.acc.handle, float(coef))
 
     # --- the step ------------------------------------------------------------------
     def update_forces(self, advance: bool = True, energy: bool = False, counts: bool = False):
