@@ -107,7 +107,7 @@ CROSSING_STEPS = 700     # ceil(140 / (0.4 * 0.5)), scripts/xenon.jl:306-309
 
 
 def run_crossing(va, npart: int, nfine: int, mode: str, warmup: int, steps: int, profile: bool,
-                 seed: int = -1):
+                 seed: int = -1, precision: str = "fp32"):
     """The crossing at `npart` particles: `warmup` untimed steps, then `steps`
     timed ones, then, untimed, the rest of the 700 steps, after which the
     capture charge q(< 8 a0) is read (step 700 whatever `steps` is).  The
@@ -120,7 +120,8 @@ def run_crossing(va, npart: int, nfine: int, mode: str, warmup: int, steps: int,
     t0 = time.perf_counter()
     p = va.xenon_parameters(npart, nfine)
     proj = va.xenon_projectile()
-    sim = va.Simulation(p, va.read_radial_profile(), projectile=proj, softening_mode=mode, seed=seed)
+    sim = va.Simulation(p, va.read_radial_profile(), projectile=proj, softening_mode=mode, seed=seed,
+                        poisson_precision=precision)
     log(f"built in {time.perf_counter() - t0:.1f} s: {npart} particles, {sim.fine.n}^3 grids")
     nst = len(va._lib.STAGES)
     field_bit = 1 << va._lib.STAGES.index("field_kernel")
@@ -193,6 +194,25 @@ def capture_spread(va, npart: int, nfine: int, mode: str, seeds, q_main: float):
             "std": float(a.std(ddof=1)), "range": float(a.max() - a.min())}
 
 
+def precision_extension(va, npart: int, nfine: int, mode: str, warmup: int, steps: int):
+    """Config 3: the crossing at `npart` particles on the nfine grid, once
+    with FP32 rotating products and once with the Poisson solve's
+    contractions in split-BF16 x3 on the matrix cores: ms/step over the timed
+    steps and q(< 8 a0) at step 700 of each (the errors of the split solve
+    against the FP64 oracle are the GPU tests' at this size)."""
+    out = {"particles": npart, "grid": 2 * nfine + 2}
+    for prec in ("fp32", "bf16x3"):
+        log(f"precision extension: {prec}, {npart} particles, {2 * nfine + 2}^3...")
+        r = run_crossing(va, npart, nfine, mode, warmup, steps, False, precision=prec)
+        out[prec] = {"ms_per_step": 1000.0 * r["elapsed"] / steps, "capture_q8": r["q"],
+                     "capture_step": r["step"], "projectile_energy_loss_eV": r["eloss_ev"]}
+    out["capture_q_diff"] = abs(out["bf16x3"]["capture_q8"] - out["fp32"]["capture_q8"])
+    out["speedup"] = out["fp32"]["ms_per_step"] / out["bf16x3"]["ms_per_step"]
+    out["error_vs_fp64_at_258"] = ("tests/test_gpu_config3.py: fine potential 2.1e-5 (FP32 1.0e-6), "
+                                   "field 2.5e-3 (FP32 9.8e-5), max |diff| / max |ref|")
+    return out
+
+
 def isolated_drift(va, npart: int, nfine: int, steps: int, every: int):
     """Energy conservation of the isolated Na196 cluster, the reference's
     validation observable (src/energy.jl:1-7): the budget every `every` steps
@@ -232,6 +252,9 @@ def main():
     ap.add_argument("--spread-seeds", type=int, default=3,
                     help="extra Ran2 seeds (-2, -3, ...) of the crossing whose step-700 capture "
                          "charges give the run-to-run spread (0: skip)")
+    ap.add_argument("--precision-particles", type=int, default=80_000_000,
+                    help="particles of the precision extension's crossings on nfine=128 "
+                         "(FP32 and split-BF16 x3 Poisson; 0: skip)")
     ap.add_argument("--cpu-particles", type=int, default=100_000)
     ap.add_argument("--cpu-nfine", type=int, default=28)
     ap.add_argument("--cpu-steps", type=int, default=150)
@@ -315,6 +338,9 @@ def main():
             "ms_per_step": 1000.0 * rr["elapsed"] / args.steps,
             "projectile_energy_loss_eV": rr["eloss_ev"],
             "published_q8": {"8e6 (RTX 4070 / M1 Max)": [0.842, 0.840], "8e7 (M1 Max)": 0.81}}
+    if args.precision_particles > 0:
+        line["precision_extension"] = precision_extension(va, args.precision_particles, 128,
+                                                          args.mode, args.warmup, args.steps)
     if not args.no_cpu_baseline:
         log("cpu baseline...")
         line["cpu_baseline"] = cpu_baseline(va, args.cpu_nfine, args.cpu_particles, args.cpu_steps)

@@ -88,6 +88,18 @@ int vh_mesh_create(vh_context *ctx, int n, int nk,
                    double loc_x0, double loc_invwidth,
                    const int32_t *loc_cells, int nbin, vh_mesh **out);
 int vh_mesh_destroy(vh_mesh *m);
+/* Options of a mesh (no counterpart in the reference, whose products are
+ * BLAS GEMMs in the device precision, src/tensorsolver.jl:217-223).
+ *   VH_MESH_OPT_BF16X3  which rotating products run in split-BF16 x3 on
+ *                       v_mfma_f32_16x16x32_bf16 with FP32 accumulation
+ *                       (hi = bf16(x), lo = bf16(x - hi); A S ~ A_lo S_hi +
+ *                       A_hi S_lo + A_hi S_hi; operators split from FP64)
+ *                       instead of FP32 on v_mfma_f32_16x16x4_f32: 0 none
+ *                       (default), 1 the Poisson solve's M^-1 and M
+ *                       contractions, 2 the S^-1 transforms (spline
+ *                       coefficients, mean field), 3 both. */
+enum { VH_MESH_OPT_BF16X3 = 0 };
+int vh_mesh_set_option(vh_mesh *m, int option, int value);
 /* `Array(dev.rho[l])` etc. (src/simulation.jl:544-552): n^3 floats. */
 int vh_mesh_grid_download(vh_mesh *m, int which, float *host);
 int vh_mesh_grid_upload(vh_mesh *m, int which, const float *host);

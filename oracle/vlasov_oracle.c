@@ -35,6 +35,8 @@ typedef struct {
   float pd[3];
 } rec_t;
 
+int or_num_threads(void);
+
 static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 /* clamped 0-based cell key: src/kernels.jl:902-908, src/sorting.jl:134-151 */
@@ -130,26 +132,58 @@ static inline int cell_fits8(int64_t key, int nk, int ngrid, int *b) {
 
 /* sorted smoothed deposit, device semantics (src/kernels.jl:601-726), sums in
  * FP64; rho is the raw deposit (n^3 doubles, i fastest), not renormalised */
+/* Threads take contiguous ranges of occupied cells, each into its own FP64
+ * grid; the grids are summed in thread order (so the sum depends on the
+ * thread count only through FP64 rounding). */
+static double **thread_grids(int nt, int64_t npts) {
+  double **g = (double **)calloc((size_t)nt, sizeof(double *));
+  for (int t = 0; t < nt; ++t) g[t] = (double *)calloc((size_t)npts, sizeof(double));
+  return g;
+}
+
+static void sum_thread_grids(double **g, int nt, int64_t npts, double *out) {
+#pragma omp parallel for
+  for (int64_t i = 0; i < npts; ++i) {
+    double s = 0.0;
+    for (int t = 0; t < nt; ++t) s += g[t][i];
+    out[i] = s;
+  }
+  for (int t = 0; t < nt; ++t) free(g[t]);
+  free(g);
+}
+
 void or_deposit_smoothed(const rec_t *parts, const int32_t *occupied, const int32_t *bounds,
                          int64_t nocc, int nk, int ngrid, const double *nodes, float spacing,
                          float nbdt, int ncol, double *rho) {
-  int64_t nn = ngrid;
-  memset(rho, 0, sizeof(double) * nn * nn * nn);
-  for (int64_t g = 0; g < nocc; ++g) {
-    int b[3];
-    if (!cell_fits8(occupied[g], nk, ngrid, b)) continue;
-    for (int32_t s = bounds[g]; s < bounds[g + 1]; ++s) {
-      const double *gx = nodes + 8 * (col_f32(parts[s].d[0], spacing, nbdt, ncol) - 1);
-      const double *gy = nodes + 8 * (col_f32(parts[s].d[1], spacing, nbdt, ncol) - 1);
-      const double *gz = nodes + 8 * (col_f32(parts[s].d[2], spacing, nbdt, ncol) - 1);
-      for (int kk = 0; kk < 8; ++kk)
-        for (int jj = 0; jj < 8; ++jj) {
-          double w = gy[jj] * gz[kk];
-          double *row = rho + b[0] + nn * ((b[1] + jj) + nn * (b[2] + kk));
-          for (int ii = 0; ii < 8; ++ii) row[ii] += gx[ii] * w;
-        }
+  int64_t nn = ngrid, npts = nn * nn * nn;
+  int nt = or_num_threads();
+  double **tg = thread_grids(nt, npts);
+#pragma omp parallel num_threads(nt)
+  {
+#ifdef _OPENMP
+    const int t = omp_get_thread_num();
+#else
+    const int t = 0;
+#endif
+    double *dst = tg[t];
+    const int64_t lo = nocc * t / nt, hi = nocc * (t + 1) / nt;
+    for (int64_t g = lo; g < hi; ++g) {
+      int b[3];
+      if (!cell_fits8(occupied[g], nk, ngrid, b)) continue;
+      for (int32_t s = bounds[g]; s < bounds[g + 1]; ++s) {
+        const double *gx = nodes + 8 * (col_f32(parts[s].d[0], spacing, nbdt, ncol) - 1);
+        const double *gy = nodes + 8 * (col_f32(parts[s].d[1], spacing, nbdt, ncol) - 1);
+        const double *gz = nodes + 8 * (col_f32(parts[s].d[2], spacing, nbdt, ncol) - 1);
+        for (int kk = 0; kk < 8; ++kk)
+          for (int jj = 0; jj < 8; ++jj) {
+            double w = gy[jj] * gz[kk];
+            double *row = dst + b[0] + nn * ((b[1] + jj) + nn * (b[2] + kk));
+            for (int ii = 0; ii < 8; ++ii) row[ii] += gx[ii] * w;
+          }
+      }
     }
   }
+  sum_thread_grids(tg, nt, npts, rho);
 }
 
 /* _locate_cell in Float32 (src/kernels.jl:1069-1081): 1-based cell, 0 outside */
@@ -178,10 +212,19 @@ int or_locate(const float *gt, int ng, float x0t, float iwt, const int32_t *cell
  * weights in Float32 as the device computes them, products and sums in FP64.
  * rho raw (not divided by the dual volumes). */
 void or_deposit_cic(const rec_t *parts, int64_t n, float x0f, float hf, const float *gt, int ng,
-                    float x0t, float iwt, const int32_t *cells, int nbin, double *rho) {
-  int64_t nn = ng;
-  memset(rho, 0, sizeof(double) * nn * nn * nn);
-  for (int64_t s = 0; s < n; ++s) {
+                    float x0t, float iwt, const int32_t *cells, int nbin, double *out) {
+  int64_t nn = ng, npts = nn * nn * nn;
+  int nt = or_num_threads();
+  double **tg = thread_grids(nt, npts);
+#pragma omp parallel num_threads(nt)
+  {
+#ifdef _OPENMP
+  const int t = omp_get_thread_num();
+#else
+  const int t = 0;
+#endif
+  double *rho = tg[t];
+  for (int64_t s = n * t / nt; s < n * (t + 1) / nt; ++s) {
     int ix[3];
     float a[3];
     int in = 1;
@@ -206,6 +249,8 @@ void or_deposit_cic(const rec_t *parts, int64_t n, float x0f, float hf, const fl
     rho[(i + 1) + nn * ((j + 1) + nn * k)] += bx * by * az;
     rho[(i + 1) + nn * ((j + 1) + nn * (k + 1))] += bx * by * bz;
   }
+  }
+  sum_thread_grids(tg, nt, npts, out);
 }
 
 /* --- projectile laws (src/projectile.jl:79-108) ----------------------------- */

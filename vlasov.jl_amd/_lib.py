@@ -17,6 +17,7 @@ VH_PROF_ALL = -1
 VH_SOFT_DEVICE, VH_SOFT_BALL = 0, 1
 VH_OPT_EXACT_FIELD = 0
 VH_OPT_STEP_GRAPH = 1
+VH_MESH_OPT_BF16X3 = 0
 STAGES = ("sort", "deposit_fine", "deposit_cic", "poisson", "spline", "meanfield", "field",
           "verlet", "field_kernel", "deposit_kernel")
 
@@ -39,6 +40,7 @@ _SIGNATURES = {
     "vh_mesh_create": [_vp, C.c_int, C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
                        _f64p, _f64p, C.c_double, C.c_double, _i32p, C.c_int, C.POINTER(_vp)],
     "vh_mesh_destroy": [_vp],
+    "vh_mesh_set_option": [_vp, C.c_int, C.c_int],
     "vh_mesh_grid_download": [_vp, C.c_int, _f32p],
     "vh_mesh_grid_upload": [_vp, C.c_int, _f32p],
     "vh_mesh_grid_copy": [_vp, C.c_int, C.c_int],

@@ -167,7 +167,7 @@ extern "C" int vh_mesh_destroy(vh_mesh *m) {
   void *ptrs[] = {m->colloc, m->knots, m->colloc64, m->knots64, m->lap, m->Minv_op, m->M_op,
                   m->Sinv_op, m->invlsum, m->dual, m->dual32, m->duallen, m->loc_cells,
                   m->rho, m->phi, m->csol, m->extra, m->work, m->inner, m->work1, m->work2,
-                  m->partials, m->scal, m->acc64};
+                  m->partials, m->scal, m->acc64, m->Minv_b3, m->M_b3, m->Sinv_b3};
   for (void *p : ptrs) vh_free(p);
   delete m;
   return VH_OK;
@@ -209,6 +209,9 @@ extern "C" int vh_mesh_create(vh_context *ctx, int n, int nk, const double *coll
   MK(vh_gemm_pack(ctx, &m->Minv_op, Minv, ni));
   MK(vh_gemm_pack(ctx, &m->M_op, M, ni));
   MK(vh_gemm_pack(ctx, &m->Sinv_op, Sinv, n));
+  MK(vh_gemm_pack_b3(ctx, &m->Minv_b3, Minv, ni));
+  MK(vh_gemm_pack_b3(ctx, &m->M_b3, M, ni));
+  MK(vh_gemm_pack_b3(ctx, &m->Sinv_b3, Sinv, n));
   MK(upload_converted(ctx, &m->dual, dual, 3 * (size_t)n));
   MK(upload_converted(ctx, &m->dual32, dual, 3 * (size_t)n));
   MK(upload_converted(ctx, &m->duallen, duallen, n));
@@ -319,6 +322,14 @@ extern "C" int vh_mesh_grid_upload(vh_mesh *m, int which, const float *host) {
   size_t bytes = (size_t)m->n * m->n * m->n * 4;
   VH_CHECK_HIP(hipMemcpyAsync(g, host, bytes, hipMemcpyHostToDevice, m->ctx->stream));
   VH_CHECK_HIP(hipStreamSynchronize(m->ctx->stream));
+  return VH_OK;
+}
+
+extern "C" int vh_mesh_set_option(vh_mesh *m, int option, int value) {
+  VH_REQUIRE(m != nullptr, "vh_mesh_set_option: null mesh");
+  VH_REQUIRE(option == VH_MESH_OPT_BF16X3, "vh_mesh_set_option: unknown option %d", option);
+  VH_REQUIRE(value >= 0 && value <= 3, "vh_mesh_set_option: value %d outside 0..3", value);
+  m->gemm_bf16x3 = value;
   return VH_OK;
 }
 

@@ -82,6 +82,10 @@ struct vh_mesh {
   float *Minv_op;    // ni x ni, A operand of the rotating product (vh_gemm_pack order)
   float *M_op;
   float *Sinv_op;    // n x n, same order
+  // the same operators split into bf16 hi + lo (vh_gemm_pack_b3 order), and
+  // whether the mesh's products run in split-BF16 x3 (VH_MESH_OPT_BF16X3)
+  uint32_t *Minv_b3, *M_b3, *Sinv_b3;
+  int gemm_bf16x3;
   float *invlsum;    // ni^3, i fastest
   double *dual;      // 3 x n (order k at k*n), FP64
   float *dual32;     // same, narrowed
@@ -360,6 +364,7 @@ int vh_scan_exclusive(vh_context *c, uint32_t *data, int64_t n, uint32_t *tmp, i
 // A given in MFMA operand order by vh_gemm_pack (device copy allocated there)
 enum { EPI_STORE = 0, EPI_HADAMARD = 1, EPI_FILL_INTERIOR = 2, EPI_ACCUMULATE = 3 };
 int vh_gemm_pack(vh_context *c, float **dst, const double *colmajor, int n);
+int vh_gemm_pack_b3(vh_context *c, uint32_t **dst, const double *colmajor, int n);
 int vh_rotate_gemm(vh_context *c, const float *Aop, const float *src, float *out, int n, int64_t m,
                    int epi, const float *aux, int nfull);
 

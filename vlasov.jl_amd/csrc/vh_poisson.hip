@@ -14,6 +14,7 @@
 #include "vh_internal.h"
 
 #include <cmath>
+#include <cstring>
 #include <mutex>
 
 typedef float vh_f4 __attribute__((ext_vector_type(4)));
@@ -209,6 +210,261 @@ __global__ void __launch_bounds__(576)
   }
 }
 
+// ---------------------------------------------------------------------------
+// The same rotating product in split-BF16 x3 (the precision extension):
+// every operand x is split as hi = bf16(x), lo = bf16(x - hi), and
+// A S ~ A_lo S_hi + A_hi S_lo + A_hi S_hi on v_mfma_f32_16x16x32_bf16 with
+// FP32 accumulation (the dropped A_lo S_lo is ~2^-18 of each product).
+// The layout follows k_rot_gemm: one wave per row tile with its A operands in
+// registers, lane group g = l/16 owning the K slice [8 g KS, 8 (g+1) KS), one
+// 16x16x32 product per step of eight K values per lane group; the operators
+// are split on the host from FP64 (vh_gemm_pack_b3), and a source panel is
+// split once per block, when it is written to LDS, into a hi plane and a lo
+// plane of bf16 (the same 4 bytes per value as FP32), which the waves read
+// with 16-byte loads (column stride ldb = 8 mod 64 bf16: the eight lanes of a
+// load phase hit different 16-byte bank groups).
+// ---------------------------------------------------------------------------
+typedef __bf16 vh_bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned int vh_u4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t bf16_rne(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
+constexpr int gemm3_np(int ks) { return ks <= 5 ? 5 : ks <= 7 ? 4 : 3; }
+// launch shape: waves (row tiles) per block; one block per compute unit
+#define GEMM_WAVES(mt) ((mt) <= 9 ? (mt) : ((mt) + 1) / 2)
+
+template <int KS, int PF, int EPI>
+__global__ void __launch_bounds__(576)
+    k_rot_gemm_b3(const vh_u4 *__restrict__ A3, const float *__restrict__ src, float *__restrict__ out,
+                  int n, int m, int ldb, const float *__restrict__ aux, int nfull) {
+  extern __shared__ __align__(16) unsigned short Bs3[];   // NP buffers of (hi, lo) planes
+  constexpr int NC = 16;
+  constexpr int NP = gemm3_np(KS);
+  const int nthr = blockDim.x;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int lc = l & 15, lk = l >> 4;
+  const int mt = (n + 15) >> 4;
+  const int r = blockIdx.y * (nthr >> 6) + w;
+  const int npanel = (m + NC - 1) / NC;
+  const int xcd = blockIdx.x & 7;
+  const int chunk = (npanel + 7) / 8;
+  const int per = gridDim.x >> 3;
+  int p = xcd * chunk + (blockIdx.x >> 3);
+  const int pend = (xcd + 1) * chunk < npanel ? (xcd + 1) * chunk : npanel;
+  if (p >= pend) return;
+  // A operands of row tile r: step s of lane l is 32 bytes, hi then lo
+  vh_u4 ah[KS], al[KS];
+  const vh_u4 *ar = A3 + (size_t)(r < mt ? r : mt - 1) * KS * 128 + 2 * l;
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    ah[s] = ar[128 * s];
+    al[s] = ar[128 * s + 1];
+  }
+  const int tot2 = m * n / 2;
+  const int per2 = NC / 2 * n;
+  const int plane = NC * ldb;        // bf16 per plane
+  const int pbuf = 2 * plane;        // bf16 per buffer
+  const unsigned long long *src8 = (const unsigned long long *)src;
+  auto fetch = [&](float2 (&pv)[PF], int q) {
+    const int base2 = q * NC * n / 2;
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int g = base2 + threadIdx.x + u * nthr;
+      const unsigned long long v =
+          __hip_atomic_load(src8 + (g < tot2 ? g : tot2 - 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      pv[u] = make_float2(__uint_as_float((unsigned)v), __uint_as_float((unsigned)(v >> 32)));
+    }
+  };
+  int dst[PF];
+#pragma unroll
+  for (int u = 0; u < PF; ++u) {
+    const int e = threadIdx.x + u * nthr;
+    const int f = 2 * e, cc = f / n, k = f - cc * n;
+    dst[u] = e < per2 ? cc * ldb + k : -1;
+  }
+  // a pair (k, k+1) of a source column split into its hi and lo planes
+  auto stash = [&](const float2 (&pv)[PF], int q, unsigned short *B) {
+    const int base2 = q * NC * n / 2;
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int e = threadIdx.x + u * nthr;
+      const float2 v = base2 + e < tot2 ? pv[u] : make_float2(0.f, 0.f);
+      const uint32_t h0 = bf16_rne(v.x), h1 = bf16_rne(v.y);
+      const uint32_t l0 = bf16_rne(v.x - __uint_as_float(h0 << 16));
+      const uint32_t l1 = bf16_rne(v.y - __uint_as_float(h1 << 16));
+      uint32_t *dh = dst[u] >= 0 ? (uint32_t *)(B + dst[u]) : (uint32_t *)(Bs3 + NP * pbuf);
+      uint32_t *dl = dst[u] >= 0 ? (uint32_t *)(B + plane + dst[u]) : (uint32_t *)(Bs3 + NP * pbuf + 2);
+      *dh = h0 | (h1 << 16);
+      *dl = l0 | (l1 << 16);
+    }
+  };
+  const __amdgpu_buffer_rsrc_t ro =
+      vh_rsrc(out, 4 * (EPI == EPI_FILL_INTERIOR ? nfull * nfull * nfull : n * m));
+  auto emit = [&](int orow, int j, float val, float pv) {
+    const bool valid = orow < n && j < m;
+    if (EPI == EPI_STORE) {
+      vh_bstore_at(ro, valid, orow * m + j, val);
+    } else if (EPI == EPI_HADAMARD) {
+      vh_bstore_at(ro, valid, orow * m + j, val * pv);
+    } else if (EPI == EPI_ACCUMULATE) {
+      vh_bstore_at(ro, valid, orow * m + j, pv + val);
+    } else {
+      const int i = j % n, jj = j / n;
+      vh_bstore_at(ro, valid, (i + 1) + nfull * ((jj + 1) + nfull * (orow + 1)), val);
+    }
+  };
+  auto panel = [&](int q, const unsigned short *B, const float (&pre)[4]) {
+    vh_f4 acc = (vh_f4){0.f, 0.f, 0.f, 0.f};
+    const unsigned short *bh = B + lc * ldb + 8 * KS * lk;
+    const unsigned short *bl = bh + plane;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const vh_bf8 h = __builtin_bit_cast(vh_bf8, *(const vh_u4 *)(bh + 8 * s));
+      const vh_bf8 o = __builtin_bit_cast(vh_bf8, *(const vh_u4 *)(bl + 8 * s));
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(vh_bf8, al[s]), h, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(vh_bf8, ah[s]), o, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(vh_bf8, ah[s]), h, acc, 0, 0, 0);
+    }
+    const int j = q * NC + lc;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) emit(r * 16 + 4 * lk + v, j, acc[v], pre[v]);
+  };
+  // rows k in [n, ldb) of both planes of every buffer stay zero
+  for (int e = threadIdx.x; e < NP * 2 * NC * (ldb - n); e += nthr) {
+    const int row = e / (ldb - n), k = n + e % (ldb - n);
+    Bs3[row * ldb + k] = 0;
+  }
+  float2 R[NP][PF];
+  float pre[NP][4];
+  auto fetch_round = [&](int p0) {
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      const int q = p0 + i * per < pend ? p0 + i * per : p0;
+      fetch(R[i], q);
+      if (EPI == EPI_HADAMARD || EPI == EPI_ACCUMULATE) {
+        const float *srcp = EPI == EPI_HADAMARD ? aux : out;
+        const int last = n * m - 1;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int idx = (r * 16 + 4 * lk + v) * m + q * NC + lc;
+          pre[i][v] = srcp[idx < last ? idx : last];
+        }
+      }
+    }
+  };
+  auto run_round = [&](int p0) {
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      const int pi = p0 + i * per;
+      if (pi >= pend) break;
+      unsigned short *B = Bs3 + i * pbuf;
+      stash(R[i], pi, B);
+      if (EPI == EPI_HADAMARD || EPI == EPI_ACCUMULATE)
+        asm volatile("" ::"v"(pre[i][0]), "v"(pre[i][1]), "v"(pre[i][2]), "v"(pre[i][3]));
+      __syncthreads();
+      panel(pi, B, pre[i]);
+    }
+  };
+  fetch_round(p);
+  run_round(p);
+  for (p += NP * per; p < pend; p += NP * per) {
+    __syncthreads();
+    fetch_round(p);
+    run_round(p);
+  }
+}
+
+// steps of 32 K values: ceil(n / 32)
+static int gemm_ks(int n) { return (n + 31) / 32; }
+
+// A (n x n, column-major FP64) split into bf16 hi + lo in the operand order
+// of k_rot_gemm_b3: 32 bytes per (row tile r, step s, lane l), the eight hi
+// values of A[16 r + l % 16][8 KS (l / 16) + 8 s + j], j < 8, then the eight lo
+int vh_gemm_pack_b3(vh_context *c, uint32_t **dst, const double *colmajor, int n) {
+  const int mt = (n + 15) / 16, ks = gemm_ks(n);
+  VH_REQUIRE(n >= 2 && mt <= 17 && ks <= 9, "split-BF16 rotating product: n = %d beyond 272", n);
+  std::vector<uint32_t> op((size_t)mt * ks * 64 * 8);
+  auto rne = [](float x) -> uint32_t {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+  };
+  for (int r = 0; r < mt; ++r)
+    for (int s = 0; s < ks; ++s)
+      for (int l = 0; l < 64; ++l) {
+        uint32_t *o = &op[(((size_t)r * ks + s) * 64 + l) * 8];
+        for (int j = 0; j < 8; ++j) {
+          const int row = 16 * r + (l & 15), k = 8 * ks * (l >> 4) + 8 * s + j;
+          const double x = (row < n && k < n) ? colmajor[row + (size_t)n * k] : 0.0;
+          const uint32_t hi = rne((float)x);
+          float hf;
+          const uint32_t hb = hi << 16;
+          std::memcpy(&hf, &hb, 4);
+          const uint32_t lo = rne((float)(x - (double)hf));
+          o[j / 2] |= hi << (16 * (j & 1));
+          o[4 + j / 2] |= lo << (16 * (j & 1));
+        }
+      }
+  VH_TRY(vh_alloc((void **)dst, op.size() * sizeof(uint32_t)));
+  VH_CHECK_HIP(hipMemcpy(*dst, op.data(), op.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  return VH_OK;
+}
+
+static int vh_rotate_gemm_b3(vh_context *c, const uint32_t *A3, const float *src, float *out, int n,
+                             int64_t m, int epi, const float *aux, int nfull) {
+  const int mt = (n + 15) / 16, ks = gemm_ks(n);
+  VH_REQUIRE(mt >= 1 && mt <= 17 && ks <= 9, "split-BF16 rotating product: n = %d beyond 272", n);
+  VH_REQUIRE(m >= 1 && 4 * (int64_t)(nfull > 0 ? (int64_t)nfull * nfull * nfull : n * m) < VH_OOB,
+             "rotating product: n m = %lld beyond 32-bit byte offsets", (long long)((int64_t)n * m));
+  VH_REQUIRE(n % 2 == 0 && ((uintptr_t)src & 7) == 0,
+             "rotating product: n (%d) must be even and the source 8-byte aligned", n);
+  // smallest >= 32 ks with ldb = 8 (mod 64) (bf16 per column)
+  const int ldb = 32 * ks + ((8 - (32 * ks) % 64) + 64) % 64;
+  const int wv = GEMM_WAVES(mt);
+  constexpr int nc = 16;
+  const int pf = mt <= 9 ? 2 : 4;
+  VH_REQUIRE(nc / 2 * n <= pf * 64 * wv, "rotating product: panel of n = %d too large", n);
+  const int gy = (mt + wv - 1) / wv;
+  const int64_t npanel = (m + nc - 1) / nc;
+  int64_t gx = ((int64_t)c->num_cus + gy - 1) / gy;
+  if (gx > npanel) gx = npanel;
+  gx = (gx + 7) / 8 * 8;
+  dim3 grid((unsigned)gx, (unsigned)gy), block(64 * wv);
+  const size_t lds = 2 * ((size_t)gemm3_np(ks) * 2 * nc * ldb + 4);
+  VH_REQUIRE(lds <= 65536, "split-BF16 rotating product: %zu B of LDS for n = %d", lds, n);
+  const vh_u4 *A = (const vh_u4 *)A3;
+#define LAUNCH3(T, P, E)                                                                           \
+  hipLaunchKernelGGL((k_rot_gemm_b3<T, P, E>), grid, block, lds, c->stream, A, src, out, n, (int)m, \
+                     ldb, aux, nfull)
+#define EPIS3(T, P)                                                                                \
+  if (epi == EPI_STORE) LAUNCH3(T, P, EPI_STORE);                                                  \
+  else if (epi == EPI_HADAMARD) LAUNCH3(T, P, EPI_HADAMARD);                                       \
+  else if (epi == EPI_ACCUMULATE) LAUNCH3(T, P, EPI_ACCUMULATE);                                   \
+  else LAUNCH3(T, P, EPI_FILL_INTERIOR);
+  switch (ks) {
+    case 1: EPIS3(1, 2) break;
+    case 2: EPIS3(2, 2) break;
+    case 3: EPIS3(3, 2) break;
+    case 4: EPIS3(4, 2) break;
+    case 5:
+      if (pf == 2) { EPIS3(5, 2) } else { EPIS3(5, 4) }
+      break;
+    case 6: EPIS3(6, 4) break;
+    case 7: EPIS3(7, 4) break;
+    case 8: EPIS3(8, 4) break;
+    case 9: EPIS3(9, 4) break;
+  }
+#undef EPIS3
+#undef LAUNCH3
+  VH_LAUNCH_CHECK();
+  return VH_OK;
+}
+
 // K slice per lane group: the smallest odd bucket >= n/4 (0 beyond n = 276)
 static int gemm_kq(int n) {
   static const int buckets[] = {5, 9, 15, 17, 25, 33, 41, 49, 57, 65, 69};
@@ -236,8 +492,6 @@ int vh_gemm_pack(vh_context *c, float **dst, const double *colmajor, int n) {
   return VH_OK;
 }
 
-// launch shape: waves (row tiles) per block; one block per compute unit
-#define GEMM_WAVES(mt) ((mt) <= 9 ? (mt) : ((mt) + 1) / 2)
 
 int vh_rotate_gemm(vh_context *c, const float *A, const float *src, float *out, int n, int64_t m,
                    int epi, const float *aux, int nfull) {
@@ -291,13 +545,31 @@ int vh_rotate_gemm(vh_context *c, const float *A, const float *src, float *out, 
   return VH_OK;
 }
 
+// an operator of a mesh in both forms; b3 != null: the mesh runs its
+// products in split-BF16 x3
+struct vh_op {
+  const float *f32;
+  const uint32_t *b3;
+};
+// bit 0 of gemm_bf16x3: the solve's M^-1 and M products; bit 1: the S^-1 ones
+static vh_op mop(const vh_mesh *m, const float *f32, const uint32_t *b3) {
+  const int bit = f32 == m->Sinv_op ? 2 : 1;
+  return vh_op{f32, (m->gemm_bf16x3 & bit) ? b3 : nullptr};
+}
+
+static int rotate(vh_context *c, const vh_op &A, const float *src, float *out, int n, int64_t m,
+                  int epi, const float *aux, int nfull) {
+  if (A.b3) return vh_rotate_gemm_b3(c, A.b3, src, out, n, m, epi, aux, nfull);
+  return vh_rotate_gemm(c, A.f32, src, out, n, m, epi, aux, nfull);
+}
+
 // three rotations with A along each axis: src -> t1 -> t2 -> last(epi)
-static int apply_all(vh_context *c, const float *A, int n, const float *src, float *t1, float *t2,
+static int apply_all(vh_context *c, const vh_op &A, int n, const float *src, float *t1, float *t2,
                      float *dst, int epi_last, const float *aux, int nfull) {
   const int64_t m = (int64_t)n * n;
-  VH_TRY(vh_rotate_gemm(c, A, src, t1, n, m, EPI_STORE, nullptr, 0));
-  VH_TRY(vh_rotate_gemm(c, A, t1, t2, n, m, EPI_STORE, nullptr, 0));
-  VH_TRY(vh_rotate_gemm(c, A, t2, dst, n, m, epi_last, aux, nfull));
+  VH_TRY(rotate(c, A, src, t1, n, m, EPI_STORE, nullptr, 0));
+  VH_TRY(rotate(c, A, t1, t2, n, m, EPI_STORE, nullptr, 0));
+  VH_TRY(rotate(c, A, t2, dst, n, m, epi_last, aux, nfull));
   return VH_OK;
 }
 
@@ -710,10 +982,10 @@ static int solve_interior(vh_mesh *m) {
                      m->lap, m->n);
   VH_LAUNCH_CHECK();
   // forward transform, Hadamard fused into the third product
-  VH_TRY(apply_all(c, m->Minv_op, ni, m->inner, m->work1, m->work2, m->work1, EPI_HADAMARD,
+  VH_TRY(apply_all(c, mop(m, m->Minv_op, m->Minv_b3), ni, m->inner, m->work1, m->work2, m->work1, EPI_HADAMARD,
                    m->invlsum, 0));
   // inverse transform, the copy into phi's interior fused into the third product
-  VH_TRY(apply_all(c, m->M_op, ni, m->work1, m->inner, m->work2, m->phi, EPI_FILL_INTERIOR,
+  VH_TRY(apply_all(c, mop(m, m->M_op, m->M_b3), ni, m->work1, m->inner, m->work2, m->phi, EPI_FILL_INTERIOR,
                    nullptr, m->n));
   return VH_OK;
 }
@@ -730,7 +1002,7 @@ int vh_poisson_single_impl(vh_mesh *m) {
 
 int vh_spline_coefficients_impl(vh_mesh *m, const float *src, float *dst) {
   // src -> dst -> work -> dst, as apply_all_rotating! alternates (src/tensorsolver.jl:243-254)
-  VH_TRY(apply_all(m->ctx, m->Sinv_op, m->n, src, dst, m->work, dst, EPI_STORE, nullptr, 0));
+  VH_TRY(apply_all(m->ctx, mop(m, m->Sinv_op, m->Sinv_b3), m->n, src, dst, m->work, dst, EPI_STORE, nullptr, 0));
   return VH_OK;
 }
 
@@ -792,7 +1064,7 @@ int vh_effective_potential_impl(vh_mesh *m, double nions, double radius) {
                      m->n, (float)nions, (float)radius, (const float *)nullptr);
   VH_LAUNCH_CHECK();
   // extra -> work -> extra -> csol (+=)
-  VH_TRY(apply_all(c, m->Sinv_op, m->n, m->extra, m->work, m->extra, m->csol, EPI_ACCUMULATE,
+  VH_TRY(apply_all(c, mop(m, m->Sinv_op, m->Sinv_b3), m->n, m->extra, m->work, m->extra, m->csol, EPI_ACCUMULATE,
                    nullptr, 0));
   return VH_OK;
 }
@@ -807,13 +1079,13 @@ int vh_mean_field_coefficients_impl(vh_mesh *m, double nions, double radius) {
                      m->n, (float)nions, (float)radius, (const float *)nullptr);
   VH_LAUNCH_CHECK();
   // extra -> work -> extra -> csol
-  VH_TRY(apply_all(c, m->Sinv_op, m->n, m->extra, m->work, m->extra, m->csol, EPI_STORE, nullptr, 0));
+  VH_TRY(apply_all(c, mop(m, m->Sinv_op, m->Sinv_b3), m->n, m->extra, m->work, m->extra, m->csol, EPI_STORE, nullptr, 0));
   return VH_OK;
 }
 
 int vh_hartree_coefficients_add_impl(vh_mesh *m) {
   // phi -> extra -> work -> csol (+=)
-  VH_TRY(apply_all(m->ctx, m->Sinv_op, m->n, m->phi, m->extra, m->work, m->csol, EPI_ACCUMULATE,
+  VH_TRY(apply_all(m->ctx, mop(m, m->Sinv_op, m->Sinv_b3), m->n, m->phi, m->extra, m->work, m->csol, EPI_ACCUMULATE,
                    nullptr, 0));
   return VH_OK;
 }

@@ -65,17 +65,49 @@ __global__ void k_scan_reduce(const uint32_t *__restrict__ data, int64_t n, uint
   if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
-// Each block adds the totals of the blocks before it itself (at most a few
-// thousand, read once per block), so no pass scans the block totals; the
-// last block writes the array's total at sums[nb].
-__global__ void k_scan_down(uint32_t *data, int64_t n, uint32_t *sums, int64_t nb) {
+// Up to SCAN_FEW blocks, each block adds the totals of the blocks before it
+// itself (read once per block), so no pass scans the block totals; beyond,
+// k_scan_mid first turns the totals into their exclusive prefix (prefixed),
+// so that a block's carry is one load rather than O(nb).  The last block
+// writes the array's total at sums[nb].
+#define SCAN_FEW 8192
+#define SCAN_MID_ITEMS 16
+__global__ void __launch_bounds__(1024) k_scan_mid(uint32_t *sums, int64_t nb) {
+  __shared__ uint32_t lds[16];
+  sums += blockIdx.y * (nb + 1);
+  uint32_t carry = 0;
+  for (int64_t base = 0; base < nb; base += (int64_t)blockDim.x * SCAN_MID_ITEMS) {
+    const int64_t i0 = base + (int64_t)threadIdx.x * SCAN_MID_ITEMS;
+    uint32_t v[SCAN_MID_ITEMS], s = 0;
+    for (int it = 0; it < SCAN_MID_ITEMS; ++it) {
+      v[it] = i0 + it < nb ? sums[i0 + it] : 0;
+      s += v[it];
+    }
+    uint32_t ex;
+    const uint32_t tot = block_excl_scan_u32(s, ex, lds);
+    uint32_t run = carry + ex;
+    for (int it = 0; it < SCAN_MID_ITEMS; ++it) {
+      if (i0 + it < nb) sums[i0 + it] = run;
+      run += v[it];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0) sums[nb] = carry;
+}
+
+__global__ void k_scan_down(uint32_t *data, int64_t n, uint32_t *sums, int64_t nb, int prefixed) {
   __shared__ uint32_t lds[16];
   data += blockIdx.y * n;
   sums += blockIdx.y * (nb + 1);
-  uint32_t before = 0;
-  for (int64_t i = threadIdx.x; i < (int64_t)blockIdx.x; i += blockDim.x) before += sums[i];
-  uint32_t unused;
-  const uint32_t carry = block_excl_scan_u32(before, unused, lds);
+  uint32_t carry;
+  if (prefixed) {
+    carry = sums[blockIdx.x];
+  } else {
+    uint32_t before = 0;
+    for (int64_t i = threadIdx.x; i < (int64_t)blockIdx.x; i += blockDim.x) before += sums[i];
+    uint32_t unused;
+    carry = block_excl_scan_u32(before, unused, lds);
+  }
   int64_t base = (int64_t)blockIdx.x * SCAN_THREADS * SCAN_ITEMS;
   // each thread owns SCAN_ITEMS consecutive elements
   int64_t i0 = base + (int64_t)threadIdx.x * SCAN_ITEMS;
@@ -92,7 +124,7 @@ __global__ void k_scan_down(uint32_t *data, int64_t n, uint32_t *sums, int64_t n
     if (i0 + it < n) data[i0 + it] = run;
     run += v[it];
   }
-  if (blockIdx.x == nb - 1 && threadIdx.x == 0) sums[nb] = carry + tot;
+  if (!prefixed && blockIdx.x == nb - 1 && threadIdx.x == 0) sums[nb] = carry + tot;
 }
 
 // exclusive scans of `count` consecutive arrays of n entries each, in place
@@ -105,8 +137,13 @@ static int scan_arrays(vh_context *c, uint32_t *data, int64_t n, int count, uint
   hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb, count), dim3(SCAN_THREADS), 0, c->stream,
                      data, n, tmp, nb);
   VH_LAUNCH_CHECK();
+  const int prefixed = nb > SCAN_FEW;
+  if (prefixed) {
+    hipLaunchKernelGGL(k_scan_mid, dim3(1, count), dim3(1024), 0, c->stream, tmp, nb);
+    VH_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb, count), dim3(SCAN_THREADS), 0, c->stream,
-                     data, n, tmp, nb);
+                     data, n, tmp, nb, prefixed);
   VH_LAUNCH_CHECK();
   return VH_OK;
 }

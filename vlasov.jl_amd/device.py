@@ -84,6 +84,17 @@ class DeviceMesh:
             raise ValueError(f"grid must hold {self.n}^3 values, got {a.size}")
         _lib.call("vh_mesh_grid_upload", self.handle, int(which), a)
 
+    def set_bf16x3(self, solve: bool = True, transforms: bool | None = None):
+        """Rotating products in split-BF16 x3 on the matrix cores (FP32
+        accumulation) instead of FP32 (VH_MESH_OPT_BF16X3): the Poisson
+        solve's M^-1 and M contractions with `solve`, the S^-1 transforms
+        (spline coefficients, mean field) with `transforms` (default: as
+        `solve`)."""
+        if transforms is None:
+            transforms = solve
+        _lib.call("vh_mesh_set_option", self.handle, _lib.VH_MESH_OPT_BF16X3,
+                  (1 if solve else 0) | (2 if transforms else 0))
+
     def copy_grid(self, dst: int, src: int):
         """copyto!(dst grid, src grid) on the device (src/simulation.jl:527-530)."""
         _lib.call("vh_mesh_grid_copy", self.handle, int(dst), int(src))

@@ -77,7 +77,8 @@ class Simulation:
     def __init__(self, params: SimulationParameters, profile, *, projectile: Projectile | None = None,
                  seed: int = -1, softening_mode: str = "device", consistent_startup: bool = False,
                  device: int = 0, tables=None, positions=None, momenta=None,
-                 resident_projectile: bool = True, slice_size: int = 1 << 20):
+                 resident_projectile: bool = True, slice_size: int = 1 << 20,
+                 poisson_precision: str = "fp32"):
         self.params = params
         self._projectile = projectile
         self._proj_on_device = False
@@ -97,6 +98,13 @@ class Simulation:
         self.ctx = Context(device)
         self.dmf = DeviceMesh(self.ctx, fine)
         self.dmc = DeviceMesh(self.ctx, coarse)
+        # the precision extension: the Poisson solve's contractions in
+        # split-BF16 x3 on the matrix cores (the S^-1 transforms stay FP32)
+        if poisson_precision not in ("fp32", "bf16x3"):
+            raise ValueError(f"unknown Poisson precision {poisson_precision!r}")
+        if poisson_precision == "bf16x3":
+            for d in (self.dmf, self.dmc):
+                d.set_bf16x3(solve=True, transforms=False)
         self.acc = DeviceAccelerator(self.ctx, fine.axis, sm, params.nparticles)
         self.x0, self.h = self.acc.x0, self.acc.h
         self.nout = 0
