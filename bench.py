@@ -213,6 +213,76 @@ def precision_extension(va, npart: int, nfine: int, mode: str, warmup: int, step
     return out
 
 
+def batching_extension(va, count: int, steps: int, nfine: int = 64):
+    """Config 4: `count` independent collisions of the family (three
+    projectiles, impact parameters uniform in 0-60 a0, particle counts
+    log-uniform in 1e6-8e6) on the nfine grid.  Aggregate particle-steps/s
+    of `steps` steps: every collision run alone one after the other (each
+    step synchronised, the plain runs), then all of them as one batch whose
+    step is one replayed HIP graph, after the autotuner has picked each
+    member's work-unit size (and, between the two, the batch without the
+    graph: every member's launches on its own streams, interleaved)."""
+    fam = va.batch.collision_family(count=count, nfine=nfine)
+    t0 = time.perf_counter()
+    sims = [va.Simulation(c.params, va.read_radial_profile(), projectile=c.projectile) for c in fam]
+    log(f"batch of {count} collisions built in {time.perf_counter() - t0:.1f} s")
+    total = sum(c.params.nparticles for c in fam)
+    out = {"collisions": count, "particles": total,
+           "members": [{"projectile": c.label, "particles": c.params.nparticles,
+                        "impact": round(c.impact, 2)} for c in fam]}
+    try:
+        for s in sims:      # one warm step each
+            s.step()
+            s.ctx.sync()
+        t_seq = 0.0
+        for s in sims:
+            s.ctx.sync()
+            t1 = time.perf_counter()
+            for _ in range(steps):
+                s.step()
+            s.ctx.sync()
+            t_seq += time.perf_counter() - t1
+        out["sequential"] = {"particle_steps_per_s": total * steps / t_seq,
+                             "ms_per_batch_step": 1000.0 * t_seq / steps}
+        eager = va.batch.Batch(sims, graph=False)
+        try:
+            eager.step()
+            eager.sync()
+            t1 = time.perf_counter()
+            for _ in range(steps):
+                eager.step()
+            eager.sync()
+            t_e = time.perf_counter() - t1
+        finally:
+            eager.close()
+        out["batched_streams"] = {"particle_steps_per_s": total * steps / t_e,
+                                  "ms_per_batch_step": 1000.0 * t_e / steps}
+        batch = va.batch.Batch(sims, graph=True)
+        try:
+            t1 = time.perf_counter()
+            units, ms = batch.autotune(steps=4)
+            out["autotune"] = {"candidates": [128, 256, 512, 1024], "units": units,
+                               "s": time.perf_counter() - t1}
+            batch.step()          # recorded
+            batch.step()          # the other parity
+            batch.sync()
+            t1 = time.perf_counter()
+            for _ in range(steps):
+                batch.step()
+            batch.sync()
+            t_b = time.perf_counter() - t1
+        finally:
+            batch.close()
+        out["batched_graph"] = {"particle_steps_per_s": total * steps / t_b,
+                                "ms_per_batch_step": 1000.0 * t_b / steps}
+        out["speedup"] = t_seq / t_b
+        out["steps"] = steps
+    finally:
+        for s in sims:
+            s.close()
+    return out
+
+
 def isolated_drift(va, npart: int, nfine: int, steps: int, every: int):
     """Energy conservation of the isolated Na196 cluster, the reference's
     validation observable (src/energy.jl:1-7): the budget every `every` steps
@@ -255,6 +325,9 @@ def main():
     ap.add_argument("--precision-particles", type=int, default=80_000_000,
                     help="particles of the precision extension's crossings on nfine=128 "
                          "(FP32 and split-BF16 x3 Poisson; 0: skip)")
+    ap.add_argument("--batch-collisions", type=int, default=16,
+                    help="collisions of the batching extension's run (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=30)
     ap.add_argument("--cpu-particles", type=int, default=100_000)
     ap.add_argument("--cpu-nfine", type=int, default=28)
     ap.add_argument("--cpu-steps", type=int, default=150)
@@ -341,6 +414,9 @@ def main():
     if args.precision_particles > 0:
         line["precision_extension"] = precision_extension(va, args.precision_particles, 128,
                                                           args.mode, args.warmup, args.steps)
+    if args.batch_collisions > 0:
+        log(f"batching extension: {args.batch_collisions} collisions...")
+        line["batching_extension"] = batching_extension(va, args.batch_collisions, args.batch_steps)
     if not args.no_cpu_baseline:
         log("cpu baseline...")
         line["cpu_baseline"] = cpu_baseline(va, args.cpu_nfine, args.cpu_particles, args.cpu_steps)

@@ -175,8 +175,15 @@ int vh_accel_destroy(vh_accel *a);
  *                       one; 1: vh_step records them once into a HIP graph
  *                       per particle buffer and replays it (recorded again
  *                       when an argument changes; eager while stage timing
- *                       is on). */
-enum { VH_OPT_EXACT_FIELD = 0, VH_OPT_STEP_GRAPH = 1 };
+ *                       is on).
+ *   VH_OPT_UNIT_SLOTS   the slots of a work unit, the tile of particles one
+ *                       wave of the per-cell kernels takes (a power of two in
+ *                       [128, 4096], default 256; a cell is cut into units
+ *                       of at most this many slots at the next sort).  The
+ *                       sort and the field are independent of it to the bit;
+ *                       the deposits' FP32 partial sums group by it.
+ *                       vh_autotune_unit picks it. */
+enum { VH_OPT_EXACT_FIELD = 0, VH_OPT_STEP_GRAPH = 1, VH_OPT_UNIT_SLOTS = 2 };
 int vh_accel_set_option(vh_accel *a, int option, int value);
 
 /* upload!/download! of the particle buffer (src/accelerator.jl:48, 64-68):
@@ -311,6 +318,31 @@ int vh_projectile_step(vh_accel *a, double dt);
  * from the host per step. */
 int vh_step(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, double dt, double mass,
             double rcmax, double jel_nions, double jel_radius);
+
+/* --- the batching extension (no counterpart in the reference) -------------
+ * vh_autotune_unit: `steps` real steps of vh_step with each candidate work-
+ * unit size (VH_OPT_UNIT_SLOTS; one untimed step first, whose sort cuts the
+ * cells with it), timed with HIP events on the accelerator's stream; keeps
+ * the fastest, writes it to *chosen and each candidate's ms per step to ms
+ * (may be NULL). */
+int vh_autotune_unit(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, double dt,
+                     double mass, double rcmax, double jel_nions, double jel_radius,
+                     const int *candidates, int ncand, int steps, int *chosen, double *ms);
+/* A batch of independent simulations on one device, one accelerator (with
+ * its own context) and two meshes each; params = count x {weight, dt, mass,
+ * rcmax, jel_nions, jel_radius}, the arguments of each member's vh_step.
+ * vh_batch_step makes one vh_step of every member: with use_graph, the
+ * members' launches are recorded once (per pattern of their host state: the
+ * steps alternate two) into one HIP graph that forks from the batch's stream
+ * into every member's streams and joins back, and replayed; every member's
+ * later calls follow the batch's step.  Without, each member's vh_step in
+ * turn, on its own streams. */
+typedef struct vh_batch vh_batch;
+int vh_batch_create(int count, vh_accel *const *accs, vh_mesh *const *fines,
+                    vh_mesh *const *coarses, const double *params, int use_graph, vh_batch **out);
+int vh_batch_step(vh_batch *b);
+int vh_batch_sync(vh_batch *b);
+int vh_batch_destroy(vh_batch *b);
 
 /* interaction_energy(cloud, fine, csol_f, coarse, csol_c, sm; enclosed)
  * (src/energy.jl:19-43): sum over the particles of weight * phi(r), in FP64,

@@ -17,6 +17,7 @@ VH_PROF_ALL = -1
 VH_SOFT_DEVICE, VH_SOFT_BALL = 0, 1
 VH_OPT_EXACT_FIELD = 0
 VH_OPT_STEP_GRAPH = 1
+VH_OPT_UNIT_SLOTS = 2
 VH_MESH_OPT_BF16X3 = 0
 STAGES = ("sort", "deposit_fine", "deposit_cic", "poisson", "spline", "meanfield", "field",
           "verlet", "field_kernel", "deposit_kernel")
@@ -85,6 +86,12 @@ _SIGNATURES = {
     "vh_potential_coefficients": [_vp, C.c_double, C.c_double],
     "vh_step": [_vp, _vp, _vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                 C.c_double],
+    "vh_autotune_unit": [_vp, _vp, _vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                         C.c_double, _i32p, C.c_int, C.c_int, C.POINTER(C.c_int), _f64p],
+    "vh_batch_create": [C.c_int, _vp, _vp, _vp, _f64p, C.c_int, C.POINTER(_vp)],
+    "vh_batch_step": [_vp],
+    "vh_batch_sync": [_vp],
+    "vh_batch_destroy": [_vp],
     "vh_interaction_energy": [_vp, _vp, _vp, C.c_double, C.c_double, C.POINTER(C.c_double)],
     "vh_profile_enable": [_vp, C.c_int],
     "vh_profile_read": [_vp, _f64p, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")],

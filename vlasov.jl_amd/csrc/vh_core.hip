@@ -448,7 +448,8 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
   MK(vh_alloc((void **)&a->scan_tmp, (size_t)a->scan_tmp_len * 4));
   int64_t maxocc = a->ncell < npart ? a->ncell : npart;
   a->maxocc = maxocc;
-  a->maxunits = maxocc + (npart + VH_UNIT - 1) / VH_UNIT;
+  a->maxunits = maxocc + (npart + VH_UNIT_MIN - 1) / VH_UNIT_MIN;
+  a->unit = VH_UNIT;
   MK(vh_alloc((void **)&a->cellscratch, sizeof(uint32_t) * 5 * (a->ncell + 1)));
   MK(vh_alloc((void **)&a->units, sizeof(vh_unit) * (size_t)a->maxunits));
   MK(vh_alloc((void **)&a->unitstart, sizeof(int32_t) * (maxocc + 2)));
@@ -494,6 +495,16 @@ extern "C" int vh_accel_set_option(vh_accel *a, int option, int value) {
     case VH_OPT_STEP_GRAPH:
       VH_REQUIRE(value == 0 || value == 1, "vh_accel_set_option: step graph takes 0 or 1");
       a->step_graph = value;
+      return VH_OK;
+    case VH_OPT_UNIT_SLOTS:
+      VH_REQUIRE(value >= VH_UNIT_MIN && value <= 4096 && (value & (value - 1)) == 0,
+                 "vh_accel_set_option: unit slots %d not a power of two in [%d, 4096]", value,
+                 VH_UNIT_MIN);
+      if (value != a->unit) {
+        a->unit = value;   // the next sort cuts the cells with it
+        drop_graphs(a);    // the recorded steps carry the old one
+        ++a->config_version;
+      }
       return VH_OK;
     default:
       VH_REQUIRE(false, "vh_accel_set_option: unknown option %d", option);
@@ -1098,6 +1109,233 @@ extern "C" int vh_step(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weigh
   }
   VH_CHECK_HIP(hipGraphLaunch(a->graph[b], st));
   set_host_state(a, a->graph_after[b]);  // what the recorded step did on the host
+  return VH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// autotuning of the work-unit size: real steps of the simulation, timed with
+// events on the accelerator's stream
+// ---------------------------------------------------------------------------
+extern "C" int vh_autotune_unit(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight,
+                                double dt, double mass, double rcmax, double jel_nions,
+                                double jel_radius, const int *candidates, int ncand, int steps,
+                                int *chosen, double *ms) {
+  VH_REQUIRE(a && fine && coarse && candidates && chosen, "vh_autotune_unit: null argument");
+  VH_REQUIRE(ncand >= 1 && steps >= 1, "vh_autotune_unit: %d candidates, %d steps", ncand, steps);
+  for (int i = 0; i < ncand; ++i)
+    VH_REQUIRE(candidates[i] >= VH_UNIT_MIN && candidates[i] <= 4096 &&
+                   (candidates[i] & (candidates[i] - 1)) == 0,
+               "vh_autotune_unit: candidate %d not a power of two in [%d, 4096]", candidates[i],
+               VH_UNIT_MIN);
+  VH_CHECK_HIP(hipSetDevice(a->ctx->device));
+  hipEvent_t e0, e1;
+  VH_CHECK_HIP(hipEventCreate(&e0));
+  VH_CHECK_HIP(hipEventCreate(&e1));
+  int rc = VH_OK, best = -1;
+  float best_ms = 0.f;
+  for (int i = 0; i < ncand && rc == VH_OK; ++i) {
+    rc = vh_accel_set_option(a, VH_OPT_UNIT_SLOTS, candidates[i]);
+    // one untimed step: its sort cuts the cells with the candidate (and
+    // records the step's graph when the accelerator replays graphs)
+    if (rc == VH_OK)
+      rc = vh_step(a, fine, coarse, weight, dt, mass, rcmax, jel_nions, jel_radius);
+    if (rc != VH_OK) break;
+    hipEventRecord(e0, a->ctx->stream);
+    for (int k = 0; k < steps && rc == VH_OK; ++k)
+      rc = vh_step(a, fine, coarse, weight, dt, mass, rcmax, jel_nions, jel_radius);
+    hipEventRecord(e1, a->ctx->stream);
+    float t = 0.f;
+    if (rc == VH_OK && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&t, e0, e1) == hipSuccess) {
+      t /= steps;
+      if (ms) ms[i] = t;
+      if (best < 0 || t < best_ms) {
+        best = i;
+        best_ms = t;
+      }
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc != VH_OK) return rc;
+  VH_REQUIRE(best >= 0, "vh_autotune_unit: no candidate could be timed");
+  *chosen = candidates[best];
+  return vh_accel_set_option(a, VH_OPT_UNIT_SLOTS, candidates[best]);
+}
+
+// ---------------------------------------------------------------------------
+// batches: independent simulations stepped as one HIP graph
+// ---------------------------------------------------------------------------
+struct vh_batch_entry {
+  std::vector<int> key;
+  hipGraphExec_t exec = nullptr;
+  std::vector<vh_step_state> after;
+};
+
+struct vh_batch {
+  int device = 0;
+  int use_graph = 1;
+  std::vector<vh_accel *> accs;
+  std::vector<vh_mesh *> fines, coarses;
+  std::vector<double> params;          // 6 per member: weight dt mass rcmax nions radius
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr, done = nullptr;
+  std::vector<hipEvent_t> join, ready;
+  std::vector<vh_batch_entry> cache;   // at most VH_BATCH_GRAPHS recorded steps
+};
+#define VH_BATCH_GRAPHS 4
+
+static void batch_free(vh_batch *b) {
+  if (!b) return;
+  hipSetDevice(b->device);
+  if (b->stream) hipStreamSynchronize(b->stream);
+  for (auto &e : b->cache)
+    if (e.exec) hipGraphExecDestroy(e.exec);
+  for (hipEvent_t e : b->join) hipEventDestroy(e);
+  for (hipEvent_t e : b->ready) hipEventDestroy(e);
+  if (b->fork) hipEventDestroy(b->fork);
+  if (b->done) hipEventDestroy(b->done);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+}
+
+extern "C" int vh_batch_create(int count, vh_accel *const *accs, vh_mesh *const *fines,
+                               vh_mesh *const *coarses, const double *params, int use_graph,
+                               vh_batch **out) {
+  VH_REQUIRE(count >= 1 && accs && fines && coarses && params && out,
+             "vh_batch_create: null argument or empty batch");
+  *out = nullptr;
+  const int dev = accs[0]->ctx->device;
+  for (int i = 0; i < count; ++i) {
+    VH_REQUIRE(accs[i] && fines[i] && coarses[i], "vh_batch_create: null member %d", i);
+    VH_REQUIRE(accs[i]->ctx->device == dev, "vh_batch_create: members on different devices");
+    VH_REQUIRE(fines[i]->ctx == accs[i]->ctx && coarses[i]->ctx == accs[i]->ctx &&
+                   fines[i]->n == accs[i]->n,
+               "vh_batch_create: member %d's meshes do not belong to its accelerator", i);
+    for (int j = 0; j < i; ++j)
+      VH_REQUIRE(accs[j]->ctx != accs[i]->ctx,
+                 "vh_batch_create: members %d and %d share a context (one stream each)", j, i);
+    const double *p = params + 6 * i;
+    VH_REQUIRE(std::isfinite(p[0]) && p[1] > 0 && p[2] > 0 && p[5] > 0,
+               "vh_batch_create: member %d: weight finite, dt, mass and jellium radius positive", i);
+  }
+  VH_CHECK_HIP(hipSetDevice(dev));
+  vh_batch *b = new vh_batch();
+  b->device = dev;
+  b->use_graph = use_graph ? 1 : 0;
+  b->accs.assign(accs, accs + count);
+  b->fines.assign(fines, fines + count);
+  b->coarses.assign(coarses, coarses + count);
+  b->params.assign(params, params + 6 * (size_t)count);
+  bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&b->done, hipEventDisableTiming) == hipSuccess;
+  for (int i = 0; i < count && ok; ++i) {
+    hipEvent_t j = nullptr, r = nullptr;
+    ok = hipEventCreateWithFlags(&j, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&r, hipEventDisableTiming) == hipSuccess;
+    b->join.push_back(j);
+    b->ready.push_back(r);
+  }
+  if (!ok) {
+    batch_free(b);
+    vh_set_error("vh_batch_create: could not create the batch's stream and events");
+    return VH_EHIP;
+  }
+  *out = b;
+  return VH_OK;
+}
+
+extern "C" int vh_batch_destroy(vh_batch *b) {
+  batch_free(b);
+  return VH_OK;
+}
+
+static int batch_member_step(vh_batch *b, size_t i) {
+  const double *p = &b->params[6 * i];
+  return step_eager(b->accs[i], b->fines[i], b->coarses[i], p[0], p[1], p[2], p[3], p[4], p[5]);
+}
+
+extern "C" int vh_batch_step(vh_batch *b) {
+  VH_REQUIRE(b != nullptr, "vh_batch_step: null batch");
+  VH_CHECK_HIP(hipSetDevice(b->device));
+  const size_t n = b->accs.size();
+  bool profiling = false;
+  for (vh_accel *a : b->accs) profiling = profiling || a->ctx->prof_on;
+  if (!b->use_graph || profiling) {
+    // every member on its own streams: their launches interleave on the device
+    for (size_t i = 0; i < n; ++i) VH_TRY(batch_member_step(b, i));
+    return VH_OK;
+  }
+  // the recorded steps depend on each member's host state (current buffer,
+  // fresh keys, records in place) and options
+  std::vector<int> key;
+  key.reserve(8 * n);
+  for (vh_accel *a : b->accs) {
+    key.push_back(a->cur);
+    key.push_back(a->keys_fresh + 2 * a->hist_fresh + 4 * a->rec_pending + 8 * a->pushed_fused);
+    key.push_back(a->config_version);
+    key.push_back(a->has_dproj + 2 * a->exact_field);
+    key.push_back(a->nout_idx);
+  }
+  // the batch stream follows each member's earlier work
+  for (size_t i = 0; i < n; ++i) {
+    VH_CHECK_HIP(hipEventRecord(b->ready[i], b->accs[i]->ctx->stream));
+    VH_CHECK_HIP(hipStreamWaitEvent(b->stream, b->ready[i], 0));
+  }
+  vh_batch_entry *e = nullptr;
+  for (auto &c : b->cache)
+    if (c.key == key) e = &c;
+  if (!e) {
+    if (b->cache.size() >= VH_BATCH_GRAPHS) {
+      hipStreamSynchronize(b->stream);
+      hipGraphExecDestroy(b->cache.front().exec);
+      b->cache.erase(b->cache.begin());
+    }
+    // record: fork from the batch stream into every member's streams, each
+    // member's step, and join back
+    VH_CHECK_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = hipEventRecord(b->fork, b->stream) == hipSuccess ? VH_OK : VH_EHIP;
+    for (size_t i = 0; i < n && rc == VH_OK; ++i) {
+      hipStream_t ms = b->accs[i]->ctx->stream;
+      if (hipStreamWaitEvent(ms, b->fork, 0) != hipSuccess) { rc = VH_EHIP; break; }
+      rc = batch_member_step(b, i);
+      if (rc == VH_OK && (hipEventRecord(b->join[i], ms) != hipSuccess ||
+                          hipStreamWaitEvent(b->stream, b->join[i], 0) != hipSuccess))
+        rc = VH_EHIP;
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+    if (rc != VH_OK || ec != hipSuccess) {
+      if (g) hipGraphDestroy(g);
+      if (rc == VH_OK) vh_set_error("vh_batch_step: graph capture: %s", hipGetErrorString(ec));
+      return rc != VH_OK ? rc : VH_EHIP;
+    }
+    vh_batch_entry ne;
+    ne.key = key;
+    const hipError_t ei = hipGraphInstantiate(&ne.exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      vh_set_error("vh_batch_step: graph instantiation: %s", hipGetErrorString(ei));
+      return VH_EHIP;
+    }
+    for (vh_accel *a : b->accs) ne.after.push_back(host_state(a));
+    b->cache.push_back(ne);
+    e = &b->cache.back();
+  } else {
+    for (size_t i = 0; i < n; ++i) set_host_state(b->accs[i], e->after[i]);
+  }
+  VH_CHECK_HIP(hipGraphLaunch(e->exec, b->stream));
+  // every member's later work follows the batch's step
+  VH_CHECK_HIP(hipEventRecord(b->done, b->stream));
+  for (vh_accel *a : b->accs) VH_CHECK_HIP(hipStreamWaitEvent(a->ctx->stream, b->done, 0));
+  return VH_OK;
+}
+
+extern "C" int vh_batch_sync(vh_batch *b) {
+  VH_REQUIRE(b != nullptr, "vh_batch_sync: null batch");
+  VH_CHECK_HIP(hipSetDevice(b->device));
+  VH_CHECK_HIP(hipStreamSynchronize(b->stream));
+  for (vh_accel *a : b->accs) VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
   return VH_OK;
 }
 

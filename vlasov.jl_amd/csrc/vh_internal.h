@@ -192,7 +192,8 @@ enum {
 // Work units of the per-cell kernels: an occupied cell's particles in runs of
 // at most VH_UNIT slots.  A cell of the cluster's core holds thousands of
 // particles; as units they spread over many waves instead of one.
-#define VH_UNIT 256
+#define VH_UNIT 256        // default slots per work unit (VH_OPT_UNIT_SLOTS)
+#define VH_UNIT_MIN 128    // the smallest allowed: the unit buffers are sized for it
 struct vh_unit {
   int32_t key;   // 0-based cell key
   int32_t lo;    // first slot
@@ -237,7 +238,9 @@ struct vh_accel {
   int32_t *bounds;             // n_occ + 1
   vh_unit *units;              // CNT_NUNITS work units, in slot order
   int32_t *unitstart;          // n_occ + 1: first unit of each occupied cell
-  int64_t maxunits;            // maxocc + ceil(npart / VH_UNIT)
+  int64_t maxunits;            // maxocc + ceil(npart / VH_UNIT_MIN)
+  int unit;                    // slots per work unit (VH_OPT_UNIT_SLOTS)
+  int config_version;          // bumped when an option changes what a step launches
   int32_t *cellmap;            // ncell -> fitting occupied index or -1
   int32_t *fitlist;            // occupied index of fitting cells
   int32_t *nonfit;             // occupied index of non-fitting cells

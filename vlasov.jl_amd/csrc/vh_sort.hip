@@ -357,10 +357,10 @@ __global__ void k_run_mark(const uint32_t *__restrict__ keys, int64_t n, int32_t
 // Five flag arrays over ncell + 1 entries (the last zero, so that the
 // exclusive scans end with the totals): occupied, fitting, non-fitting,
 // particles of a non-fitting cell (the out-of-stencil list), work units of
-// at most VH_UNIT slots.
+// at most `unit` slots (VH_OPT_UNIT_SLOTS).
 __global__ void k_cell_classify(const int32_t *__restrict__ run, int ncell, int nk, int n,
                                 uint32_t *occidx, uint32_t *fitidx, uint32_t *nfidx, uint32_t *nfpre,
-                                uint32_t *nunit) {
+                                uint32_t *nunit, int unit) {
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= ncell; c += gridDim.x * blockDim.x) {
     const int first = c < ncell ? run[2 * (int64_t)c] : -1;
     if (first < 0) {
@@ -375,7 +375,7 @@ __global__ void k_cell_classify(const int32_t *__restrict__ run, int ncell, int 
     fitidx[c] = ok ? 1u : 0u;
     nfidx[c] = ok ? 0u : 1u;
     nfpre[c] = ok ? 0u : count;
-    nunit[c] = (count + VH_UNIT - 1) / VH_UNIT;
+    nunit[c] = (count + unit - 1) / unit;
   }
 }
 
@@ -386,7 +386,7 @@ __global__ void k_cell_lists(const uint32_t *__restrict__ occidx, const uint32_t
                              const uint32_t *__restrict__ ustart, int32_t *run, int ncell, int64_t n,
                              int32_t *occupied, int32_t *bounds, int32_t *cellmap, int32_t *fitlist,
                              int32_t *nonfit, int32_t *nonfit_prefix, vh_unit *units,
-                             int32_t *unitstart, int32_t *cnt) {
+                             int32_t *unitstart, int32_t *cnt, int unit) {
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= ncell; c += gridDim.x * blockDim.x) {
     if (c == ncell) {
       const uint32_t nocc = occidx[c];
@@ -417,11 +417,11 @@ __global__ void k_cell_lists(const uint32_t *__restrict__ occidx, const uint32_t
     }
     uint32_t u = ustart[c];
     unitstart[g] = (int32_t)u;
-    for (int32_t s = lo; s < hi; s += VH_UNIT, ++u) {
+    for (int32_t s = lo; s < hi; s += unit, ++u) {
       vh_unit un;
       un.key = c;
       un.lo = s;
-      un.hi = hi - s > VH_UNIT ? s + VH_UNIT : hi;
+      un.hi = hi - s > unit ? s + unit : hi;
       un.g = (int32_t)g;
       units[u] = un;
     }
@@ -545,12 +545,12 @@ int vh_sort_impl(vh_accel *a) {
   uint32_t *occidx = a->cellscratch, *fitidx = occidx + len, *nfidx = fitidx + len,
            *nfpre = nfidx + len, *nunit = nfpre + len;
   hipLaunchKernelGGL(k_cell_classify, dim3(1024), dim3(256), 0, st, a->cellrun, (int)a->ncell, a->nk,
-                     a->n, occidx, fitidx, nfidx, nfpre, nunit);
+                     a->n, occidx, fitidx, nfidx, nfpre, nunit, a->unit);
   VH_LAUNCH_CHECK();
   VH_TRY(scan_arrays(c, occidx, len, 5, a->scan_tmp, a->scan_tmp_len));   // the five at once
   hipLaunchKernelGGL(k_cell_lists, dim3(1024), dim3(256), 0, st, occidx, fitidx, nfidx, nfpre, nunit,
                      a->cellrun, (int)a->ncell, n, a->occupied, a->bounds, a->cellmap, a->fitlist,
-                     a->nonfit, a->nonfit_prefix, a->units, a->unitstart, a->counters);
+                     a->nonfit, a->nonfit_prefix, a->units, a->unitstart, a->counters, a->unit);
   VH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_outlist, dim3(1024), dim3(256), 0, st, a->nonfit, a->nonfit_prefix,
                      a->bounds, a->counters, a->outlist);
