@@ -406,6 +406,64 @@ int vh_sample_rejection(const double *grid, const double *potential, int ngrid,
                         double rmax, double pmax, double fermi, int64_t npart,
                         double weight, int seed, double *pos, double *mom);
 
+/* --- the learned denoiser -------------------------------------------------
+ * No counterpart in the reference.  A three-level 3-D U-Net (16 / 32 / 64
+ * channels, 3^3 convolutions with zero padding, 2^3 average pooling, nearest
+ * 2x upsampling, skip connections by concatenation, ReLU) mapping the fine
+ * density deposited by N pseudo-particles to the one 10 N deposit: the input
+ * x = rho / rho0 (rho0 the jellium's mean density), the output
+ * rho0 (x + o(x)).  Products use bf16 roundings of the weights with FP32
+ * accumulation; the optimiser (Adam) keeps FP32 master weights.
+ *
+ * Weights, vh_denoiser_param_count() floats: for each layer l = 0..10 in
+ * order, w[tap][cin][cout] then b[cout], tap = 9 dz + 3 dy + dx over the
+ * offsets (dx, dy, dz) - 1 in {-1, 0, 1}^3 (x the grid's fastest index i).
+ * Layers (cin -> cout): 1->16, 16->16 | pool | 16->32, 32->32 | pool |
+ * 32->64, 64->64, 64->32 | up, concat [up, level-2 skip] | 64->32, 32->16 |
+ * up, concat [up, level-1 skip] | 32->16, 16->1 (no ReLU on the last).
+ * weights NULL: He-normal weights from `seed`, zero biases and a zero last
+ * layer (the untrained net is the identity). */
+typedef struct vh_denoiser vh_denoiser;
+int vh_denoiser_param_count(int64_t *n);
+int vh_denoiser_create(vh_context *ctx, const float *weights, int64_t nweights, uint64_t seed,
+                       double rho0, vh_denoiser **out);
+/* waits for the device; may follow the destruction of its context */
+int vh_denoiser_destroy(vh_denoiser *d);
+/* master weights and the bf16 roundings the products use (either NULL) */
+int vh_denoiser_get_weights(vh_denoiser *d, float *w, float *weff);
+/* One cube of side D (multiple of 4, <= 160) through the net: x, t the
+ * scaled input and target ([z][y][x], D^3 each), o the correction (may be
+ * NULL), *loss = mean((x + o - t)^2), grad (may be NULL) its gradient with
+ * respect to every parameter (the master layout). */
+int vh_denoiser_eval(vh_denoiser *d, int D, const float *x, const float *t, float *o,
+                     double *loss, float *grad);
+/* The training set in device memory: pairs of n^3 fine densities. */
+int vh_denoiser_dataset_reserve(vh_denoiser *d, int n, int64_t capacity);
+/* appends (rho of `low`, rho of `high`): the N and the 10 N density */
+int vh_denoiser_dataset_add(vh_denoiser *d, vh_mesh *low, vh_mesh *high);
+int vh_denoiser_dataset_size(vh_denoiser *d, int64_t *count);
+/* `iters` Adam steps (learning rate lr), each on the mean gradient of
+ * `batch` crops of side C (multiple of 4) from pairs [first, last), their
+ * corners within `radius` points of the centred corner, drawn from `seed`;
+ * losses[it] (may be NULL) the batch's mean crop loss. */
+int vh_denoiser_train(vh_denoiser *d, int iters, int C, int batch, int64_t first, int64_t last,
+                      int radius, double lr, uint64_t seed, double *losses);
+/* mean((x - t)^2) and mean((x + o - t)^2) over the central cube of side C
+ * of pair k: the plain and the denoised density's error */
+int vh_denoiser_dataset_error(vh_denoiser *d, int64_t k, int C, double *mse_plain,
+                              double *mse_net);
+/* The net on a mesh's density in place (the density padded with zeros to
+ * a multiple of 4 points per axis), then the residual charge check: with q
+ * the density's total charge and q' the net's, the net's density scaled by
+ * q / q' replaces it when |q' / q - 1| <= tol; otherwise the density stays
+ * as deposited and the fallback count grows.  Asynchronous. */
+int vh_denoiser_apply(vh_denoiser *d, vh_mesh *m, double tol);
+int vh_denoiser_fallbacks(vh_denoiser *d, int32_t *count);
+/* vh_step runs vh_denoiser_apply(d, fine, tol) between the fine deposit and
+ * the Poisson solve (d NULL detaches; the denoiser must share the
+ * accelerator's context and outlive the attachment). */
+int vh_accel_set_denoiser(vh_accel *a, vh_denoiser *d, double tol);
+
 #ifdef __cplusplus
 }
 #endif

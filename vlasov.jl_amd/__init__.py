@@ -7,7 +7,7 @@ for gfx950 behind the C ABI of include/vlasov_hip.h.  This package is the
 host side: FP64 construction of the tables, the device-layer objects of the
 reference (DeviceMesh, DeviceAccelerator) and the simulation driver.
 """
-from . import _lib, batch, grids, physics
+from . import _lib, batch, denoiser, grids, physics
 from ._lib import VlasovHIPError, device_count
 from .device import Context, DeviceAccelerator, DeviceMesh, ForceAccelerator, poisson_nested
 from .grids import GaussianSmoothing, SplineAxis, SplineMesh, stretched_axis, uniform_axis
@@ -21,4 +21,5 @@ __all__ = [
     "uniform_axis", "BallSoftening", "GaussianSoftening", "Jellium", "PACKED_DTYPE",
     "Projectile", "RadialProfile", "read_radial_profile", "sample_thomas_fermi", "Simulation",
     "SimulationParameters", "xenon_parameters", "xenon_projectile", "grids", "physics", "batch",
+    "denoiser",
 ]

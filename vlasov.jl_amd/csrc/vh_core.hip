@@ -512,6 +512,21 @@ extern "C" int vh_accel_set_option(vh_accel *a, int option, int value) {
   return VH_EINVAL;
 }
 
+extern "C" int vh_accel_set_denoiser(vh_accel *a, vh_denoiser *d, double tol) {
+  VH_REQUIRE(a != nullptr, "vh_accel_set_denoiser: null accelerator");
+  VH_REQUIRE(std::isfinite(tol) && tol >= 0, "vh_accel_set_denoiser: tolerance must be >= 0");
+  if (d) {
+    VH_REQUIRE(vh_denoiser_context(d) == a->ctx,
+               "vh_accel_set_denoiser: denoiser and accelerator on different contexts");
+    VH_TRY(vh_denoiser_prepare(d, a->n));
+  }
+  a->denoiser = d;
+  a->denoise_tol = tol;
+  drop_graphs(a);    // the recorded steps carry the old pipeline
+  ++a->config_version;
+  return VH_OK;
+}
+
 // 48-byte PackedParticle records <-> the buffer's current and previous arrays
 __global__ void k_records_in(const int32_t *__restrict__ rec, int32_t *__restrict__ P, int64_t n,
                              int64_t npad) {
@@ -1348,6 +1363,7 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
   if (c->prof_on & staged) {
     // timing of these stages: the stages one after the other
     VH_TRY(vh_deposit_smoothed(a, fine, weight, nullptr));
+    if (a->denoiser) VH_TRY(vh_denoiser_apply_impl(a->denoiser, fine, a->denoise_tol));
     VH_TRY(vh_deposit_cic(a, coarse, weight));
     VH_TRY(vh_poisson_nested(fine, coarse));
     VH_TRY(vh_potential_coefficients(fine, jel_nions, jel_radius));
@@ -1374,6 +1390,7 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
     // for the second join.
     VH_CHECK_HIP(hipEventRecord(c->fork_ev, c->stream));
     VH_TRY(vh_deposit_fine_impl(a, fine, weight));
+    if (a->denoiser) VH_TRY(vh_denoiser_apply_impl(a->denoiser, fine, a->denoise_tol));
     // the fine mean field's coefficients need only the density: they fill
     // the main stream's wait for the boundary
     VH_TRY(vh_mean_field_coefficients_impl(fine, jel_nions, jel_radius));

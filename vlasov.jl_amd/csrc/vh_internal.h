@@ -293,6 +293,10 @@ struct vh_accel {
   double *host_knots64;
   int64_t host_csol_cap, host_knots_cap;
   double *force64;             // npart x 3, the cloud's order
+  // the learned denoiser run on the fine density between the deposit and
+  // the Poisson solve of vh_step (vh_accel_set_denoiser), and its tolerance
+  vh_denoiser *denoiser;
+  double denoise_tol;
 };
 
 // ---------------------------------------------------------------------------
@@ -403,6 +407,11 @@ int vh_potential_coefficients_impl(vh_mesh *m, double nions, double radius);
 int vh_mean_field_coefficients_impl(vh_mesh *m, double nions, double radius);
 int vh_hartree_coefficients_add_impl(vh_mesh *m);
 int vh_total_charge_impl(vh_mesh *m, const float *rho, double *dq);
+// the denoiser on the fine density in place, with the residual charge check
+// (no host synchronisation, nothing allocated once attached)
+int vh_denoiser_apply_impl(vh_denoiser *d, vh_mesh *m, double tol);
+int vh_denoiser_prepare(vh_denoiser *d, int n);
+vh_context *vh_denoiser_context(const vh_denoiser *d);
 // push_a > 0: the field and fallback kernels also push their particles
 // (Verlet factor dt^2 / mass) and leave the next sort's keys; requires the
 // matrix-core field kernel and vh_verlet_tiles

@@ -112,6 +112,7 @@ class Simulation:
         self.red = np.zeros(4)
         self.step_count = 0
         self.diagnostics = None
+        self._denoiser, self._denoise_tol = None, 0.0
         if positions is None:
             # the Thomas-Fermi sample drawn and uploaded slice by slice
             sampler = RadialSampler(profile, self.weight, seed)
@@ -178,6 +179,8 @@ class Simulation:
         nout = self.acc.deposit_smoothed(self.dmf, w, count=counts)
         if counts:
             self.nout = nout
+        if self._denoiser is not None:
+            self._denoiser.apply(self.dmf, self._denoise_tol)
         self.acc.deposit_cic(self.dmc, w)
         poisson_nested(self.dmf, self.dmc)        # leaves csol_c = S^-1 phi_c
         hartree = None
@@ -203,6 +206,15 @@ class Simulation:
             if advance and self._projectile is not None:
                 self.advance_projectile()
         return hartree
+
+    def attach_denoiser(self, den, tol: float = 0.02):
+        """Runs the learned denoiser `den` (a denoiser.Denoiser on this
+        simulation's context) on the fine density of every later step,
+        between the deposit and the Poisson solve, with the residual charge
+        check at `tol` (None detaches)."""
+        _lib.call("vh_accel_set_denoiser", self.acc.handle, den.handle if den is not None else None,
+                  float(tol))
+        self._denoiser, self._denoise_tol = den, float(tol)
 
     def projectile_forces(self):
         """projectile_forces!(cloud, acc, proj, jel) (src/accelerator.jl:746-756)."""
@@ -280,6 +292,8 @@ class Simulation:
                 "csol": (self.dmf.csol, self.dmc.csol)}
 
     def close(self):
+        if getattr(self, "_denoiser", None) is not None:
+            self.attach_denoiser(None)
         for o in (getattr(self, "acc", None), getattr(self, "dmf", None), getattr(self, "dmc", None)):
             if o is not None:
                 o.close()
