@@ -107,7 +107,7 @@ CROSSING_STEPS = 700     # ceil(140 / (0.4 * 0.5)), scripts/xenon.jl:306-309
 
 
 def run_crossing(va, npart: int, nfine: int, mode: str, warmup: int, steps: int, profile: bool,
-                 seed: int = -1, precision: str = "fp32"):
+                 seed: int = -1, precision: str = "fp32", denoise=None):
     """The crossing at `npart` particles: `warmup` untimed steps, then `steps`
     timed ones, then, untimed, the rest of the 700 steps, after which the
     capture charge q(< 8 a0) is read (step 700 whatever `steps` is).  The
@@ -122,6 +122,12 @@ def run_crossing(va, npart: int, nfine: int, mode: str, warmup: int, steps: int,
     proj = va.xenon_projectile()
     sim = va.Simulation(p, va.read_radial_profile(), projectile=proj, softening_mode=mode, seed=seed,
                         poisson_precision=precision)
+    den = None
+    if denoise is not None:
+        # the learned denoiser between the deposit and the Poisson solve
+        weights, tol = denoise
+        den = va.denoiser.Denoiser(sim.ctx, weights, rho0=va.denoiser.jellium_density(sim.jellium))
+        sim.attach_denoiser(den, tol)
     log(f"built in {time.perf_counter() - t0:.1f} s: {npart} particles, {sim.fine.n}^3 grids")
     nst = len(va._lib.STAGES)
     field_bit = 1 << va._lib.STAGES.index("field_kernel")
@@ -174,8 +180,11 @@ def run_crossing(va, npart: int, nfine: int, mode: str, warmup: int, steps: int,
                outcount=outcount, timed_end=timed_end, step=sim.step_count,
                nocc_end=nocc_end, outcount_end=outcount_end, rest_s=rest,
                rest_steps=sim.step_count - timed_end, e_first=e_first, e_last=e_last,
-               eloss_ev=sim.projectile.energy_loss() * va.physics.HARTREE_TO_EV)
+               eloss_ev=sim.projectile.energy_loss() * va.physics.HARTREE_TO_EV,
+               fallbacks=den.fallbacks if den is not None else None)
     sim.close()
+    if den is not None:
+        den.close()
     return out
 
 
@@ -283,14 +292,21 @@ def batching_extension(va, count: int, steps: int, nfine: int = 64):
     return out
 
 
-def isolated_drift(va, npart: int, nfine: int, steps: int, every: int):
+def isolated_drift(va, npart: int, nfine: int, steps: int, every: int, denoise=None):
     """Energy conservation of the isolated Na196 cluster, the reference's
     validation observable (src/energy.jl:1-7): the budget every `every` steps
-    over `steps` steps, no projectile."""
+    over `steps` steps, no projectile (with `denoise`, the learned denoiser
+    on every step's fine density)."""
     p = va.xenon_parameters(npart, nfine)
     sim = va.Simulation(p, va.read_radial_profile())
+    den = None
+    if denoise is not None:
+        den = va.denoiser.Denoiser(sim.ctx, denoise[0], rho0=va.denoiser.jellium_density(sim.jellium))
+        sim.attach_denoiser(den, denoise[1])
     hist = sim.run(steps, energy_every=every)
     sim.close()
+    if den is not None:
+        den.close()
     tot = np.array([b.total for b in hist])
     b1 = hist[0]
     # the reference's measure (test/runtests.jl:1138-1148): the range of the
@@ -304,6 +320,62 @@ def isolated_drift(va, npart: int, nfine: int, steps: int, every: int):
             "range_over_scale": float((tot.max() - tot.min()) / scale),
             "oscillates": bool(not np.all(dif < 0) and not np.all(dif > 0)),
             "reference_bar": "range/scale < 1e-3 and not monotonic (test/runtests.jl:1138-1148)"}
+
+
+def denoiser_extension(va, weights_path, npart: int, nfine: int, mode: str, warmup: int,
+                       steps: int, seeds, tol: float, plain: dict, reference: dict | None,
+                       drift_steps: int, reference_particles: int):
+    """Config 5: the crossing at `npart` particles with the trained denoiser
+    run on every step's fine density (matrix-core inference, residual charge
+    check at `tol`, the plain density as fallback): ms/step over the timed
+    steps, q(< 8 a0) at step 700 over the seeds of the plain spread, against
+    the plain runs and the converged plain run; the isolated cluster's energy
+    drift with the denoiser, beside the plain one at `npart` and at the
+    converged run's count; the wall-clock fraction of the denoised run
+    against the converged plain run."""
+    w = np.load(weights_path, allow_pickle=False)
+    report = Path(weights_path).with_suffix(".json")
+    out = {"weights": Path(weights_path).name, "particles": npart, "grid": 2 * nfine + 2,
+           "tol": tol}
+    if report.exists():
+        t = json.loads(report.read_text())
+        out["training"] = {k: t[k] for k in ("pairs", "training_pairs", "collisions", "iters",
+                                             "holdout_mse_plain", "holdout_mse_net", "holdout_gain")
+                           if k in t}
+    log("denoiser extension: timed crossing...")
+    r = run_crossing(va, npart, nfine, mode, warmup, steps, False, denoise=(w, tol))
+    qs = [r["q"]]
+    fb = [r["fallbacks"]]
+    for sd in seeds:
+        log(f"denoiser extension: seed {sd}...")
+        rs = run_crossing(va, npart, nfine, mode, 0, 0, False, seed=sd, denoise=(w, tol))
+        qs.append(rs["q"])
+        fb.append(rs["fallbacks"])
+    a = np.array(qs)
+    out["ms_per_step"] = 1000.0 * r["elapsed"] / steps
+    out["plain_ms_per_step"] = plain["ms_per_step"]
+    out["capture_q8"] = {"seeds": [-1] + list(seeds), "denoised": qs, "mean": float(a.mean()),
+                         "std": float(a.std(ddof=1)) if len(a) > 1 else None,
+                         "plain": plain["q8"], "plain_mean": float(np.mean(plain["q8"]))}
+    out["fallbacks"] = fb
+    out["projectile_energy_loss_eV"] = r["eloss_ev"]
+    if reference is not None:
+        q_ref = reference["capture_q8"]
+        out["capture_q_err"] = {"denoised_seed_-1": abs(qs[0] - q_ref),
+                                "denoised_mean": abs(float(a.mean()) - q_ref),
+                                "plain_seed_-1": abs(plain["q8"][0] - q_ref),
+                                "plain_mean": abs(float(np.mean(plain["q8"])) - q_ref),
+                                "reference_q8": q_ref, "reference_particles": reference["particles"]}
+        out["wall_clock_fraction"] = out["ms_per_step"] / reference["ms_per_step"]
+    if drift_steps > 0:
+        log("denoiser extension: isolated cluster energy drift...")
+        out["isolated_energy_drift"] = isolated_drift(va, npart, nfine, drift_steps, 10,
+                                                      denoise=(w, tol))
+        if reference_particles > 0:
+            log(f"isolated cluster energy drift at {reference_particles} particles (plain)...")
+            out["isolated_energy_drift_reference"] = isolated_drift(va, reference_particles, nfine,
+                                                                    drift_steps, 10)
+    return out
 
 
 def main():
@@ -328,6 +400,11 @@ def main():
     ap.add_argument("--batch-collisions", type=int, default=16,
                     help="collisions of the batching extension's run (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=30)
+    ap.add_argument("--denoiser-weights", type=Path, default=ROOT / "vlasov.jl_amd" / "data" /
+                    "denoiser_na196_130.npy",
+                    help="trained denoiser of the learned extension (missing file: skip)")
+    ap.add_argument("--denoiser-tol", type=float, default=0.05,
+                    help="residual charge check of the denoiser (relative)")
     ap.add_argument("--cpu-particles", type=int, default=100_000)
     ap.add_argument("--cpu-nfine", type=int, default=28)
     ap.add_argument("--cpu-steps", type=int, default=150)
@@ -391,9 +468,9 @@ def main():
         # electrons only: the projectile's interaction is not in the budget
         "electron_energy": {"step1": e0.total if e0 else None, "last": e1.total if e1 else None},
     }
+    seeds = [-(2 + i) for i in range(args.spread_seeds)]
     if args.spread_seeds > 0:
-        line["capture_spread"] = capture_spread(va, args.particles, args.nfine, args.mode,
-                                                [-(2 + i) for i in range(args.spread_seeds)],
+        line["capture_spread"] = capture_spread(va, args.particles, args.nfine, args.mode, seeds,
                                                 r["q"])
     if args.drift_steps > 0:
         log("isolated cluster energy drift...")
@@ -411,6 +488,15 @@ def main():
             "ms_per_step": 1000.0 * rr["elapsed"] / args.steps,
             "projectile_energy_loss_eV": rr["eloss_ev"],
             "published_q8": {"8e6 (RTX 4070 / M1 Max)": [0.842, 0.840], "8e7 (M1 Max)": 0.81}}
+    if args.denoiser_weights is not None and args.denoiser_weights.exists():
+        plain = {"ms_per_step": ms_step,
+                 "q8": line["capture_spread"]["q8"] if "capture_spread" in line else [r["q"]]}
+        line["denoiser_extension"] = denoiser_extension(
+            va, args.denoiser_weights, args.particles, args.nfine, args.mode, args.warmup,
+            args.steps, seeds, args.denoiser_tol, plain, line.get("capture_reference"),
+            args.drift_steps, args.reference_particles)
+    else:
+        log(f"denoiser extension skipped: no weights at {args.denoiser_weights}")
     if args.precision_particles > 0:
         line["precision_extension"] = precision_extension(va, args.precision_particles, 128,
                                                           args.mode, args.warmup, args.steps)

@@ -431,6 +431,8 @@ int vh_denoiser_create(vh_context *ctx, const float *weights, int64_t nweights, 
 int vh_denoiser_destroy(vh_denoiser *d);
 /* master weights and the bf16 roundings the products use (either NULL) */
 int vh_denoiser_get_weights(vh_denoiser *d, float *w, float *weff);
+/* replaces the master weights (and restarts the optimiser) */
+int vh_denoiser_set_weights(vh_denoiser *d, const float *w, int64_t nweights);
 /* One cube of side D (multiple of 4, <= 160) through the net: x, t the
  * scaled input and target ([z][y][x], D^3 each), o the correction (may be
  * NULL), *loss = mean((x + o - t)^2), grad (may be NULL) its gradient with
@@ -459,6 +461,12 @@ int vh_denoiser_dataset_error(vh_denoiser *d, int64_t k, int C, double *mse_plai
  * as deposited and the fallback count grows.  Asynchronous. */
 int vh_denoiser_apply(vh_denoiser *d, vh_mesh *m, double tol);
 int vh_denoiser_fallbacks(vh_denoiser *d, int32_t *count);
+/* The inference path of vh_denoiser_apply: 0 (default) the matrix cores,
+ * activations rounded to bf16 between layers (v_mfma_f32_16x16x32_bf16,
+ * FP32 accumulation, output rho + rho0 o); 1 the FP32-activation kernels of
+ * the training, output rho0 (x + o).  Choose before attaching to a recorded
+ * step. */
+int vh_denoiser_set_inference(vh_denoiser *d, int fp32);
 /* vh_step runs vh_denoiser_apply(d, fine, tol) between the fine deposit and
  * the Poisson solve (d NULL detaches; the denoiser must share the
  * accelerator's context and outlive the attachment). */

@@ -111,17 +111,20 @@ def up_back(dout):
 
 
 # --- the network -----------------------------------------------------------------
-def forward(params, x):
-    """x (D, D, D) with D a multiple of 4 -> (o (D, D, D), cache)."""
+def forward(params, x, act_bf16=False):
+    """x (D, D, D) with D a multiple of 4 -> (o (D, D, D), cache).  With
+    act_bf16 the input, every ReLU output and every pooled tensor are rounded
+    to bfloat16 (the matrix-core inference's storage)."""
     L = unpack(params)
-    relu = lambda z: np.maximum(z, 0.0)
-    c = {"x": x[..., None]}
+    q = bf16 if act_bf16 else (lambda a: a)
+    relu = lambda z: q(np.maximum(z, 0.0))
+    c = {"x": q(x)[..., None]}
     c["a0"] = relu(conv3(c["x"], *L[0]))
     c["a1"] = relu(conv3(c["a0"], *L[1]))
-    c["p1"] = pool(c["a1"])
+    c["p1"] = q(pool(c["a1"]))
     c["a2"] = relu(conv3(c["p1"], *L[2]))
     c["a3"] = relu(conv3(c["a2"], *L[3]))
-    c["p2"] = pool(c["a3"])
+    c["p2"] = q(pool(c["a3"]))
     c["a4"] = relu(conv3(c["p2"], *L[4]))
     c["a5"] = relu(conv3(c["a4"], *L[5]))
     c["a6"] = relu(conv3(c["a5"], *L[6]))

@@ -96,6 +96,7 @@ _SIGNATURES = {
     "vh_denoiser_create": [_vp, _vp, C.c_int64, C.c_uint64, C.c_double, C.POINTER(_vp)],
     "vh_denoiser_destroy": [_vp],
     "vh_denoiser_get_weights": [_vp, _vp, _vp],
+    "vh_denoiser_set_weights": [_vp, _f32p, C.c_int64],
     "vh_denoiser_eval": [_vp, C.c_int, _f32p, _f32p, _vp, C.POINTER(C.c_double), _vp],
     "vh_denoiser_dataset_reserve": [_vp, C.c_int, C.c_int64],
     "vh_denoiser_dataset_add": [_vp, _vp, _vp],
@@ -106,6 +107,7 @@ _SIGNATURES = {
                                   C.POINTER(C.c_double)],
     "vh_denoiser_apply": [_vp, _vp, C.c_double],
     "vh_denoiser_fallbacks": [_vp, C.POINTER(C.c_int32)],
+    "vh_denoiser_set_inference": [_vp, C.c_int],
     "vh_accel_set_denoiser": [_vp, _vp, C.c_double],
     "vh_interaction_energy": [_vp, _vp, _vp, C.c_double, C.c_double, C.POINTER(C.c_double)],
     "vh_profile_enable": [_vp, C.c_int],

@@ -350,9 +350,256 @@ __global__ void k_select(float *__restrict__ rho, const float *__restrict__ net,
     rho[v] = net[v] * f;
 }
 
+
+// ---------------------------------------------------------------------------
+// inference on the matrix cores
+//
+// Activations in bf16, channels-last, each level's tensors stored with a
+// zero border: voxel (z, y, x) of a level of interior side D at
+// ((z + 1) Ey + y + 1) Ex + x + 1, Ey = Ex = roundup(D, 8) + 2 and
+// roundup(D, 4) + 2 planes, everything outside the interior zero (the
+// convolutions' zero padding, read without bounds checks).  A convolution
+// is an implicit GEMM out[v][co] = sum_k A[v][k] B[k][co] over k = (tap, ci)
+// on v_mfma_f32_16x16x32_bf16: a workgroup stages a brick of BZ x 8 x 8
+// output voxels plus its one-voxel halo in LDS (channels padded by 8 so the
+// 16 voxel rows of a fragment fall in distinct bank groups), wave w computes
+// plane w of the brick (four 16-voxel row tiles) for every output channel,
+// with the weights packed in the MFMA B-operand order (bf16 roundings of the
+// same weights as the FP32 path) streamed from L2.  Bias, ReLU and the
+// rounding to bf16 in the epilogue.  The first layer reads the FP32 density
+// itself (x = rho / rho0 rounded to bf16), the last writes
+// rho + rho0 o in FP32 into the output grid.  Concatenated inputs
+// [up(coarse), skip] are staged straight from the coarse tensor.
+// ---------------------------------------------------------------------------
+typedef __bf16 un_bf8 __attribute__((ext_vector_type(8)));
+typedef float un_f4 __attribute__((ext_vector_type(4)));
+typedef unsigned int un_u4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t un_rne(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
+// MFMA K steps of a layer with cin input channels: 32 k values a step
+__host__ __device__ constexpr int conv_ks(int cin) {
+  return cin == 1 ? 1 : cin == 16 ? 14 : 27 * (cin / 32);
+}
+
+// (tap, first channel) of the eight k values of k-group lk in step s
+__device__ __forceinline__ void conv_k(int cin, int s, int lk, int &tap, int &c0) {
+  if (cin >= 32) {
+    const int sp = cin / 32;
+    tap = s / sp;
+    c0 = (s - tap * sp) * 32 + 8 * lk;
+  } else {
+    tap = 2 * s + (lk >> 1);
+    c0 = 8 * (lk & 1);
+  }
+}
+
+// B operand of layer (cin, cout) from w[tap][cin][cout]: u4 (eight bf16) per
+// (step s, column tile nt, lane l), k values as conv_k, zero beyond the taps
+// and the channels
+__global__ void k_pack_conv(const float *__restrict__ w, int cin, int cout, int ks, int nt,
+                            un_u4 *__restrict__ out) {
+  const int total = ks * nt * 64;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+    const int l = e & 63, t = (e >> 6) % nt, st = (e >> 6) / nt;
+    const int co = 16 * t + (l & 15), lk = l >> 4;
+    uint32_t h[8];
+    for (int j = 0; j < 8; ++j) {
+      int tap, ci;
+      if (cin == 1) {
+        tap = 8 * lk + j;
+        ci = 0;
+      } else {
+        conv_k(cin, st, lk, tap, ci);
+        ci += j;
+      }
+      const bool ok = tap < 27 && ci < cin && co < cout;
+      h[j] = ok ? un_rne(w[((int64_t)tap * cin + ci) * cout + co]) : 0u;
+    }
+    out[e] = (un_u4){h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16)};
+  }
+}
+
+enum { CM_HIDDEN = 0, CM_FIRST = 1, CM_LAST = 2 };
+
+struct conv_args {
+  const unsigned short *srcA;  // coarse tensor of the concatenation (CA channels) or null
+  int EyA, ExA;
+  const unsigned short *srcB;  // same-level tensor (CB channels)
+  int Ey, Ex;
+  const float *rho;            // CM_FIRST, CM_LAST: the n^3 density
+  int n;
+  float inv, rho0;             // 1 / rho0, rho0
+  const un_u4 *wp;             // packed B operand
+  const float *bias;
+  unsigned short *out;         // CM_HIDDEN: bf16 tensor of COUT channels (Ey, Ex geometry)
+  float *net;                  // CM_LAST: n^3 output density
+  int D;                       // interior side of this level
+  int nbx, nby, nbz;           // bricks per axis
+};
+
+template <int CA, int CB, int COUT, int BZ, int MODE>
+__global__ void __launch_bounds__(64 * BZ) k_conv_mfma(conv_args a) {
+  constexpr int CIN = CA + CB;
+  constexpr int SV = MODE == CM_FIRST ? 1 : CIN + 8;   // LDS elements a voxel
+  constexpr int KS = conv_ks(CIN);
+  constexpr int NT = (COUT + 15) / 16;
+  constexpr int HV = (BZ + 2) * 100;
+  extern __shared__ __align__(16) unsigned short sm[];
+  const int nb = a.nbx * a.nby * a.nbz;
+  // XCD-aware: the bricks of one L2 are a contiguous range
+  const int per = (nb + 7) / 8;
+  const int brick = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (brick >= nb) return;
+  const int bx = brick % a.nbx, by = (brick / a.nbx) % a.nby, bz = brick / (a.nbx * a.nby);
+  const int x0 = 8 * bx, y0 = 8 * by, z0 = BZ * bz;
+  // stage the brick and its halo
+  if (MODE == CM_FIRST) {
+    float *f = (float *)sm;
+    for (int e = threadIdx.x; e < HV; e += blockDim.x) {
+      const int hx = e % 10, hy = (e / 10) % 10, hz = e / 100;
+      const int z = z0 + hz - 1, y = y0 + hy - 1, x = x0 + hx - 1;
+      const bool in = (unsigned)z < (unsigned)a.n && (unsigned)y < (unsigned)a.n && (unsigned)x < (unsigned)a.n;
+      f[e] = in ? a.rho[((int64_t)z * a.n + y) * a.n + x] * a.inv : 0.f;
+    }
+  } else {
+    constexpr int CH = CIN / 8;
+    for (int e = threadIdx.x; e < HV * CH; e += blockDim.x) {
+      const int hv = e / CH, c = 8 * (e - hv * CH);
+      const int hx = hv % 10, hy = (hv / 10) % 10, hz = hv / 100;
+      const int z = z0 + hz - 1, y = y0 + hy - 1, x = x0 + hx - 1;
+      un_u4 v = (un_u4){0u, 0u, 0u, 0u};
+      if (CA > 0 && c < CA) {
+        const int Df = a.D;
+        if ((unsigned)z < (unsigned)Df && (unsigned)y < (unsigned)Df && (unsigned)x < (unsigned)Df) {
+          const int64_t cv = ((int64_t)((z >> 1) + 1) * a.EyA + (y >> 1) + 1) * a.ExA + (x >> 1) + 1;
+          v = *(const un_u4 *)(a.srcA + cv * CA + c);
+        }
+      } else {
+        const int64_t sv = ((int64_t)(z + 1) * a.Ey + y + 1) * a.Ex + x + 1;
+        v = *(const un_u4 *)(a.srcB + sv * CB + (c - CA));
+      }
+      *(un_u4 *)(sm + hv * SV + c) = v;
+    }
+  }
+  __syncthreads();
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lc = l & 15, lk = l >> 4;
+  un_f4 acc[4][NT];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[mt][t] = (un_f4){0.f, 0.f, 0.f, 0.f};
+  const un_u4 *wp = a.wp + l;
+#pragma unroll 2
+  for (int st = 0; st < KS; ++st) {
+    un_u4 b[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) b[t] = wp[(st * NT + t) * 64];
+    if (MODE == CM_FIRST) {
+      const float *f = (const float *)sm;
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        const int vy = 2 * mt + (lc >> 3), vx = lc & 7;
+        uint32_t h[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int tap = 8 * lk + j;
+          const int tp = tap < 27 ? tap : 0;
+          const int dz = tp / 9, dy = (tp / 3) % 3, dx = tp % 3;
+          h[j] = tap < 27 ? un_rne(f[((w + dz) * 10 + vy + dy) * 10 + vx + dx]) : 0u;
+        }
+        const un_u4 av = (un_u4){h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16),
+                                 h[6] | (h[7] << 16)};
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+          acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(un_bf8, av),
+                                                               __builtin_bit_cast(un_bf8, b[t]),
+                                                               acc[mt][t], 0, 0, 0);
+      }
+    } else {
+      int tap, c0;
+      conv_k(CIN, st, lk, tap, c0);
+      const bool ok = tap < 27;
+      const int tp = ok ? tap : 0;
+      const int dz = tp / 9, dy = (tp / 3) % 3, dx = tp % 3;
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        const int vy = 2 * mt + (lc >> 3), vx = lc & 7;
+        un_u4 av = *(const un_u4 *)(sm + (((w + dz) * 10 + vy + dy) * 10 + vx + dx) * SV + c0);
+        if (!ok) av = (un_u4){0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+          acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(un_bf8, av),
+                                                               __builtin_bit_cast(un_bf8, b[t]),
+                                                               acc[mt][t], 0, 0, 0);
+      }
+    }
+  }
+  // epilogue: row 4 lk + v of row tile mt is voxel (2 mt + row / 8, row % 8)
+  const int z = z0 + w;
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = 4 * lk + v;
+      const int y = y0 + 2 * mt + (row >> 3), x = x0 + (row & 7);
+      if (MODE == CM_LAST) {
+        if (lc == 0 && z < a.n && y < a.n && x < a.n) {
+          const int64_t g = ((int64_t)z * a.n + y) * a.n + x;
+          a.net[g] = a.rho[g] + a.rho0 * (acc[mt][0][v] + a.bias[0]);
+        }
+      } else if (z < a.D && y < a.D && x < a.D) {
+        const int64_t ov = ((int64_t)(z + 1) * a.Ey + y + 1) * a.Ex + x + 1;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const int co = 16 * t + lc;
+          a.out[ov * COUT + co] = (unsigned short)un_rne(fmaxf(acc[mt][t][v] + a.bias[co], 0.f));
+        }
+      }
+    }
+}
+
+// 2^3 mean pooling of bf16 tensors (C channels): fine (Ey, Ex) -> coarse
+// (Eyc, Exc) of interior side Dc; eight channels a thread
+__global__ void k_pool_bf16(const unsigned short *__restrict__ fine, int Ey, int Ex,
+                            unsigned short *__restrict__ coarse, int Eyc, int Exc, int Dc, int C) {
+  const int CH = C / 8;
+  const int64_t total = (int64_t)Dc * Dc * Dc * CH;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int c = 8 * (int)(e % CH);
+    const int64_t v = e / CH;
+    const int x = (int)(v % Dc), y = (int)((v / Dc) % Dc), z = (int)(v / ((int64_t)Dc * Dc));
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < 8; ++k) {
+      const int fz = 2 * z + (k >> 2), fy = 2 * y + ((k >> 1) & 1), fx = 2 * x + (k & 1);
+      const un_u4 q = *(const un_u4 *)(fine + (((int64_t)(fz + 1) * Ey + fy + 1) * Ex + fx + 1) * C + c);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        s[2 * j] += __uint_as_float(q[j] << 16);
+        s[2 * j + 1] += __uint_as_float(q[j] & 0xffff0000u);
+      }
+    }
+    uint32_t h[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) h[j] = un_rne(0.125f * s[j]);
+    *(un_u4 *)(coarse + (((int64_t)(z + 1) * Eyc + y + 1) * Exc + x + 1) * C + c) =
+        (un_u4){h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16)};
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
+static unsigned blocks_for(int64_t n) {
+  const int64_t b = (n + 255) / 256;
+  return (unsigned)(b < 8192 ? (b > 0 ? b : 1) : 8192);
+}
+
 struct vh_unet_buffers {
   int D = 0;
   // activations: level 1 (D^3): a0 (16), a1 (16), cat1 (32: up of L8, a1), a9 (16), o (1), x (1), t (1)
@@ -387,7 +634,128 @@ struct vh_denoiser {
   int64_t dcap = 0, dcount = 0;
   float *data = nullptr;
   double rho0 = 1.0;
+  // inference on the matrix cores: packed B operands, bf16 tensors
+  un_u4 *packed = nullptr;
+  int64_t poff[NL + 1] = {};
+  struct level {
+    int D = 0, Ey = 0, Ez = 0;
+    int64_t voxels() const { return (int64_t)Ez * Ey * Ey; }
+  } lv[3];
+  int mfP = 0;
+  unsigned short *t_a0 = nullptr, *t_a1 = nullptr, *t_p1 = nullptr, *t_a2 = nullptr, *t_a3 = nullptr,
+                 *t_p2 = nullptr, *t_a4 = nullptr, *t_a5 = nullptr, *t_a6 = nullptr, *t_a7 = nullptr,
+                 *t_a8 = nullptr, *t_a9 = nullptr;
+  int fp32_infer = 0;         // vh_denoiser_set_inference
+  int prep_n = 0;             // the grid the inference buffers were made for
 };
+
+static void free_mfma(vh_denoiser *d) {
+  unsigned short *ps[] = {d->t_a0, d->t_a1, d->t_p1, d->t_a2, d->t_a3, d->t_p2, d->t_a4, d->t_a5,
+                          d->t_a6, d->t_a7, d->t_a8, d->t_a9};
+  for (unsigned short *p : ps) vh_free(p);
+  d->t_a0 = d->t_a1 = d->t_p1 = d->t_a2 = d->t_a3 = d->t_p2 = d->t_a4 = d->t_a5 = d->t_a6 = d->t_a7 =
+      d->t_a8 = d->t_a9 = nullptr;
+  d->mfP = 0;
+}
+
+// the zero-bordered bf16 tensors of a P^3 input (levels P, P/2, P/4),
+// zeroed once: only interiors are ever written
+static int ensure_mfma(vh_denoiser *d, int P) {
+  if (d->mfP == P) return VH_OK;
+  free_mfma(d);
+  for (int k = 0; k < 3; ++k) {
+    const int D = P >> k;
+    d->lv[k].D = D;
+    d->lv[k].Ey = (D + 7) / 8 * 8 + 2;
+    d->lv[k].Ez = (D + 3) / 4 * 4 + 2;
+  }
+  struct {
+    unsigned short **p;
+    int level, c;
+  } t[] = {{&d->t_a0, 0, 16}, {&d->t_a1, 0, 16}, {&d->t_a9, 0, 16}, {&d->t_p1, 1, 16},
+           {&d->t_a2, 1, 32}, {&d->t_a3, 1, 32}, {&d->t_a7, 1, 32}, {&d->t_a8, 1, 16},
+           {&d->t_p2, 2, 32}, {&d->t_a4, 2, 64}, {&d->t_a5, 2, 64}, {&d->t_a6, 2, 32}};
+  for (auto &e : t) {
+    const size_t bytes = sizeof(unsigned short) * e.c * d->lv[e.level].voxels();
+    VH_TRY(vh_alloc((void **)e.p, bytes));
+    VH_CHECK_HIP(hipMemset(*e.p, 0, bytes));
+  }
+  d->mfP = P;
+  return VH_OK;
+}
+
+// the B operands of every layer from the bf16 weights (on the stream)
+static int pack_weights(vh_denoiser *d) {
+  for (int l = 0; l < NL; ++l) {
+    const int ks = conv_ks(CIN[l]), nt = (COUT[l] + 15) / 16;
+    hipLaunchKernelGGL(k_pack_conv, dim3((ks * nt * 64 + 255) / 256), dim3(256), 0, d->ctx->stream,
+                       d->weff + woff(l), CIN[l], COUT[l], ks, nt, d->packed + d->poff[l]);
+  }
+  VH_LAUNCH_CHECK();
+  return VH_OK;
+}
+
+template <int CA, int CB, int COUT_, int BZ, int MODE>
+static int launch_conv(vh_denoiser *d, hipStream_t st, int l, int level, const unsigned short *srcA,
+                       const unsigned short *srcB, unsigned short *out, const float *rho, int n,
+                       float *net) {
+  static_assert(CA + CB == 1 || (CA + CB) % 16 == 0, "input channels");
+  VH_REQUIRE(CA + CB == CIN[l] && COUT_ == COUT[l], "denoiser: layer %d launched with the wrong shape", l);
+  const auto &L = d->lv[level];
+  conv_args a;
+  a.srcA = srcA;
+  a.EyA = level < 2 ? d->lv[level + 1].Ey : 0;
+  a.ExA = a.EyA;
+  a.srcB = srcB;
+  a.Ey = a.Ex = L.Ey;
+  a.rho = rho;
+  a.n = n;
+  a.inv = (float)(1.0 / d->rho0);
+  a.rho0 = (float)d->rho0;
+  a.wp = d->packed + d->poff[l];
+  a.bias = d->weff + woff(l) + wsize(l);
+  a.out = out;
+  a.net = net;
+  a.D = L.D;
+  a.nbx = a.nby = (L.D + 7) / 8;
+  a.nbz = (L.D + BZ - 1) / BZ;
+  const int nb = a.nbx * a.nby * a.nbz;
+  const int grid = 8 * ((nb + 7) / 8);
+  constexpr int CIN_ = CA + CB;
+  const size_t lds = MODE == CM_FIRST ? sizeof(float) * (BZ + 2) * 100
+                                      : sizeof(unsigned short) * (BZ + 2) * 100 * (CIN_ + 8);
+  hipLaunchKernelGGL((k_conv_mfma<CA, CB, COUT_, BZ, MODE>), dim3(grid), dim3(64 * BZ), lds, st, a);
+  VH_LAUNCH_CHECK();
+  return VH_OK;
+}
+
+static int pool_bf16(vh_denoiser *d, hipStream_t st, const unsigned short *fine, int level,
+                     unsigned short *coarse, int C) {
+  const auto &F = d->lv[level], &Cc = d->lv[level + 1];
+  const int64_t total = (int64_t)Cc.D * Cc.D * Cc.D * (C / 8);
+  hipLaunchKernelGGL(k_pool_bf16, dim3(blocks_for(total)), dim3(256), 0, st, fine, F.Ey, F.Ey, coarse,
+                     Cc.Ey, Cc.Ey, Cc.D, C);
+  VH_LAUNCH_CHECK();
+  return VH_OK;
+}
+
+// the network on the matrix cores: rho (n^3) -> net = rho + rho0 o (n^3)
+static int forward_mfma(vh_denoiser *d, hipStream_t st, const float *rho, int n, float *net) {
+  VH_TRY((launch_conv<0, 1, 16, 4, CM_FIRST>(d, st, 0, 0, nullptr, nullptr, d->t_a0, rho, n, nullptr)));
+  VH_TRY((launch_conv<0, 16, 16, 4, CM_HIDDEN>(d, st, 1, 0, nullptr, d->t_a0, d->t_a1, nullptr, n, nullptr)));
+  VH_TRY(pool_bf16(d, st, d->t_a1, 0, d->t_p1, 16));
+  VH_TRY((launch_conv<0, 16, 32, 4, CM_HIDDEN>(d, st, 2, 1, nullptr, d->t_p1, d->t_a2, nullptr, n, nullptr)));
+  VH_TRY((launch_conv<0, 32, 32, 4, CM_HIDDEN>(d, st, 3, 1, nullptr, d->t_a2, d->t_a3, nullptr, n, nullptr)));
+  VH_TRY(pool_bf16(d, st, d->t_a3, 1, d->t_p2, 32));
+  VH_TRY((launch_conv<0, 32, 64, 4, CM_HIDDEN>(d, st, 4, 2, nullptr, d->t_p2, d->t_a4, nullptr, n, nullptr)));
+  VH_TRY((launch_conv<0, 64, 64, 2, CM_HIDDEN>(d, st, 5, 2, nullptr, d->t_a4, d->t_a5, nullptr, n, nullptr)));
+  VH_TRY((launch_conv<0, 64, 32, 2, CM_HIDDEN>(d, st, 6, 2, nullptr, d->t_a5, d->t_a6, nullptr, n, nullptr)));
+  VH_TRY((launch_conv<32, 32, 32, 2, CM_HIDDEN>(d, st, 7, 1, d->t_a6, d->t_a3, d->t_a7, nullptr, n, nullptr)));
+  VH_TRY((launch_conv<0, 32, 16, 4, CM_HIDDEN>(d, st, 8, 1, nullptr, d->t_a7, d->t_a8, nullptr, n, nullptr)));
+  VH_TRY((launch_conv<16, 16, 16, 4, CM_HIDDEN>(d, st, 9, 0, d->t_a8, d->t_a1, d->t_a9, nullptr, n, nullptr)));
+  VH_TRY((launch_conv<0, 16, 1, 4, CM_LAST>(d, st, 10, 0, nullptr, d->t_a9, nullptr, rho, n, net)));
+  return VH_OK;
+}
 
 static void free_buffers(vh_unet_buffers &b) {
   float *ps[] = {b.x, b.t, b.a0, b.a1, b.cat1, b.a9, b.o, b.p1, b.a2, b.a3, b.cat2, b.a7, b.a8,
@@ -419,11 +787,6 @@ static int ensure_buffers(vh_unet_buffers &b, int D, bool grads) {
   }
   b.D = D;
   return VH_OK;
-}
-
-static unsigned blocks_for(int64_t n) {
-  const int64_t b = (n + 255) / 256;
-  return (unsigned)(b < 8192 ? (b > 0 ? b : 1) : 8192);
 }
 
 // one convolution of layer l: in (ldi, ic0) -> out (ldo, oc0)
@@ -587,10 +950,12 @@ extern "C" int vh_denoiser_param_count(int64_t *n) {
 
 extern "C" int vh_denoiser_destroy(vh_denoiser *d) {
   if (!d) return VH_OK;
-  hipSetDevice(d->device);
-  hipDeviceSynchronize();
+  (void)hipSetDevice(d->device);
+  (void)hipDeviceSynchronize();
   free_buffers(d->train);
   free_buffers(d->infer);
+  free_mfma(d);
+  vh_free(d->packed);
   void *ps[] = {d->w, d->weff, d->m, d->v, d->grad, d->wt, d->part, d->bpart, d->lpart, d->dscal,
                 d->net, d->fallbacks, d->data};
   for (void *p : ps) vh_free(p);
@@ -649,11 +1014,53 @@ extern "C" int vh_denoiser_create(vh_context *ctx, const float *weights, int64_t
     vh_set_error("vh_denoiser_create: upload failed");
     rc = VH_EHIP;
   }
+  if (rc == VH_OK) {
+    int64_t o = 0;
+    for (int l = 0; l < NL; ++l) {
+      d->poff[l] = o;
+      o += (int64_t)conv_ks(CIN[l]) * ((COUT[l] + 15) / 16) * 64;
+    }
+    d->poff[NL] = o;
+    rc = vh_alloc((void **)&d->packed, sizeof(un_u4) * o);
+    if (rc == VH_OK) rc = pack_weights(d);
+    if (rc == VH_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) {
+      vh_set_error("vh_denoiser_create: packing the weights failed");
+      rc = VH_EHIP;
+    }
+  }
   if (rc != VH_OK) {
     vh_denoiser_destroy(d);
     return rc;
   }
   *out = d;
+  return VH_OK;
+}
+
+// new master weights: their bf16 roundings, packed operands, and a fresh
+// optimiser (zero moments, step 0)
+__global__ void k_set_weights(float *__restrict__ w, float *__restrict__ weff, float *__restrict__ m,
+                              float *__restrict__ v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    weff[i] = bf16_round_d(w[i]);
+    m[i] = 0.f;
+    v[i] = 0.f;
+  }
+}
+
+extern "C" int vh_denoiser_set_weights(vh_denoiser *d, const float *w, int64_t nweights) {
+  VH_REQUIRE(d && w, "vh_denoiser_set_weights: null argument");
+  VH_REQUIRE(nweights == d->np, "vh_denoiser_set_weights: %lld weights for %lld parameters",
+             (long long)nweights, (long long)d->np);
+  VH_CHECK_HIP(hipSetDevice(d->ctx->device));
+  hipStream_t st = d->ctx->stream;
+  VH_CHECK_HIP(hipMemcpyAsync(d->w, w, sizeof(float) * d->np, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_set_weights, dim3(blocks_for(d->np)), dim3(256), 0, st, d->w, d->weff, d->m, d->v,
+                     d->np);
+  VH_LAUNCH_CHECK();
+  VH_TRY(pack_weights(d));
+  VH_CHECK_HIP(hipStreamSynchronize(st));
+  d->step = 0;
   return VH_OK;
 }
 
@@ -791,6 +1198,7 @@ extern "C" int vh_denoiser_train(vh_denoiser *d, int iters, int C, int batch, in
   VH_TRY(vh_alloc((void **)&dl, sizeof(double) * ns));
   std::vector<double> lh((size_t)ns);
   int rc = train_loop(d, b, iters, C, batch, pos, pick, rng, lr, dl);
+  if (rc == VH_OK) rc = pack_weights(d);     // the inference path's operands
   if (rc == VH_OK && (hipMemcpyAsync(lh.data(), dl, sizeof(double) * ns, hipMemcpyDeviceToHost, st) != hipSuccess ||
                       hipStreamSynchronize(st) != hipSuccess)) {
     vh_set_error("vh_denoiser_train: reading the losses failed");
@@ -849,14 +1257,20 @@ int vh_denoiser_apply_impl(vh_denoiser *d, vh_mesh *m, double tol) {
   VH_REQUIRE(d->ctx == m->ctx, "denoiser: mesh on another context");
   const int n = m->n, P = (n + 3) / 4 * 4;
   VH_TRY(vh_denoiser_prepare(d, n));
-  vh_unet_buffers &b = d->infer;
   const int64_t g = (int64_t)n * n * n;
-  hipLaunchKernelGGL(k_pad_in, dim3(blocks_for((int64_t)P * P * P)), dim3(256), 0, st, m->rho, n, P,
-                     (float)(1.0 / d->rho0), b.x);
-  VH_LAUNCH_CHECK();
-  VH_TRY(forward(d, b));
-  hipLaunchKernelGGL(k_unpad_out, dim3(blocks_for(g)), dim3(256), 0, st, b.x, b.o, n, P, (float)d->rho0, d->net);
-  VH_LAUNCH_CHECK();
+  if (d->fp32_infer) {
+    // the training kernels (FP32 activations)
+    vh_unet_buffers &b = d->infer;
+    hipLaunchKernelGGL(k_pad_in, dim3(blocks_for((int64_t)P * P * P)), dim3(256), 0, st, m->rho, n, P,
+                       (float)(1.0 / d->rho0), b.x);
+    VH_LAUNCH_CHECK();
+    VH_TRY(forward(d, b));
+    hipLaunchKernelGGL(k_unpad_out, dim3(blocks_for(g)), dim3(256), 0, st, b.x, b.o, n, P,
+                       (float)d->rho0, d->net);
+    VH_LAUNCH_CHECK();
+  } else {
+    VH_TRY(forward_mfma(d, st, m->rho, n, d->net));
+  }
   VH_TRY(vh_total_charge_impl(m, m->rho, d->dscal + 1));
   VH_TRY(vh_total_charge_impl(m, d->net, d->dscal + 2));
   hipLaunchKernelGGL(k_select, dim3(blocks_for(g)), dim3(256), 0, st, m->rho, d->net, g, d->dscal + 1,
@@ -870,7 +1284,11 @@ int vh_denoiser_prepare(vh_denoiser *d, int n) {
   const int P = (n + 3) / 4 * 4;
   VH_REQUIRE(P <= 160, "denoiser: grid of %d points beyond 160", n);
   VH_CHECK_HIP(hipSetDevice(d->ctx->device));
-  VH_TRY(ensure_buffers(d->infer, P, false));
+  if (d->fp32_infer)
+    VH_TRY(ensure_buffers(d->infer, P, false));
+  else
+    VH_TRY(ensure_mfma(d, P));
+  d->prep_n = n;
   const int64_t g = (int64_t)n * n * n;
   if (d->net_cap < g) {
     vh_free(d->net);
@@ -889,6 +1307,14 @@ extern "C" int vh_denoiser_apply(vh_denoiser *d, vh_mesh *m, double tol) {
   VH_REQUIRE(tol >= 0, "vh_denoiser_apply: negative tolerance");
   VH_CHECK_HIP(hipSetDevice(d->ctx->device));
   return vh_denoiser_apply_impl(d, m, tol);
+}
+
+extern "C" int vh_denoiser_set_inference(vh_denoiser *d, int fp32) {
+  VH_REQUIRE(d != nullptr, "vh_denoiser_set_inference: null denoiser");
+  VH_REQUIRE(fp32 == 0 || fp32 == 1, "vh_denoiser_set_inference: 0 (matrix cores) or 1 (FP32)");
+  d->fp32_infer = fp32;
+  if (d->prep_n > 0) VH_TRY(vh_denoiser_prepare(d, d->prep_n));
+  return VH_OK;
 }
 
 extern "C" int vh_denoiser_fallbacks(vh_denoiser *d, int32_t *count) {

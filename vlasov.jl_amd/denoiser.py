@@ -74,6 +74,11 @@ class Denoiser:
             _lib.call("vh_denoiser_get_weights", self.handle, w.ctypes.data, None)
         return w
 
+    def set_weights(self, weights):
+        """New master weights; the optimiser restarts."""
+        w = np.ascontiguousarray(weights, dtype=np.float32).ravel()
+        _lib.call("vh_denoiser_set_weights", self.handle, w, w.size)
+
     def eval(self, x, t, grad: bool = True):
         """One cube through the net: (o, loss, gradient or None)."""
         x = np.ascontiguousarray(x, dtype=np.float32)
@@ -124,6 +129,13 @@ class Denoiser:
         """The net on mesh.rho in place, with the residual charge check."""
         _lib.call("vh_denoiser_apply", self.handle, mesh.handle, float(tol))
 
+    def set_inference(self, path: str):
+        """"mfma" (default): bf16 activations on the matrix cores; "fp32":
+        the training kernels."""
+        if path not in ("mfma", "fp32"):
+            raise ValueError(f"unknown inference path {path!r}")
+        _lib.call("vh_denoiser_set_inference", self.handle, 1 if path == "fp32" else 0)
+
     @property
     def fallbacks(self) -> int:
         n = C.c_int32()
@@ -140,6 +152,27 @@ class Denoiser:
             self.close()
         except Exception:
             pass
+
+
+def training_family(count: int, npart: int = 8_000_000, nfine: int = 64, seed: int = 2024,
+                    bmax: float = 60.0, dt: float = 0.5, x0: float = -70.0):
+    """The collisions the pairs come from: Na196 at `npart` pseudo-particles,
+    the batching family's three projectiles in turn (protons at 100 keV,
+    Ar8+ at 200 keV, Xe25+ at 500 keV), impact parameters uniform in
+    [0, bmax] a0 from a seeded numpy generator."""
+    from .batch import KEV, PROJECTILES, Collision
+    from .physics import Projectile
+    from .simulation import xenon_parameters
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        b = float(rng.uniform(0.0, bmax))
+        name, mass, charge, kev, soft = PROJECTILES[i % len(PROJECTILES)]
+        proj = Projectile.of(mass=mass, charge=charge, energy=kev * KEV, impact=b, x0=x0, dt=dt,
+                             softening=soft())
+        out.append(Collision(xenon_parameters(npart, nfine, dt), proj, name, b))
+    return out
 
 
 def generate_pairs(den: Denoiser, collisions, steps: int = 700, every: int = 10, ratio: int = 10,
