@@ -412,7 +412,9 @@ int vh_sample_rejection(const double *grid, const double *potential, int ngrid,
  * 2x upsampling, skip connections by concatenation, ReLU) mapping the fine
  * density deposited by N pseudo-particles to the one 10 N deposit: the input
  * x = rho / rho0 (rho0 the jellium's mean density), the output
- * rho0 (x + o(x)).  Products use bf16 roundings of the weights with FP32
+ * rho0 (x + m o(x)) with m = [|x| > support] (default 1e-4,
+ * vh_denoiser_set_support): the net corrects the density where there is
+ * charge and moves none into the vacuum.  Products use bf16 roundings of the weights with FP32
  * accumulation; the optimiser (Adam) keeps FP32 master weights.
  *
  * Weights, vh_denoiser_param_count() floats: for each layer l = 0..10 in
@@ -435,7 +437,7 @@ int vh_denoiser_get_weights(vh_denoiser *d, float *w, float *weff);
 int vh_denoiser_set_weights(vh_denoiser *d, const float *w, int64_t nweights);
 /* One cube of side D (multiple of 4, <= 160) through the net: x, t the
  * scaled input and target ([z][y][x], D^3 each), o the correction (may be
- * NULL), *loss = mean((x + o - t)^2), grad (may be NULL) its gradient with
+ * NULL), *loss = mean((x + m o - t)^2), grad (may be NULL) its gradient with
  * respect to every parameter (the master layout). */
 int vh_denoiser_eval(vh_denoiser *d, int D, const float *x, const float *t, float *o,
                      double *loss, float *grad);
@@ -461,6 +463,9 @@ int vh_denoiser_dataset_error(vh_denoiser *d, int64_t k, int C, double *mse_plai
  * as deposited and the fallback count grows.  Asynchronous. */
 int vh_denoiser_apply(vh_denoiser *d, vh_mesh *m, double tol);
 int vh_denoiser_fallbacks(vh_denoiser *d, int32_t *count);
+/* the support threshold m = [|x| > xmin] of the correction (training and
+ * inference) */
+int vh_denoiser_set_support(vh_denoiser *d, double xmin);
 /* The inference path of vh_denoiser_apply: 0 (default) the matrix cores,
  * activations rounded to bf16 between layers (v_mfma_f32_16x16x32_bf16,
  * FP32 accumulation, output rho + rho0 o); 1 the FP32-activation kernels of

@@ -7,7 +7,7 @@ U-Net, 16 / 32 / 64 channels, 3^3 convolutions with zero padding, 2^3 mean
 pooling, nearest 2x upsampling, concatenated skips, ReLU), in numpy FP64:
 
   * forward(params, x)           the correction o and every activation,
-  * loss(params, x, t)           mean((x + o - t)^2),
+  * loss(params, x, t)           mean((x + m o - t)^2), m = [|x| > support],
   * gradient(params, x, t)       its gradient by reverse-mode differentiation,
   * fd_gradient(params, x, t, idx, h)  central finite differences.
 
@@ -137,15 +137,24 @@ def forward(params, x, act_bf16=False):
     return o, c
 
 
-def loss(params, x, t):
+SUPPORT = 1e-4   # the library's default support threshold
+
+
+def support_mask(x, support=SUPPORT):
+    """m = [|x| > support] in float32, as the device tests it."""
+    return np.abs(np.asarray(x, dtype=np.float32)) > np.float32(support)
+
+
+def loss(params, x, t, support=SUPPORT):
+    """mean((x + m o - t)^2): no correction outside the support."""
     o, _ = forward(params, x)
-    return float(np.mean((x + o - t) ** 2))
+    return float(np.mean((x + support_mask(x, support) * o - t) ** 2))
 
 
 PLANTS = ("SKIP_SWAP",)
 
 
-def gradient(params, x, t, plant=None):
+def gradient(params, x, t, plant=None, support=SUPPORT):
     """(loss, d loss / d params) by reverse-mode differentiation.  `plant`
     names a deliberate mistake, for the tests that show the checks catch
     it: "SKIP_SWAP" takes the level-1 skip's gradient from the upsampled
@@ -154,7 +163,8 @@ def gradient(params, x, t, plant=None):
         raise ValueError(f"unknown plant {plant!r}")
     L = unpack(params)
     o, c = forward(params, x)
-    r = x + o - t
+    m = support_mask(x, support)
+    r = x + m * o - t
     nv = r.size
     grads = [None] * len(LAYERS)
 
@@ -165,7 +175,7 @@ def gradient(params, x, t, plant=None):
         grads[l] = np.concatenate([dw.ravel(), db])
         return dx
 
-    d = (2.0 * r / nv)[..., None]
+    d = (2.0 * m * r / nv)[..., None]
     d = back(10, c["a9"], d, None)
     d = back(9, c["cat1"], d, c["a9"])
     da8 = up_back(d[..., :16])
@@ -184,16 +194,16 @@ def gradient(params, x, t, plant=None):
     return float(np.mean(r ** 2)), np.concatenate(grads)
 
 
-def fd_gradient(params, x, t, idx, h=1e-6):
+def fd_gradient(params, x, t, idx, h=1e-6, support=SUPPORT):
     """Central differences of the loss for the parameters `idx`."""
     p = np.array(params, dtype=np.float64)
     out = np.empty(len(idx))
     for k, i in enumerate(idx):
         keep = p[i]
         p[i] = keep + h
-        lp = loss(p, x, t)
+        lp = loss(p, x, t, support)
         p[i] = keep - h
-        lm = loss(p, x, t)
+        lm = loss(p, x, t, support)
         p[i] = keep
         out[k] = (lp - lm) / (2 * h)
     return out

@@ -154,7 +154,8 @@ def test_apply_and_residual_charge_check(ctx, orc):
         x = np.zeros((P, P, P))
         x[:n, :n, :n] = rho.astype(np.float64) / rho0
         o, _ = unet.forward(weff, x)
-        net = (rho0 * (x + o))[:n, :n, :n].ravel()
+        m = unet.support_mask(rho * np.float32(1.0 / rho0))
+        net = (rho0 * (x[:n, :n, :n] + m * o[:n, :n, :n])).ravel()
         q = orc.total_charge(rho.astype(np.float64).ravel(), mesh.dual_moments[0])
         qn = orc.total_charge(net, mesh.dual_moments[0])
         want = net * (q / qn)
@@ -202,8 +203,9 @@ PATHS = {
 @pytest.mark.parametrize("path", sorted(PATHS))
 def test_matrix_core_plumbing_exact(ctx, orc, path):
     """Single-channel routes through the network on a 0/1 input, where every
-    value is exact in bf16 (pooled means of 0/1 values): the matrix-core
-    inference equals the oracle's to the charge renormalisation's FP32
+    value is exact in bf16 (pooled means of 0/1 values), the correction kept
+    where the input is 1 (the support): the matrix-core inference equals the
+    oracle's to the charge renormalisation's FP32
     rounding (1e-6), so a wrong tap, channel group, concatenation half,
     pooling or upsampling index shows as an O(1) difference."""
     mesh = va.grids.SplineMesh.of(va.grids.uniform_axis(-40.0, 40.0, 14))
@@ -220,7 +222,8 @@ def test_matrix_core_plumbing_exact(ctx, orc, path):
         x[:n, :n, :n] = rho / rho0
         o, _ = unet.forward(p.astype(np.float64), x, act_bf16=True)
         assert np.array_equal(unet.bf16(o), o), "the route leaves bf16"
-        net = (rho.astype(np.float64) + rho0 * o[:n, :n, :n]).ravel()
+        m = unet.support_mask(rho * np.float32(1.0 / rho0))
+        net = (rho.astype(np.float64) + rho0 * m * o[:n, :n, :n]).ravel()
         q = orc.total_charge(rho.astype(np.float64).ravel(), mesh.dual_moments[0])
         qn = orc.total_charge(net, mesh.dual_moments[0])
         want = net * (q / qn)
@@ -263,7 +266,8 @@ def test_matrix_core_inference_against_oracle(ctx, orc, nfine):
         x32[:n, :n, :n] = rho * np.float32(1.0 / rho0)
         o, _ = unet.forward(weff, x32.astype(np.float64), act_bf16=True)
         corr = rho0 * np.abs(o).max()
-        net = (rho.astype(np.float64) + rho0 * o[:n, :n, :n]).ravel()
+        m = unet.support_mask(x32[:n, :n, :n])
+        net = (rho.astype(np.float64) + rho0 * m * o[:n, :n, :n]).ravel()
         q = orc.total_charge(rho.astype(np.float64).ravel(), mesh.dual_moments[0])
         qn = orc.total_charge(net, mesh.dual_moments[0])
         want = net * (q / qn)

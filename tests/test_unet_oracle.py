@@ -128,3 +128,24 @@ def test_planted_skip_swap_fails_the_difference_check():
         _, g = unet.gradient(p, x, t, plant=plant)
         errs[plant] = np.max(np.abs(fd - g[idx])) / np.max(np.abs(g[off:off + unet.layer_sizes()[1]]))
     assert errs[None] < 1e-5 and errs["SKIP_SWAP"] > 1e-2
+
+
+def test_support_mask_removes_the_correction_outside():
+    """Where |x| <= support the loss does not see the net: its gradient is
+    the finite differences' with half the input zeroed, and a network with
+    only a last-layer bias changes the loss only inside the support."""
+    p = unet.random_params(40)
+    rng = np.random.default_rng(41)
+    x = rng.standard_normal((4, 4, 4))
+    x[:, :2] = 0.0
+    t = rng.standard_normal((4, 4, 4))
+    l, g = unet.gradient(p, x, t)
+    idx = list(rng.choice(unet.param_count(), size=12, replace=False))
+    fd = unet.fd_gradient(p, x, t, idx)
+    assert np.max(np.abs(fd - g[idx])) < 1e-5 * np.max(np.abs(g))
+    q = np.zeros(unet.param_count())
+    q[-1] = 0.5                           # o = 0.5 everywhere
+    inside = unet.support_mask(x)
+    want = np.mean((x + 0.5 * inside - t) ** 2)
+    assert np.isclose(unet.loss(q, x, t), want, rtol=1e-14)
+    assert inside.sum() == 32

@@ -108,6 +108,7 @@ _SIGNATURES = {
     "vh_denoiser_apply": [_vp, _vp, C.c_double],
     "vh_denoiser_fallbacks": [_vp, C.POINTER(C.c_int32)],
     "vh_denoiser_set_inference": [_vp, C.c_int],
+    "vh_denoiser_set_support": [_vp, C.c_double],
     "vh_accel_set_denoiser": [_vp, _vp, C.c_double],
     "vh_interaction_energy": [_vp, _vp, _vp, C.c_double, C.c_double, C.POINTER(C.c_double)],
     "vh_profile_enable": [_vp, C.c_int],

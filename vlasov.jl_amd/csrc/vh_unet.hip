@@ -16,7 +16,9 @@
 //   avg-pool 2^3;  L4 32 -> 64, ReLU   L5 64 -> 64, ReLU          level 3 (D/4)
 //   L6 64 -> 32, ReLU at level 3, nearest 2x up, concat [up, L3] -> L7 64 -> 32, ReLU
 //   L8 32 -> 16, ReLU at level 2, nearest 2x up, concat [up, L1] -> L9 32 -> 16, ReLU
-//   L10 16 -> 1 (no activation): the correction o, and y = x + o.
+//   L10 16 -> 1 (no activation): the correction o, and y = x + m o with
+//   m = [|x| > support]: no correction where the deposited density is
+//   (nearly) zero, so that the net moves no charge into the vacuum.
 // x = rho / rho0 with rho0 the jellium's mean density (the input scale).
 #include "vh_internal.h"
 
@@ -242,17 +244,19 @@ __global__ void k_relu_back(const float *__restrict__ act, int lda, int ac0, flo
   }
 }
 
-// y = x + o; loss partial sums of (y - t)^2 (FP64, per block), do = 2 (y - t) / nv
+// y = x + m o (m = [|x| > support]); loss partial sums of (y - t)^2 (FP64,
+// per block), do = 2 m (y - t) / nv
 __global__ void __launch_bounds__(256)
     k_loss(const float *__restrict__ x, const float *__restrict__ o, const float *__restrict__ t,
-           int64_t nv, float *__restrict__ dout, double *__restrict__ part) {
+           int64_t nv, float support, float *__restrict__ dout, double *__restrict__ part) {
   __shared__ double lds[8];
   double s = 0.0;
   for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < nv;
        v += (int64_t)gridDim.x * blockDim.x) {
-    const float r = (x[v] + o[v]) - t[v];
+    const bool m = fabsf(x[v]) > support;
+    const float r = (m ? x[v] + o[v] : x[v]) - t[v];
     s += (double)r * r;
-    if (dout) dout[v] = 2.f * r / (float)nv;
+    if (dout) dout[v] = m ? 2.f * r / (float)nv : 0.f;
   }
   const double b = vh_block_sum_d(s, lds);
   if (threadIdx.x == 0) part[blockIdx.x] = b;
@@ -320,15 +324,15 @@ __global__ void k_pad_in(const float *__restrict__ rho, int n, int P, float inv,
   }
 }
 
-// rho_out = rho0 (x + o) on the n^3 grid (into a work grid)
+// rho_out = rho0 (x + m o) on the n^3 grid (into a work grid)
 __global__ void k_unpad_out(const float *__restrict__ x, const float *__restrict__ o, int n, int P,
-                            float rho0, float *__restrict__ out) {
+                            float rho0, float support, float *__restrict__ out) {
   const int64_t nv = (int64_t)n * n * n;
   for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < nv;
        v += (int64_t)gridDim.x * blockDim.x) {
     const int i = (int)(v % n), j = (int)((v / n) % n), k = (int)(v / ((int64_t)n * n));
     const int64_t p = ((int64_t)k * P + j) * P + i;
-    out[v] = rho0 * (x[p] + o[p]);
+    out[v] = rho0 * (fabsf(x[p]) > support ? x[p] + o[p] : x[p]);
   }
 }
 
@@ -434,6 +438,7 @@ struct conv_args {
   const float *rho;            // CM_FIRST, CM_LAST: the n^3 density
   int n;
   float inv, rho0;             // 1 / rho0, rho0
+  float support;               // CM_LAST: no correction where |rho / rho0| <= support
   const un_u4 *wp;             // packed B operand
   const float *bias;
   unsigned short *out;         // CM_HIDDEN: bf16 tensor of COUT channels (Ey, Ex geometry)
@@ -550,7 +555,8 @@ __global__ void __launch_bounds__(64 * BZ) k_conv_mfma(conv_args a) {
       if (MODE == CM_LAST) {
         if (lc == 0 && z < a.n && y < a.n && x < a.n) {
           const int64_t g = ((int64_t)z * a.n + y) * a.n + x;
-          a.net[g] = a.rho[g] + a.rho0 * (acc[mt][0][v] + a.bias[0]);
+          const float r = a.rho[g];
+          a.net[g] = fabsf(r * a.inv) > a.support ? r + a.rho0 * (acc[mt][0][v] + a.bias[0]) : r;
         }
       } else if (z < a.D && y < a.D && x < a.D) {
         const int64_t ov = ((int64_t)(z + 1) * a.Ey + y + 1) * a.Ex + x + 1;
@@ -646,6 +652,7 @@ struct vh_denoiser {
                  *t_p2 = nullptr, *t_a4 = nullptr, *t_a5 = nullptr, *t_a6 = nullptr, *t_a7 = nullptr,
                  *t_a8 = nullptr, *t_a9 = nullptr;
   int fp32_infer = 0;         // vh_denoiser_set_inference
+  float support = 1e-4f;      // vh_denoiser_set_support
   int prep_n = 0;             // the grid the inference buffers were made for
 };
 
@@ -712,6 +719,7 @@ static int launch_conv(vh_denoiser *d, hipStream_t st, int l, int level, const u
   a.n = n;
   a.inv = (float)(1.0 / d->rho0);
   a.rho0 = (float)d->rho0;
+  a.support = d->support;
   a.wp = d->packed + d->poff[l];
   a.bias = d->weff + woff(l) + wsize(l);
   a.out = out;
@@ -888,7 +896,8 @@ static int forward_backward(vh_denoiser *d, vh_unet_buffers &b, bool grads, doub
   const int D = b.D, D2 = D / 2, D4 = D / 4;
   const int64_t v1 = (int64_t)D * D * D, v2 = v1 / 8, v3 = v2 / 8;
   // loss and dO (into g1a, one channel)
-  hipLaunchKernelGGL(k_loss, dim3(512), dim3(256), 0, st, b.x, b.o, b.t, v1, grads ? b.g1a : nullptr, d->lpart);
+  hipLaunchKernelGGL(k_loss, dim3(512), dim3(256), 0, st, b.x, b.o, b.t, v1, d->support,
+                     grads ? b.g1a : nullptr, d->lpart);
   VH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_sum_d, dim3(1), dim3(256), 0, st, d->lpart, (int64_t)512, loss);
   VH_LAUNCH_CHECK();
@@ -1235,7 +1244,8 @@ extern "C" int vh_denoiser_dataset_error(vh_denoiser *d, int64_t k, int C, doubl
                      (float)(1.0 / d->rho0), b.x, b.t);
   VH_LAUNCH_CHECK();
   VH_CHECK_HIP(hipMemsetAsync(b.o, 0, sizeof(float) * nv, st));
-  hipLaunchKernelGGL(k_loss, dim3(512), dim3(256), 0, st, b.x, b.o, b.t, nv, (float *)nullptr, d->lpart);
+  hipLaunchKernelGGL(k_loss, dim3(512), dim3(256), 0, st, b.x, b.o, b.t, nv, d->support,
+                     (float *)nullptr, d->lpart);
   hipLaunchKernelGGL(k_sum_d, dim3(1), dim3(256), 0, st, d->lpart, (int64_t)512, d->dscal + 3);
   VH_LAUNCH_CHECK();
   VH_TRY(forward_backward(d, b, false, d->dscal));
@@ -1266,7 +1276,7 @@ int vh_denoiser_apply_impl(vh_denoiser *d, vh_mesh *m, double tol) {
     VH_LAUNCH_CHECK();
     VH_TRY(forward(d, b));
     hipLaunchKernelGGL(k_unpad_out, dim3(blocks_for(g)), dim3(256), 0, st, b.x, b.o, n, P,
-                       (float)d->rho0, d->net);
+                       (float)d->rho0, d->support, d->net);
     VH_LAUNCH_CHECK();
   } else {
     VH_TRY(forward_mfma(d, st, m->rho, n, d->net));
@@ -1314,6 +1324,13 @@ extern "C" int vh_denoiser_set_inference(vh_denoiser *d, int fp32) {
   VH_REQUIRE(fp32 == 0 || fp32 == 1, "vh_denoiser_set_inference: 0 (matrix cores) or 1 (FP32)");
   d->fp32_infer = fp32;
   if (d->prep_n > 0) VH_TRY(vh_denoiser_prepare(d, d->prep_n));
+  return VH_OK;
+}
+
+extern "C" int vh_denoiser_set_support(vh_denoiser *d, double xmin) {
+  VH_REQUIRE(d != nullptr, "vh_denoiser_set_support: null denoiser");
+  VH_REQUIRE(xmin >= 0 && std::isfinite(xmin), "vh_denoiser_set_support: threshold must be >= 0");
+  d->support = (float)xmin;
   return VH_OK;
 }
 

@@ -136,6 +136,10 @@ class Denoiser:
             raise ValueError(f"unknown inference path {path!r}")
         _lib.call("vh_denoiser_set_inference", self.handle, 1 if path == "fp32" else 0)
 
+    def set_support(self, xmin: float):
+        """No correction where |rho / rho0| <= xmin (default 1e-4)."""
+        _lib.call("vh_denoiser_set_support", self.handle, float(xmin))
+
     @property
     def fallbacks(self) -> int:
         n = C.c_int32()
