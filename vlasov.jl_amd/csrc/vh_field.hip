@@ -972,9 +972,12 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
   }
   // One block per work unit up to 65536 units (the dispatcher then balances
   // units of different weights); beyond, blocks stride over units.  Four
-  // FP64 partials per block fit the 8 x dpart_len buffer.
+  // FP64 partials per block and the reduction's 4 x FIELD_SLICES slice sums
+  // fit the 8 x dpart_len buffer.
+  constexpr int FIELD_SLICES = 8;
   int grid = (int)(a->maxunits < 65536 ? a->maxunits : 65536);
-  if (4 * (int64_t)grid > 8 * a->dpart_len) grid = (int)(2 * a->dpart_len);
+  if (4 * (int64_t)grid + 4 * FIELD_SLICES > 8 * a->dpart_len)
+    grid = (int)(2 * a->dpart_len - FIELD_SLICES);
   int tok = vh_prof_begin(c, VH_ST_FIELD_KERNEL);
   if (a->exact_field)
     hipLaunchKernelGGL(k_field, dim3(grid), dim3(64), 0, st, a->parts[a->cur], a->npad,
@@ -991,7 +994,7 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
   {
     // 65536 partials of 4 columns: slices over 32 blocks, then their sums
     // (4 blocks alone took 13 us); the slices' sums go after the partials
-    constexpr int nsl = 8;
+    constexpr int nsl = FIELD_SLICES;
     double *sl = a->dpart + 4 * (int64_t)grid;
     VH_REQUIRE(4 * (int64_t)grid + 4 * nsl <= 8 * a->dpart_len, "forces: partials beyond the buffer");
     hipLaunchKernelGGL(k_reduce_slices, dim3(nsl, 4), dim3(1024), 0, st, a->dpart, grid, sl);

@@ -787,19 +787,23 @@ extern "C" int vh_multipole_noalloc(int device, void *stream, const float *rho, 
 }
 
 // total charge sum(rho m0x m0y m0z) in FP64 (src/deposition.jl:213-257)
+// a wave per row (j, k) of the grid, its lanes along i (coalesced): the
+// row's sum over i in a fixed lane order, times m0[j] m0[k]
 __global__ void __launch_bounds__(256)
     k_charge(const float *__restrict__ rho, const double *__restrict__ m0, int n,
              double *__restrict__ part) {
   __shared__ double lds[8];
   double acc = 0.0;
   const int64_t njk = (int64_t)n * n;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < njk;
-       t += (int64_t)gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t t = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); t < njk; t += nw) {
     const int j = (int)(t % n), k = (int)(t / n);
+    const float *row = rho + t * n;
     double s = 0.0;
-    const float *col = rho + t * n;
-    for (int i = 0; i < n; ++i) s += (double)col[i] * m0[i];
-    acc += s * m0[j] * m0[k];
+    for (int i = lane; i < n; i += 64) s += (double)row[i] * m0[i];
+    s = vh_wave_sum(s);
+    if (lane == 0) acc += s * m0[j] * m0[k];
   }
   const double s = vh_block_sum_d(acc, lds);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
