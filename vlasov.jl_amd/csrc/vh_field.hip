@@ -822,19 +822,72 @@ __device__ __forceinline__ int nearest_knot_d(const double *__restrict__ g, int 
   return (g[i] - x) < (x - g[i - 1]) ? i : i - 1;
 }
 
-// One lane per particle, in the buffer's current order (sorted by cell after
-// the step's sort, so neighbouring lanes read the same csol rows).  Inside
-// (lo, hi)^3: smoothed_potential (src/fields.jl:405-428) of the fine csol;
-// elsewhere spline_potential (src/fields.jl:440-466) of the coarse csol, or
-// enclosed / |r| beyond both grids.  Every product and sum in the reference's
-// order, in FP64, on the FP32 coefficients widened exactly.
+// The potential at one point p (FP64): inside (lo, hi)^3 smoothed_potential
+// (src/fields.jl:405-428) of the fine csol; elsewhere spline_potential
+// (src/fields.jl:440-466) of the coarse csol, or enclosed / |r| beyond both
+// grids.  Every product and sum in the reference's order, in FP64, on the
+// FP32 coefficients widened exactly.
+__device__ double energy_point(const double (&p)[3], bool inside, const float *__restrict__ csolf,
+                               const double *__restrict__ kf, const double *__restrict__ ovl,
+                               const float *__restrict__ csolc, const double *__restrict__ kc,
+                               const vh_energy_args &ea) {
+  const int64_t nf = ea.nf, nc = ea.nc;
+  double phi = 0.0;
+  if (inside) {
+    int64_t b[3];
+    const double *o[3];
+    for (int d = 0; d < 3; ++d) {
+      const int ci = nearest_knot_d(kf, ea.nkf, p[d]);
+      b[d] = 2 * ci - 4;  // 0-based 2ci - 5
+      int col = (int)floor((p[d] - kf[ci] + ea.spacing / 2) / ea.spacing * ea.nbdt + 0.5) + 1;
+      col = col < 1 ? 1 : (col > ea.ncol ? ea.ncol : col);
+      o[d] = ovl + 10 * (col - 1);
+    }
+    for (int kk = 0; kk < 10; ++kk) {
+      const double oz = o[2][kk];
+      for (int jj = 0; jj < 10; ++jj) {
+        const double wgt = o[1][jj] * oz;
+        const float *row = csolf + b[0] + nf * ((b[1] + jj) + nf * (b[2] + kk));
+        double s = 0.0;
+#pragma unroll
+        for (int ii = 0; ii < 10; ++ii) s += (double)row[ii] * o[0][ii];
+        phi += s * wgt;
+      }
+    }
+  } else {
+    int cell[3];
+    bool out = false;
+    for (int d = 0; d < 3; ++d) {
+      cell[d] = cell_index_d(kc, ea.nkc, p[d]);
+      out = out || cell[d] < 0;
+    }
+    if (out) {
+      phi = ea.enclosed / sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    } else {
+      double v[3][4], dv;
+      for (int d = 0; d < 3; ++d)
+        for (int a = 0; a < 4; ++a) hermite_vd(kc, ea.nkc, 2 * cell[d] + a, p[d], v[d][a], dv);
+      for (int ak = 0; ak < 4; ++ak) {
+        const int64_t k = 2 * cell[2] + ak;
+        for (int aj = 0; aj < 4; ++aj) {
+          const int64_t j = 2 * cell[1] + aj;
+          const double wgt = v[1][aj] * v[2][ak];
+          for (int ai = 0; ai < 4; ++ai)
+            phi += (double)csolc[2 * cell[0] + ai + nc * (j + nc * k)] * v[0][ai] * wgt;
+        }
+      }
+    }
+  }
+  return phi;
+}
+
+// One lane per particle, in the buffer's current order (before any sort)
 __global__ void __launch_bounds__(256)
     k_energy(const int32_t *__restrict__ P, int64_t npad, int64_t npart,
              const float *__restrict__ csolf, const double *__restrict__ kf,
              const double *__restrict__ ovl, const float *__restrict__ csolc,
              const double *__restrict__ kc, vh_energy_args ea, double *__restrict__ part) {
   __shared__ double lds[8];
-  const int64_t nf = ea.nf, nc = ea.nc;
   double total = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npart;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -845,59 +898,211 @@ __global__ void __launch_bounds__(256)
       p[d] = ea.x0 + (double)(rp.k[d] - 1) * ea.h + (double)rp.d[d];
       inside = inside && ea.lo < p[d] && p[d] < ea.hi;
     }
-    double phi = 0.0;
-    if (inside) {
-      int64_t b[3];
-      const double *o[3];
-      for (int d = 0; d < 3; ++d) {
-        const int ci = nearest_knot_d(kf, ea.nkf, p[d]);
-        b[d] = 2 * ci - 4;  // 0-based 2ci - 5
-        int col = (int)floor((p[d] - kf[ci] + ea.spacing / 2) / ea.spacing * ea.nbdt + 0.5) + 1;
-        col = col < 1 ? 1 : (col > ea.ncol ? ea.ncol : col);
-        o[d] = ovl + 10 * (col - 1);
-      }
-      for (int kk = 0; kk < 10; ++kk) {
-        const double oz = o[2][kk];
-        for (int jj = 0; jj < 10; ++jj) {
-          const double wgt = o[1][jj] * oz;
-          const float *row = csolf + b[0] + nf * ((b[1] + jj) + nf * (b[2] + kk));
-          double s = 0.0;
-#pragma unroll
-          for (int ii = 0; ii < 10; ++ii) s += (double)row[ii] * o[0][ii];
-          phi += s * wgt;
-        }
-      }
-    } else {
-      int cell[3];
-      bool out = false;
-      for (int d = 0; d < 3; ++d) {
-        cell[d] = cell_index_d(kc, ea.nkc, p[d]);
-        out = out || cell[d] < 0;
-      }
-      if (out) {
-        phi = ea.enclosed / sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
-      } else {
-        double v[3][4], dv;
-        for (int d = 0; d < 3; ++d)
-          for (int a = 0; a < 4; ++a) hermite_vd(kc, ea.nkc, 2 * cell[d] + a, p[d], v[d][a], dv);
-        for (int ak = 0; ak < 4; ++ak) {
-          const int64_t k = 2 * cell[2] + ak;
-          for (int aj = 0; aj < 4; ++aj) {
-            const int64_t j = 2 * cell[1] + aj;
-            const double wgt = v[1][aj] * v[2][ak];
-            for (int ai = 0; ai < 4; ++ai)
-              phi += (double)csolc[2 * cell[0] + ai + nc * (j + nc * k)] * v[0][ai] * wgt;
-          }
-        }
-      }
-    }
-    total += ea.w * phi;
+    total += ea.w * energy_point(p, inside, csolf, kf, ovl, csolc, kc, ea);
   }
   const double s = vh_block_sum_d(total, lds);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// The same sum over the work units of the last sort (a wave per unit, the
+// slots of one cell), on the matrix cores in FP64.  A particle's cell is
+// its nearest knot c0 (0-based) on each axis, so nearest_knot_d gives c0
+// for all but the particles moved off their cell since the sort (decided
+// here by the same comparisons, between the knot's neighbours), and those
+// use the cell's 10^3 block of csol, starting at 2 c0 - 4.  The first stage
+//   T[p, (jj, kk)] = sum_ii c[ii, jj, kk] ox[p, ii]
+// is a product of the block (rows x ii, the cell's, in registers) by 16
+// particles' x weights, v_mfma_f64_16x16x4f64 (lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16] and receives D[l / 16 + 4 v][l % 16],
+// v = 0..3), then the y and z sums per particle on the VALU; the rows of each
+// lane group are the field kernel's (slot q = 4 t + v of group g: planes g
+// and g + 4, and half of plane 8 + g / 2), 25 rows in 28 slots.  The other
+// particles take energy_point.  Every load is unconditional (a conditional
+// one becomes a branch that waits for every load in flight), its value
+// selected afterwards.  FP64 throughout; the sums run in another order than
+// the reference's chain (agreement to FP64 rounding).
+#define EN_STRIDE 33   // doubles a particle row: ox, oy, oz, flag (odd stride)
+
+__global__ void __launch_bounds__(64)
+    k_energy_units(const int32_t *__restrict__ P, const vh_unit *__restrict__ units,
+                   const int32_t *__restrict__ cnt, int nk, const float *__restrict__ csolf,
+                   const double *__restrict__ kf, const double *__restrict__ ovl,
+                   const float *__restrict__ csolc, const double *__restrict__ kc,
+                   vh_energy_args ea, double *__restrict__ part) {
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  __shared__ double tab[64 * EN_STRIDE];
+  // the particles that take energy_point, queued so that they run 64 at a
+  // time (a lone one would hold the whole wave through it): position and
+  // the inside flag, in the order the wave meets them
+  __shared__ double qp[128][4];
+  int qn = 0;
+  const int l = threadIdx.x, j16 = l & 15, g = l >> 4;
+  const int nunits = cnt[CNT_NUNITS];
+  const int64_t nf = ea.nf;
+  const double half = ea.spacing / 2;
+  double total = 0.0;
+  for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const vh_unit un = units[u];
+    const int kc3[3] = {un.key % nk, (un.key / nk) % nk, un.key / (nk * nk)};
+    bool okw = true;
+    double gm[3], g0[3], gp[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int c0 = kc3[d];
+      okw = okw && c0 >= 2 && c0 + 1 <= ea.nkf - 1 && 2 * c0 + 5 <= ea.nf - 1;
+      const int cz = c0 < ea.nkf - 1 ? c0 : ea.nkf - 2;
+      gm[d] = kf[cz >= 1 ? cz - 1 : 0];
+      g0[d] = kf[cz];
+      gp[d] = kf[cz + 1];
+    }
+    // A operands: row j16 of tile t is slot q = 4 t + j16 / 4 of lane group
+    // j16 % 4, K index ii = 4 s + g; padding (q >= 25, ii >= 10) reads an
+    // in-window point and keeps 0
+    double A[7][3];
+    {
+      const int gg = j16 & 3;
+      const float *tb = csolf + (okw ? (2 * kc3[0] - 4) + nf * ((2 * kc3[1] - 4) + nf * (2 * kc3[2] - 4)) : 0);
+      float cv[7][3];
+#pragma unroll
+      for (int t = 0; t < 7; ++t) {
+        const int q0 = 4 * t + (j16 >> 2), q = q0 < 25 ? q0 : 0;
+        const int jj = q < 10 ? q : q < 20 ? q - 10 : q - 20 + 5 * (gg & 1);
+        const int kk = q < 10 ? gg : q < 20 ? gg + 4 : 8 + (gg >> 1);
+#pragma unroll
+        for (int s2 = 0; s2 < 3; ++s2) {
+          const int ii0 = 4 * s2 + g, ii = ii0 < 10 ? ii0 : 0;
+          cv[t][s2] = tb[ii + nf * (jj + nf * kk)];
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 7; ++t)
+#pragma unroll
+        for (int s2 = 0; s2 < 3; ++s2)
+          A[t][s2] = (okw && 4 * t + (j16 >> 2) < 25 && 4 * s2 + g < 10) ? (double)cv[t][s2] : 0.0;
+    }
+    for (int base = un.lo; base < un.hi; base += 64) {
+      const int nlive = min(64, un.hi - base);
+      const bool live = l < nlive;
+      double p[3];
+      bool inside = live;
+      const vh_pos rp = vh_load_pos(P, live ? base + l : base);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        p[d] = ea.x0 + (double)(rp.k[d] - 1) * ea.h + (double)rp.d[d];
+        inside = inside && ea.lo < p[d] && p[d] < ea.hi;
+      }
+      // nearest_knot_d between the knot's neighbours: searchsortedlast is
+      // c0 or c0 + 1 there
+      bool fast = inside && okw;
+      int col[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const bool near = gm[d] <= p[d] && p[d] < gp[d];
+        const bool right = g0[d] <= p[d];          // searchsortedlast = c0 + 1
+        const double gi = right ? gp[d] : g0[d], gi1 = right ? g0[d] : gm[d];
+        const bool upper = (gi - p[d]) < (p[d] - gi1);
+        // the nearest knot is c0 when it is i - 1 with i = c0 + 1, or i with i = c0
+        fast = fast && near && (right ? !upper : upper);
+        const int cc = (int)floor((p[d] - g0[d] + half) / ea.spacing * ea.nbdt + 0.5) + 1;
+        col[d] = (cc < 1 ? 1 : (cc > ea.ncol ? ea.ncol : cc)) - 1;
+      }
+      {
+        const bool slow = live && !fast;
+        const uint64_t m = __ballot(slow);
+        const int at = qn + __popcll(m & ((1ull << l) - 1ull));
+        if (slow) {
+          qp[at][0] = p[0];
+          qp[at][1] = p[1];
+          qp[at][2] = p[2];
+          qp[at][3] = inside ? 1.0 : 0.0;
+        }
+        qn += __popcll(m);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        if (qn >= 64) {   // 64 queued: a full wave of them
+          const double q3[3] = {qp[l][0], qp[l][1], qp[l][2]};
+          total += ea.w * energy_point(q3, qp[l][3] != 0.0, csolf, kf, ovl, csolc, kc, ea);
+          double keep[4] = {0.0, 0.0, 0.0, 0.0};
+          if (l < qn - 64)
+            for (int c = 0; c < 4; ++c) keep[c] = qp[64 + l][c];
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+          if (l < qn - 64)
+            for (int c = 0; c < 4; ++c) qp[l][c] = keep[c];
+          qn -= 64;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        }
+      }
+      double ov[3][10];
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int a = 0; a < 10; ++a) ov[d][a] = ovl[10 * col[d] + a];
+      double *row = tab + l * EN_STRIDE;
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int a = 0; a < 10; ++a) row[10 * d + a] = fast ? ov[d][a] : 0.0;
+      row[30] = fast ? 1.0 : 0.0;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      const int nsg = (nlive + 15) >> 4;
+      for (int sg = 0; sg < nsg; ++sg) {
+        const double *pr = tab + (16 * sg + j16) * EN_STRIDE;
+        double b[3], y[10], yh[5], z[3];
+#pragma unroll
+        for (int s2 = 0; s2 < 3; ++s2) {
+          const int i0 = 4 * s2 + g;
+          const double v = pr[i0 < 10 ? i0 : 0];
+          b[s2] = i0 < 10 ? v : 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < 10; ++a) y[a] = pr[10 + a];
+#pragma unroll
+        for (int a = 0; a < 5; ++a) yh[a] = pr[10 + 5 * (g & 1) + a];
+        z[0] = pr[20 + g];
+        z[1] = pr[24 + g];
+        z[2] = pr[28 + (g >> 1)];
+        const double flag = pr[30];
+        double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+#pragma unroll
+        for (int t = 0; t < 7; ++t) {
+          d4 T = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int s2 = 0; s2 < 3; ++s2) T = __builtin_amdgcn_mfma_f64_16x16x4f64(A[t][s2], b[s2], T, 0, 0, 0);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int q = 4 * t + v;
+            if (q < 10) u0 += y[q] * T[v];
+            else if (q < 20) u1 += y[q - 10] * T[v];
+            else if (q < 25) u2 += yh[q - 20] * T[v];
+          }
+        }
+        double phi = (z[0] * u0 + z[1] * u1) + z[2] * u2;
+        phi += __shfl_xor(phi, 16, 64);
+        phi += __shfl_xor(phi, 32, 64);
+        if (g == 0 && flag != 0.0) total += ea.w * phi;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+  }
+  if (qn > 0 && l < qn) {   // the rest of the queue
+    const double q3[3] = {qp[l][0], qp[l][1], qp[l][2]};
+    total += ea.w * energy_point(q3, qp[l][3] != 0.0, csolf, kf, ovl, csolc, kc, ea);
+  }
+  total = vh_wave_sum(total);
+  if (l == 0) part[blockIdx.x] = total;
+}
+
 #define ENERGY_BLOCKS 2048
+#define ENERGY_UNIT_BLOCKS 8192   // waves striding over the units (partials fit 8 dpart_len)
 
 int vh_energy_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, double enclosed) {
   vh_context *c = a->ctx;
@@ -916,11 +1121,20 @@ int vh_energy_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, d
   ea.nc = coarse->n;
   ea.ncol = a->ncol;
   VH_TRY(vh_rec_settle(a));
-  hipLaunchKernelGGL(k_energy, dim3(ENERGY_BLOCKS), dim3(256), 0, c->stream, a->parts[a->cur],
-                     a->npad, a->npart, fine->csol, a->knots64, a->ovl64, coarse->csol,
-                     coarse->knots64, ea, a->dpart);
+  int blocks = ENERGY_BLOCKS;
+  if (a->sorted_once && fine->n == a->n) {
+    // the work units of the last sort partition the slots: a wave per unit
+    blocks = ENERGY_UNIT_BLOCKS;
+    hipLaunchKernelGGL(k_energy_units, dim3(blocks), dim3(64), 0, c->stream, a->parts[a->cur],
+                       a->units, a->counters, a->nk, fine->csol, a->knots64, a->ovl64,
+                       coarse->csol, coarse->knots64, ea, a->dpart);
+  } else {
+    hipLaunchKernelGGL(k_energy, dim3(blocks), dim3(256), 0, c->stream, a->parts[a->cur],
+                       a->npad, a->npart, fine->csol, a->knots64, a->ovl64, coarse->csol,
+                       coarse->knots64, ea, a->dpart);
+  }
   VH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, c->stream, a->dpart, ENERGY_BLOCKS, 1,
+  hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(256), 0, c->stream, a->dpart, blocks, 1,
                      a->dscal + 18);
   VH_LAUNCH_CHECK();
   return VH_OK;
