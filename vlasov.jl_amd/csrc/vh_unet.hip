@@ -462,33 +462,66 @@ __global__ void __launch_bounds__(64 * BZ) k_conv_mfma(conv_args a) {
   if (brick >= nb) return;
   const int bx = brick % a.nbx, by = (brick / a.nbx) % a.nby, bz = brick / (a.nbx * a.nby);
   const int x0 = 8 * bx, y0 = 8 * by, z0 = BZ * bz;
-  // stage the brick and its halo
-  if (MODE == CM_FIRST) {
+  // stage the brick and its halo.  Every thread's loads are issued
+  // unconditionally (from a valid address, the value selected afterwards)
+  // and in batches before their LDS writes: a load under a branch or in a
+  // rolled loop is waited for on its own.
+  constexpr int NTH = 64 * BZ;
+  if constexpr (MODE == CM_FIRST) {
     float *f = (float *)sm;
-    for (int e = threadIdx.x; e < HV; e += blockDim.x) {
+    constexpr int NE = (HV + NTH - 1) / NTH;
+    float v[NE];
+    bool in[NE];
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+      const int e0 = threadIdx.x + k * NTH, e = e0 < HV ? e0 : 0;
       const int hx = e % 10, hy = (e / 10) % 10, hz = e / 100;
       const int z = z0 + hz - 1, y = y0 + hy - 1, x = x0 + hx - 1;
-      const bool in = (unsigned)z < (unsigned)a.n && (unsigned)y < (unsigned)a.n && (unsigned)x < (unsigned)a.n;
-      f[e] = in ? a.rho[((int64_t)z * a.n + y) * a.n + x] * a.inv : 0.f;
+      in[k] = (unsigned)z < (unsigned)a.n && (unsigned)y < (unsigned)a.n && (unsigned)x < (unsigned)a.n;
+      v[k] = a.rho[in[k] ? ((int64_t)z * a.n + y) * a.n + x : 0];
+    }
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+      const int e0 = threadIdx.x + k * NTH;
+      if (e0 < HV) f[e0] = in[k] ? v[k] * a.inv : 0.f;
     }
   } else {
     constexpr int CH = CIN / 8;
-    for (int e = threadIdx.x; e < HV * CH; e += blockDim.x) {
-      const int hv = e / CH, c = 8 * (e - hv * CH);
-      const int hx = hv % 10, hy = (hv / 10) % 10, hz = hv / 100;
-      const int z = z0 + hz - 1, y = y0 + hy - 1, x = x0 + hx - 1;
-      un_u4 v = (un_u4){0u, 0u, 0u, 0u};
-      if (CA > 0 && c < CA) {
-        const int Df = a.D;
-        if ((unsigned)z < (unsigned)Df && (unsigned)y < (unsigned)Df && (unsigned)x < (unsigned)Df) {
-          const int64_t cv = ((int64_t)((z >> 1) + 1) * a.EyA + (y >> 1) + 1) * a.ExA + (x >> 1) + 1;
-          v = *(const un_u4 *)(a.srcA + cv * CA + c);
-        }
-      } else {
+    constexpr int NE = (HV * CH + NTH - 1) / NTH;
+    constexpr int NB = NE < 12 ? NE : 12;   // loads in flight a batch
+#pragma unroll
+    for (int k0 = 0; k0 < NE; k0 += NB) {
+      un_u4 v[NB];
+      bool zr[NB];
+#pragma unroll
+      for (int k = 0; k < NB; ++k) {
+        const int e0 = threadIdx.x + (k0 + k) * NTH;
+        const int e = (k0 + k < NE && e0 < HV * CH) ? e0 : 0;
+        const int hv = e / CH, c = 8 * (e - hv * CH);
+        const int hx = hv % 10, hy = (hv / 10) % 10, hz = hv / 100;
+        const int z = z0 + hz - 1, y = y0 + hy - 1, x = x0 + hx - 1;
         const int64_t sv = ((int64_t)(z + 1) * a.Ey + y + 1) * a.Ex + x + 1;
-        v = *(const un_u4 *)(a.srcB + sv * CB + (c - CA));
+        const unsigned short *src = a.srcB + sv * CB + (c - CA);
+        bool zero = false;
+        if (CA > 0) {
+          const bool up = c < CA;
+          const bool inD = (unsigned)z < (unsigned)a.D && (unsigned)y < (unsigned)a.D &&
+                           (unsigned)x < (unsigned)a.D;
+          const int64_t cv = inD ? ((int64_t)((z >> 1) + 1) * a.EyA + (y >> 1) + 1) * a.ExA + (x >> 1) + 1 : 0;
+          src = up ? a.srcA + cv * CA + c : src;
+          zero = up && !inD;
+        }
+        v[k] = *(const un_u4 *)src;
+        zr[k] = zero;
       }
-      *(un_u4 *)(sm + hv * SV + c) = v;
+#pragma unroll
+      for (int k = 0; k < NB; ++k) {
+        const int e0 = threadIdx.x + (k0 + k) * NTH;
+        if (k0 + k < NE && e0 < HV * CH) {
+          const int hv = e0 / CH, c = 8 * (e0 - hv * CH);
+          *(un_u4 *)(sm + hv * SV + c) = zr[k] ? (un_u4){0u, 0u, 0u, 0u} : v[k];
+        }
+      }
     }
   }
   __syncthreads();
@@ -499,12 +532,14 @@ __global__ void __launch_bounds__(64 * BZ) k_conv_mfma(conv_args a) {
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[mt][t] = (un_f4){0.f, 0.f, 0.f, 0.f};
   const un_u4 *wp = a.wp + l;
-#pragma unroll 2
-  for (int st = 0; st < KS; ++st) {
-    un_u4 b[NT];
+  // the weights of step st (clamped: a load past the last step reads it again)
+  auto ldb = [&](un_u4 (&b)[NT], int st) {
+    const int s2 = st < KS ? st : KS - 1;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) b[t] = wp[(st * NT + t) * 64];
-    if (MODE == CM_FIRST) {
+    for (int t = 0; t < NT; ++t) b[t] = wp[(s2 * NT + t) * 64];
+  };
+  auto step = [&](int st, const un_u4 (&b)[NT]) {
+    if constexpr (MODE == CM_FIRST) {
       const float *f = (const float *)sm;
 #pragma unroll
       for (int mt = 0; mt < 4; ++mt) {
@@ -515,7 +550,8 @@ __global__ void __launch_bounds__(64 * BZ) k_conv_mfma(conv_args a) {
           const int tap = 8 * lk + j;
           const int tp = tap < 27 ? tap : 0;
           const int dz = tp / 9, dy = (tp / 3) % 3, dx = tp % 3;
-          h[j] = tap < 27 ? un_rne(f[((w + dz) * 10 + vy + dy) * 10 + vx + dx]) : 0u;
+          const float fv = f[((w + dz) * 10 + vy + dy) * 10 + vx + dx];
+          h[j] = tap < 27 ? un_rne(fv) : 0u;
         }
         const un_u4 av = (un_u4){h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16),
                                  h[6] | (h[7] << 16)};
@@ -543,6 +579,18 @@ __global__ void __launch_bounds__(64 * BZ) k_conv_mfma(conv_args a) {
                                                                acc[mt][t], 0, 0, 0);
       }
     }
+  };
+  // the weights two steps ahead of their products
+  un_u4 bA[NT], bB[NT], bC[NT];
+  ldb(bA, 0);
+  ldb(bB, 1);
+  for (int st = 0; st < KS; st += 3) {
+    ldb(bC, st + 2);
+    step(st, bA);
+    ldb(bA, st + 3);
+    if (st + 1 < KS) step(st + 1, bB);
+    ldb(bB, st + 4);
+    if (st + 2 < KS) step(st + 2, bC);
   }
   // epilogue: row 4 lk + v of row tile mt is voxel (2 mt + row / 8, row % 8)
   const int z = z0 + w;
