@@ -59,25 +59,31 @@ __global__ void __launch_bounds__(64)
     vh_f4d acc[4];
 #pragma unroll
     for (int tt = 0; tt < 4; ++tt) acc[tt] = (vh_f4d){0.f, 0.f, 0.f, 0.f};
-    // slot s's record is record perm[s]; the index is loaded a chunk ahead
-    // of the record
+    // slot s's record is record perm[s]; the index is loaded two chunks
+    // ahead of its products, the offsets and the table rows one chunk ahead
+    // (in flight during the previous chunk's products)
+    auto rows = [&](float x, float y, float z, float4 (&g)[6]) {
+      const int c1 = table_col(x, half, spacing, nbdt, ncol) - 1;
+      const int c2 = table_col(y, half, spacing, nbdt, ncol) - 1;
+      const int c3 = table_col(z, half, spacing, nbdt, ncol) - 1;
+      const float4 *n1 = (const float4 *)(nodes + c1 * 8), *n2 = (const float4 *)(nodes + c2 * 8),
+                   *n3 = (const float4 *)(nodes + c3 * 8);
+      g[0] = n1[0];
+      g[1] = n1[1];
+      g[2] = n2[0];
+      g[3] = n2[1];
+      g[4] = n3[0];
+      g[5] = n3[1];
+    };
     float dx, dy, dz;
     vh_load_delta(P, perm[lo + l < hi ? lo + l : lo], dx, dy, dz);
-    uint32_t inext = perm[lo + DEP_STAGE + l < hi ? lo + DEP_STAGE + l : lo];
+    float4 g[6];
+    rows(dx, dy, dz, g);
+    vh_load_delta(P, perm[lo + DEP_STAGE + l < hi ? lo + DEP_STAGE + l : lo], dx, dy, dz);
+    uint32_t inext = perm[lo + 2 * DEP_STAGE + l < hi ? lo + 2 * DEP_STAGE + l : lo];
     for (int cur = lo; cur < hi; cur += DEP_STAGE) {
       const int chunk = min(DEP_STAGE, hi - cur);
       const float live = l < chunk ? 1.f : 0.f;
-      const int c1 = table_col(dx, half, spacing, nbdt, ncol) - 1;
-      const int c2 = table_col(dy, half, spacing, nbdt, ncol) - 1;
-      const int c3 = table_col(dz, half, spacing, nbdt, ncol) - 1;
-      const float4 *n1 = (const float4 *)(nodes + c1 * 8), *n2 = (const float4 *)(nodes + c2 * 8),
-                   *n3 = (const float4 *)(nodes + c3 * 8);
-      const float4 g[6] = {n1[0], n1[1], n2[0], n2[1], n3[0], n3[1]};
-      {  // the next chunk's offsets, in flight during this chunk's products
-        const int sn = cur + 2 * DEP_STAGE + l;
-        vh_load_delta(P, inext, dx, dy, dz);
-        inext = perm[sn < hi ? sn : lo];
-      }
       __syncthreads();   // the previous chunk's products have read vals
       float *v = vals + DEP_ROW * l;
 #pragma unroll
@@ -88,6 +94,12 @@ __global__ void __launch_bounds__(64)
         v[4 * a + 3] = g[a].w * live;
       }
       __syncthreads();
+      if (cur + DEP_STAGE < hi) {   // the next chunk's rows and the offsets after it
+        rows(dx, dy, dz, g);
+        const int sn = cur + 3 * DEP_STAGE + l;
+        vh_load_delta(P, inext, dx, dy, dz);
+        inext = perm[sn < hi ? sn : lo];
+      }
       const int ngroups = (chunk + 3) >> 2;
 #pragma unroll 2
       for (int q = 0; q < ngroups; ++q) {
