@@ -34,11 +34,11 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FIELD_FLOP_PER_PARTICLE = 4900.0      # 2300 FMA + 300 MUL of contract_spline_10
 # HBM bytes per k_field_mfma launch of the step (forces and push) at 8e6 particles
-# on 130^3, from two rocprofv3 counter passes over `bench.py --steps 20 --warmup 5`
-# (the average of the launches that push: FETCH_SIZE 325 821 KiB x 2.000 + WRITE_SIZE
-# 537 051 KiB x 1.000, the factors that make a 1 GiB device copy read and write 1 GiB);
-# recorded with the command in profiles/hbm_traffic_xenon_8M_130.md
-FIELD_TRAFFIC_BYTES_8M = (325821 * 2.0 + 537051 * 1.0) * 1024
+# on 130^3, from two rocprofv3 counter passes over `vlasov.jl_amd/profile_step.py
+# --steps 20` (the average of the 19 launches that push: FETCH_SIZE 331 153 KiB x 2.000
+# + WRITE_SIZE 534 149 KiB x 1.000, the factors that make a 1 GiB device copy read and
+# write 1 GiB); recorded with the commands in profiles/hbm_traffic_xenon_8M_130.md
+FIELD_TRAFFIC_BYTES_8M = (331153 * 2.0 + 534149 * 1.0) * 1024
 METRIC = "ms/step & particle-steps/s @8M,130^3; capture-q err vs 80M plain run"
 
 
@@ -450,7 +450,7 @@ def main():
                                f"dt=0.5, steps {args.warmup + 1}-{args.warmup + args.steps} of 700",
                    "particles": args.particles, "grid": 2 * args.nfine + 2, "softening": args.mode},
         "roofline": {"bound": "mfma",
-                     "engine": "FP32 MFMA 16x16x4 (table contraction) + VALU FP32 (second stage); the launch also pushes its particles (Verlet, keys of the next sort), reading the records through the sort's permutation and writing them in slot order",
+                     "engine": "x contraction on v_mfma_f32_16x16x32_f16 over fp16 splits (hi*hi + hi*lo + lo*hi, FP32 accumulation), y and z contractions on the FP32 VALU; achieved counts the FP32 algorithmic work of contract_spline_10 against the FP32 matrix-core peak. The launch also pushes its particles (Verlet, keys of the next sort), reading the records through the sort's permutation and writing them in slot order",
                      "kernel": "k_field_mfma",
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,

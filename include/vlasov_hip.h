@@ -182,8 +182,13 @@ int vh_accel_destroy(vh_accel *a);
  *                       of at most this many slots at the next sort).  The
  *                       sort and the field are independent of it to the bit;
  *                       the deposits' FP32 partial sums group by it.
- *                       vh_autotune_unit picks it. */
-enum { VH_OPT_EXACT_FIELD = 0, VH_OPT_STEP_GRAPH = 1, VH_OPT_UNIT_SLOTS = 2 };
+ *                       vh_autotune_unit picks it.
+ *   VH_OPT_FIELD_FP32   0 (default): the matrix-core contraction's first
+ *                       stage multiplies fp16 splits of the csol block and
+ *                       of the x columns (hi*hi + hi*lo + lo*hi, FP32
+ *                       accumulation; about 2^-21 of the largest term);
+ *                       1: FP32 products.  Ignored with VH_OPT_EXACT_FIELD. */
+enum { VH_OPT_EXACT_FIELD = 0, VH_OPT_STEP_GRAPH = 1, VH_OPT_UNIT_SLOTS = 2, VH_OPT_FIELD_FP32 = 3 };
 int vh_accel_set_option(vh_accel *a, int option, int value);
 
 /* upload!/download! of the particle buffer (src/accelerator.jl:48, 64-68):

@@ -296,14 +296,25 @@ def test_field_contraction_bit_exact(setup, orc):
     assert relerr(f[outl], wfb[outl]) < REL
 
 
-def test_field_matrix_cores(setup, orc):
-    """The default variant (first stage on MFMA): within FP32 rounding of the
-    FP64 contraction, identical to the bit from one call to the next, and
-    within a few FP32 ulps of the exact-order variant."""
+@pytest.mark.parametrize("products", ["split16", "fp32"])
+def test_field_matrix_cores(setup, orc, products):
+    """The matrix-core variant (first stage on MFMA; fp16 split products by
+    default, FP32 products as an option): within FP32 rounding of the FP64
+    contraction, identical to the bit from one call to the next, and within
+    1e-6 of the exact-order variant (a few FP32 ulps; the split's dropped
+    terms are about 2^-21 of the largest product)."""
     s = setup
     srec, occ, bnd, _, csol_c = _forces_setup(s, orc, seed=8)
     sm, ost = s["sm"], s["ost"]
     acc = s["acc"]
+    acc.set_field_fp32(products == "fp32")
+    try:
+        _field_matrix_cores(s, orc, srec, occ, bnd, sm, ost, acc)
+    finally:
+        acc.set_field_fp32(False)
+
+
+def _field_matrix_cores(s, orc, srec, occ, bnd, sm, ost, acc):
     # a zero-mean rough potential: no constant part whose cancelling
     # contributions would make the rounding of any summation order look large
     rng = np.random.default_rng(18)

@@ -354,7 +354,7 @@ extern "C" int vh_accel_destroy(vh_accel *a) {
   hipSetDevice(a->ctx->device);
   hipStreamSynchronize(a->ctx->stream);
   drop_graphs(a);
-  void *ptrs[] = {a->ovl, a->grad, a->nodes, a->og, a->ovl64, a->knots64, a->parts[0], a->parts[1], a->force,
+  void *ptrs[] = {a->ovl, a->grad, a->nodes, a->og, a->xsplit, a->ovl64, a->knots64, a->parts[0], a->parts[1], a->force,
                   a->keys[0], a->keys[1], a->vals[0], a->vals[1], a->counts, a->scan_tmp,
                   a->occupied, a->bounds, a->cellmap, a->fitlist, a->nonfit, a->nonfit_prefix,
                   a->outlist, a->counters, a->cellrun, a->tiles, a->cicwins, a->dpart, a->dscal, a->cellscratch,
@@ -372,7 +372,8 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
   VH_REQUIRE(npart >= 1 && npart <= ((int64_t)1 << 31) - 4096,
              "vh_accel_create: npart = %lld out of range", (long long)npart);
   VH_REQUIRE(nk >= 6 && nk <= 512, "vh_accel_create: nk = %d out of range", nk);
-  VH_REQUIRE(nbdt >= 1, "vh_accel_create: nbdt must be positive");
+  // the field kernel packs three table columns into one word, 10 bits each
+  VH_REQUIRE(nbdt >= 1 && nbdt <= 1023, "vh_accel_create: nbdt = %d out of [1, 1023]", nbdt);
   const double x0 = knots[0];
   const double h = (knots[nk - 1] - knots[0]) / (nk - 1);
   VH_REQUIRE(h > 0 && std::isfinite(h) && std::isfinite(x0), "vh_accel_create: bad axis");
@@ -425,14 +426,41 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
   MK(upload_converted(ctx, &a->grad, gradient, (size_t)10 * a->ncol));
   MK(upload_converted(ctx, &a->nodes, nodes, (size_t)8 * a->ncol));
   MK(upload_converted(ctx, &a->ovl64, overlap, (size_t)10 * a->ncol));
-  {  // overlap and gradient of a column side by side, 80 bytes per column
-    std::vector<double> og((size_t)20 * a->ncol);
+  {  // overlap and gradient of a column side by side, 80 bytes of a 128-byte
+     // line per column (the field kernel loads a column as one line)
+    std::vector<double> og((size_t)32 * a->ncol, 0.0);
     for (int c = 0; c < a->ncol; ++c)
       for (int k = 0; k < 10; ++k) {
-        og[(size_t)20 * c + k] = overlap[(size_t)10 * c + k];
-        og[(size_t)20 * c + 10 + k] = gradient[(size_t)10 * c + k];
+        og[(size_t)32 * c + k] = overlap[(size_t)10 * c + k];
+        og[(size_t)32 * c + 10 + k] = gradient[(size_t)10 * c + k];
       }
     MK(upload_converted(ctx, &a->og, og.data(), og.size()));
+  }
+  {  // the x columns split into fp16 pairs for the first stage of the
+     // matrix-core field kernel, each kind (overlap, gradient) scaled by the
+     // power of two that puts its largest FP32 value in [2^14, 2^15)
+    std::vector<uint16_t> xs((size_t)64 * a->ncol);
+    const double *tab[2] = {overlap, gradient};
+    for (int t = 0; t < 2; ++t) {
+      float mx = 0.f;
+      for (size_t i = 0; i < (size_t)10 * a->ncol; ++i) mx = std::max(mx, std::fabs((float)tab[t][i]));
+      int e = 0;
+      std::frexp(mx > 0.f ? mx : 1.f, &e);  // mx = m 2^e, m in [0.5, 1)
+      const float sc = std::ldexp(1.f, 15 - e);
+      a->xs_inv[t] = std::ldexp(1.f, e - 15);
+      for (int c = 0; c < a->ncol; ++c)
+        for (int k = 0; k < 32; ++k) {
+          int ii, cp, wp;
+          vh_s16_slot(k, ii, cp, wp);
+          const float v = (float)tab[t][(size_t)10 * c + ii] * sc;  // exact: a power of two
+          const _Float16 hi = (_Float16)v;
+          const _Float16 lo = (_Float16)(v - (float)hi);
+          const _Float16 x = wp < 0 ? (_Float16)0.f : wp == 0 ? hi : lo;
+          xs[(size_t)64 * c + 32 * t + k] = __builtin_bit_cast(uint16_t, x);
+        }
+    }
+    MK(vh_alloc((void **)&a->xsplit, xs.size() * sizeof(uint16_t)));
+    VH_CHECK_HIP(hipMemcpy(a->xsplit, xs.data(), xs.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
   }
   MK(upload_converted(ctx, &a->knots64, knots, (size_t)nk));
   size_t np = (size_t)a->npad;
@@ -491,6 +519,14 @@ extern "C" int vh_accel_set_option(vh_accel *a, int option, int value) {
     case VH_OPT_EXACT_FIELD:
       VH_REQUIRE(value == 0 || value == 1, "vh_accel_set_option: exact field takes 0 or 1");
       a->exact_field = value;
+      return VH_OK;
+    case VH_OPT_FIELD_FP32:
+      VH_REQUIRE(value == 0 || value == 1, "vh_accel_set_option: FP32 field products take 0 or 1");
+      if (value != a->field_fp32) {
+        a->field_fp32 = value;
+        drop_graphs(a);
+        ++a->config_version;
+      }
       return VH_OK;
     case VH_OPT_STEP_GRAPH:
       VH_REQUIRE(value == 0 || value == 1, "vh_accel_set_option: step graph takes 0 or 1");

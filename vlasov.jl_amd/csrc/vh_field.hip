@@ -305,12 +305,22 @@ __global__ void __launch_bounds__(64)
 // The sums run in another order than the reference's FMA chain, so the result
 // agrees to FP32 rounding, not to the bit; k_field keeps the exact order.
 //
-// Per chunk of 64 particles: phase A (a lane per particle) stages the six
-// table columns of each particle in LDS, rows of FT_STRIDE floats (odd, so
-// that consecutive particles fall in different banks): ox, gx, oy, gy, oz, gz
-// at offsets 0, 10, 20, 30, 40, 50; phase B runs the four sub-groups of 16
-// particles; phase C stores.
-#define FT_STRIDE 61
+// Per chunk of 64 particles: phase A (a lane per particle for the offsets,
+// five lanes per particle for the table loads) stages the x, y and z
+// columns of each particle in LDS, each in a row of 20 floats (overlap, then
+// gradient) of its own region, twelve rows a 1 KiB block (ft_row: rows of
+// the 16 particles of a sub-group fall at most two to a bank group); phase
+// B runs the four sub-groups of 16 particles; phase C stores.
+#define FT_REGION 1280
+
+// the LDS row (20 floats; FT_REGION floats a region) of the chunk's particle p
+__device__ __forceinline__ int ft_row(int p) { return 20 * p; }
+
+// 16 bytes per lane from global memory to LDS: lane l writes bytes
+// 16 l .. 16 l + 15 of the 1 KiB block at lds (a wave-uniform address)
+__device__ __forceinline__ void ft_dma16(const void *src, float *lds) {
+  __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void *)lds, 16, 0, 0);
+}
 
 typedef float ft_f4 __attribute__((ext_vector_type(4)));
 
@@ -331,37 +341,37 @@ __device__ __forceinline__ void ft_bops(const float *row, int grp, float b[6]) {
   b[5] = top ? b5 : 0.f;
 }
 
-// The contraction of NS sub-groups at once (sg0 .. sg0 + NS - 1), pipelined
-// over the 7 row tiles: tile t's products (T_o and T_g chains of every
-// sub-group interleaved over the three K steps) are issued, then the y
-// contraction of tile t - 1, whose products are complete, runs on the VALU
-// while tile t's are in flight.  Slot q = 4t + v of a lane adds into
-// accumulator m = q / 10 (planes g, g + 4 and the half plane).  Then the z
-// contraction, the sum over the four lane groups of a particle, and the
-// store of -w F into the chunk's slot (all four lanes store the same value).
+// The contraction of one sub-group, pipelined over the 7 row tiles: tile
+// t's products (T_o and T_g chains interleaved over the three K steps) are
+// issued, then the y contraction of tile t - 1, whose products are complete,
+// runs on the VALU while tile t's are in flight.  Slot q = 4t + v of a lane
+// adds into accumulator m = q / 10 (planes g, g + 4 and the half plane).
+// Then the z contraction and the sum over the four lane groups of a
+// particle: every lane of the particle returns -w F.  rx, ry, rz: the
+// particle's x, y and z columns in LDS (overlap, then gradient).
 template <int NS>
-__device__ __forceinline__ void ft_contract(const float A[7][3], const float *tabj, int sg0, int grp,
-                                            int j16, float w, float (*fld)[64]) {
+__device__ __forceinline__ ft_f4 ft_contract(const float A[7][3], const float *rx, const float *ry,
+                                             const float *rz, int grp, float w) {
+  static_assert(NS == 1, "one sub-group at a time");
   const int half = 5 * (grp & 1), hp = grp >> 1;
   float b[NS][6], y[NS][20], yh[NS][10], z[NS][6];
 #pragma unroll
   for (int u = 0; u < NS; ++u) {
-    const float *row = tabj + 16 * (sg0 + u) * FT_STRIDE;
-    ft_bops(row, grp, b[u]);
+    ft_bops(rx, grp, b[u]);
 #pragma unroll
-    for (int a = 0; a < 20; ++a) y[u][a] = row[20 + a];  // oy, then gy
+    for (int a = 0; a < 20; ++a) y[u][a] = ry[a];  // oy, then gy
     // the half plane's y weights: oy and gy at jj = half .. half + 4
 #pragma unroll
     for (int a = 0; a < 5; ++a) {
-      yh[u][a] = row[20 + half + a];
-      yh[u][5 + a] = row[30 + half + a];
+      yh[u][a] = ry[half + a];
+      yh[u][5 + a] = ry[10 + half + a];
     }
-    z[u][0] = row[40 + grp];
-    z[u][1] = row[44 + grp];
-    z[u][2] = row[48 + hp];
-    z[u][3] = row[50 + grp];
-    z[u][4] = row[54 + grp];
-    z[u][5] = row[58 + hp];
+    z[u][0] = rz[grp];
+    z[u][1] = rz[4 + grp];
+    z[u][2] = rz[8 + hp];
+    z[u][3] = rz[10 + grp];
+    z[u][4] = rz[14 + grp];
+    z[u][5] = rz[18 + hp];
   }
   float ux[NS][3], uy[NS][3], uz[NS][3];
 #pragma unroll
@@ -433,17 +443,171 @@ __device__ __forceinline__ void ft_contract(const float A[7][3], const float *ta
     fx += __shfl_xor(fx, 32, 64);
     fy += __shfl_xor(fy, 32, 64);
     fz += __shfl_xor(fz, 32, 64);
-    const int slot = 16 * (sg0 + u) + j16;
-    fld[0][slot] = -w * fx;
-    fld[1][slot] = -w * fy;
-    fld[2][slot] = -w * fz;
+    return ft_f4{-w * fx, -w * fy, -w * fz, 0.f};
   }
 }
 
+// ---- the fp16 split variant of the first stage (default) ----
+// One v_mfma_f32_16x16x32_f16 per row tile and kind replaces the three FP32
+// K steps: its 32 K slots hold, for the ten x columns ii, c_hi w_hi,
+// c_hi w_lo and c_lo w_hi (hi = fp16(v), lo = fp16(v - hi)), in the order of
+// vh_s16_slot.  The cell's block is scaled by the power of two that puts its
+// largest |c| in [2^14, 2^15) before the split, the x columns (a table split
+// once on the host, xsplit) by their own; the results are scaled back in the
+// final product by -w.  Lane l supplies A[l%16][8(l/16) + j] and
+// B[8(l/16) + j][l%16], j = 0..7, and receives D as the FP32 product does,
+// so stages 2 and 3 are unchanged.  The dropped lo*lo term and the
+// residuals of the splits leave about 2^-21 of the largest term per
+// product.
+//
+// LDS per chunk: the y and z rows as in the FP32 variant, and the x splits
+// in eight 1 KiB blocks: block 2 s + h holds, at 16 (16 g + j),
+// the 16 bytes of the B operand of lane group g for particle 16 s + j
+// (h = 0 overlap, 1 gradient), so that a sub-group reads its operands at
+// 16 consecutive addresses per lane group.
+
+typedef _Float16 ft_h8 __attribute__((ext_vector_type(8)));
+typedef unsigned int ft_u4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t ft_h2(_Float16 a, _Float16 b) {
+  return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+}
+
+// A operand of one row from its scaled c at ii = 2 grp, 2 grp + 1 and
+// 8 + grp / 2, in the K order of vh_s16_slot
+__device__ __forceinline__ ft_u4 ft_split_row(float va, float vb, float vc, int grp) {
+  const _Float16 ha = (_Float16)va, hb = (_Float16)vb, hc = (_Float16)vc;
+  const _Float16 la = (_Float16)(va - (float)ha), lb = (_Float16)(vb - (float)hb);
+  const _Float16 lc = (_Float16)(vc - (float)hc);
+  // both forms of the last word, selected by a mask (a select here became a
+  // branch around the lo computation)
+  const uint32_t odd = 0u - (uint32_t)(grp & 1);
+  const uint32_t w3 = (ft_h2(lc, (_Float16)0.f) & odd) | (ft_h2(hc, hc) & ~odd);
+  return ft_u4{ft_h2(ha, ha), ft_h2(la, hb), ft_h2(hb, lb), w3};
+}
+
+__device__ __forceinline__ ft_f4 ft_mfma16(ft_u4 a, ft_u4 b) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(ft_h8, a), __builtin_bit_cast(ft_h8, b),
+                                                ft_f4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+}
+
+// One sub-group of 16 particles: tile t's two products, then the y
+// contraction of tile t - 1 on the VALU (the slot layout of ft_contract);
+// the z contraction, the sum over the four lane groups, and the field times
+// the scale factors mx (x component) and myz, returned by every lane of the
+// particle.
+__device__ __forceinline__ ft_f4 ft_contract16(const ft_u4 A[7], const float *ry, const float *rz,
+                                               const float *bj, int grp, float mx, float myz) {
+  const int half = 5 * (grp & 1), hp = grp >> 1;
+  const ft_u4 bo = *(const ft_u4 *)bj;
+  const ft_u4 bg = *(const ft_u4 *)(bj + 256);
+  float y[20], yh[10], z[6];
+#pragma unroll
+  for (int a = 0; a < 20; ++a) y[a] = ry[a];
+#pragma unroll
+  for (int a = 0; a < 5; ++a) {
+    yh[a] = ry[half + a];
+    yh[5 + a] = ry[10 + half + a];
+  }
+  z[0] = rz[grp];
+  z[1] = rz[4 + grp];
+  z[2] = rz[8 + hp];
+  z[3] = rz[10 + grp];
+  z[4] = rz[14 + grp];
+  z[5] = rz[18 + hp];
+  float ux[3] = {0.f, 0.f, 0.f}, uy[3] = {0.f, 0.f, 0.f}, uz[3] = {0.f, 0.f, 0.f};
+  ft_f4 To[7], Tg[7];
+#pragma unroll
+  for (int t = 0; t <= 7; ++t) {
+    if (t < 7) {
+      To[t] = ft_mfma16(A[t], bo);
+      Tg[t] = ft_mfma16(A[t], bg);
+    }
+    if (t >= 1) {
+      const int tp = t - 1;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int q = 4 * tp + v;
+        if (q < 20) {
+          const int m = q / 10, jj = q % 10;
+          ux[m] = __builtin_fmaf(y[jj], Tg[tp][v], ux[m]);
+          uy[m] = __builtin_fmaf(y[10 + jj], To[tp][v], uy[m]);
+          uz[m] = __builtin_fmaf(y[jj], To[tp][v], uz[m]);
+        } else if (q < 25) {
+          const int i = q - 20;
+          ux[2] = __builtin_fmaf(yh[i], Tg[tp][v], ux[2]);
+          uy[2] = __builtin_fmaf(yh[5 + i], To[tp][v], uy[2]);
+          uz[2] = __builtin_fmaf(yh[i], To[tp][v], uz[2]);
+        }
+      }
+    }
+  }
+  float fx = 0.f, fy = 0.f, fz = 0.f;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    fx = __builtin_fmaf(z[m], ux[m], fx);
+    fy = __builtin_fmaf(z[m], uy[m], fy);
+    fz = __builtin_fmaf(z[3 + m], uz[m], fz);
+  }
+  fx += __shfl_xor(fx, 16, 64);
+  fy += __shfl_xor(fy, 16, 64);
+  fz += __shfl_xor(fz, 16, 64);
+  fx += __shfl_xor(fx, 32, 64);
+  fy += __shfl_xor(fy, 32, 64);
+  fz += __shfl_xor(fz, 32, 64);
+  return ft_f4{mx * fx, myz * fy, myz * fz, 0.f};
+}
+
+// The table columns of a chunk's particles (the three columns of lane l's
+// particle packed in `packed`, 10 bits each) go from the caches straight to
+// LDS with 16-byte LDS DMA loads (lane l of a load writes bytes
+// 16 l .. 16 l + 15 of a 1 KiB block), a line at a time: a column's 80
+// bytes (overlap, then gradient) sit in one 128-byte line, fetched as five
+// 16-byte pieces by five consecutive lanes (slot 64 i + l of load i is
+// piece k % 5 of particle k / 5: 80-byte rows, five loads an axis), and a
+// column's fp16 splits fill one line, fetched by four lanes a half (block
+// 2 s + h: particle 16 s + l % 16, piece 4 h + l / 16).  A load then touches
+// 13 or 16 cache lines where a lane per particle would touch 64 (a scattered
+// load costs the cache about a cycle per line), and holds no register.
+template <bool S16>
+__device__ __forceinline__ void ft_stage(int packed, const float4 *og, const ft_u4 *xsplit, float *tx,
+                                         float *ty, float *tz, int l) {
+  const ft_f4 *og4 = (const ft_f4 *)og;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    const int k = 64 * i + l, pk = k / 5, a = k - 5 * pk;
+    const int c = __shfl(packed, pk, 64);
+    if constexpr (!S16) ft_dma16(og4 + 8 * (c & 1023) + a, tx + i * 256);
+    ft_dma16(og4 + 8 * ((c >> 10) & 1023) + a, ty + i * 256);
+    ft_dma16(og4 + 8 * (c >> 20) + a, tz + i * 256);
+  }
+  if constexpr (S16) {
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4) {
+      const int c = __shfl(packed, 16 * s4 + (l & 15), 64) & 1023;
+      ft_dma16(xsplit + 8 * c + (l >> 4), tx + (2 * s4) * 256);
+      ft_dma16(xsplit + 8 * c + 4 + (l >> 4), tx + (2 * s4 + 1) * 256);
+    }
+  }
+}
+
+// the three table columns of a particle (src/kernels.jl:414-419), packed
+__device__ __forceinline__ int ft_columns(const vh_pos &r, float half, float spacing, float nbdt, int ncol) {
+  int c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int cc = (int)floorf((r.d[d] + half) / spacing * nbdt + 0.5f) + 1;
+    c[d] = min(max(cc, 1), ncol) - 1;
+  }
+  return c[0] | (c[1] << 10) | (c[2] << 20);  // columns < 1024
+}
+
+template <bool S16>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     k_field_mfma(const int32_t *__restrict__ P, const uint32_t *__restrict__ perm, int64_t npad,
                  const float *__restrict__ csol, int n,
-                 const float4 *__restrict__ og, const float *__restrict__ zero,
+                 const float4 *__restrict__ og, const ft_u4 *__restrict__ xsplit, float xs_inv_o,
+                 float xs_inv_g, const float *__restrict__ zero,
                  const vh_unit *__restrict__ units, const int32_t *__restrict__ cnt, int nk,
                  float x0, float h, float spacing, float nbdt, float w, int ncol,
                  vh_proj_args pa, vh_push_args pu, float *__restrict__ force,
@@ -459,8 +623,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     pa.py = (float)pa.dev[1];
     pa.pz = (float)pa.dev[2];
   }
-  __shared__ float fld[3][64];
-  __shared__ float tab[64 * FT_STRIDE];
+  // column regions of a chunk: x (S16: the splits, eight 1 KiB blocks), y, z
+  constexpr int xreg = S16 ? 2048 : FT_REGION;
+  __shared__ __attribute__((aligned(16))) float tab[xreg + 2 * FT_REGION];
+  float *const tx = tab, *const ty = tab + xreg, *const tz = ty + FT_REGION;
   const int l = threadIdx.x;
   const int j16 = l & 15, grp = l >> 4;
   const float half = spacing * 0.5f;
@@ -477,7 +643,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
     const int lo = un.lo, hi = un.hi;
     // A operands of the cell: row j16 of tile t (slot q = 4 t + j16 % 4 of
     // lane group g = j16 / 4), k = ii = 4 s + grp
+    // (split variant: k = ii = 2 grp, 2 grp + 1, 8 + grp / 2 for s = 0, 1, 2)
     float A[7][3];
+    ft_u4 A16[7];
+    float mx = 0.f, myz = 0.f;
     if (ok) {
       const float *tb = csol + (bx - 1) + nn * ((by - 1) + nn * (bz - 1));
       const int g = j16 >> 2;
@@ -488,39 +657,47 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
         const int kk = q < 10 ? g : q < 20 ? g + 4 : 8 + (g >> 1);
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
-          const int ii = 4 * s + grp;
+          const int ii = S16 ? (s < 2 ? 2 * grp + s : 8 + (grp >> 1)) : 4 * s + grp;
           const bool in = q < 25 && ii < 10;
           // padding reads a zero word: the loaded value is used as it is, so
           // the loads stay in flight until the first product
           A[t][s] = *(in ? tb + ii + nn * (jj + nn * kk) : zero);
         }
       }
+      if constexpr (S16) {
+        // the block's largest |c| (every value is loaded by some lane), its
+        // power-of-two scale, and the split
+        float m = 0.f;
+#pragma unroll
+        for (int t = 0; t < 7; ++t)
+#pragma unroll
+          for (int s = 0; s < 3; ++s) m = fmaxf(m, fabsf(A[t][s]));
+#pragma unroll
+        for (int o = 32; o; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        int e = 0;
+        (void)frexpf(m, &e);  // m = f 2^e, f in [0.5, 1); e = 0 for m = 0
+        const float sc = ldexpf(1.f, 15 - e), inv = ldexpf(1.f, e - 15);
+#pragma unroll
+        for (int t = 0; t < 7; ++t) A16[t] = ft_split_row(A[t][0] * sc, A[t][1] * sc, A[t][2] * sc, grp);
+        mx = -w * inv * xs_inv_g;
+        myz = -w * inv * xs_inv_o;
+      }
     }
-    // the first chunk's records; each later chunk's are loaded during the
-    // previous chunk's contraction
-    // record indices are loaded a chunk ahead of the records
+    // The first chunk's records and their table columns in LDS; each later
+    // chunk's records are loaded during the previous chunk's contraction,
+    // and its columns staged after it (while the previous chunk stores).
+    // Record indices are loaded a chunk ahead of the records.
     uint32_t inow = perm[lo + l < hi ? lo + l : lo];
-    vh_pos rnext = vh_load_pos(P, inow);
+    vh_pos rcur = vh_load_pos(P, inow);
     uint32_t inext = perm[lo + 64 + l < hi ? lo + 64 + l : lo];
+    if (ok) ft_stage<S16>(ft_columns(rcur, half, spacing, nbdt, ncol), og, xsplit, tx, ty, tz, l);
     for (int base = lo; base < hi; base += 64) {
       const int nlive = min(64, hi - base);
-      // phase A, one lane per particle: offsets, the six table columns of the
-      // particle staged in LDS, and the projectile term (independent of the
-      // contraction, so its latency overlaps the table loads)
-      const int s = base + l;
       const bool mine = l < nlive;
-      const vh_pos rp = rnext;
+      const vh_pos rp = rcur;
       // the previous positions, for the push after the contraction
       const vh_pos qp = vh_load_pos(Q, inow);
       const float dx0 = rp.d[0], dy0 = rp.d[1], dz0 = rp.d[2];
-      // everything that reads the record comes before the first branch (a
-      // load still in flight at a branch is waited for with every later one)
-      int cx = (int)floorf((dx0 + half) / spacing * nbdt + 0.5f) + 1;
-      int cy = (int)floorf((dy0 + half) / spacing * nbdt + 0.5f) + 1;
-      int cz = (int)floorf((dz0 + half) / spacing * nbdt + 0.5f) + 1;
-      cx = min(max(cx, 1), ncol) - 1;
-      cy = min(max(cy, 1), ncol) - 1;
-      cz = min(max(cz, 1), ncol) - 1;
       float px, py, pz;
       if (pa.ball) {  // the particle's own position
         px = x0 + (float)(rp.k[0] - 1) * h + dx0;
@@ -531,19 +708,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
         py = x0 + (float)(ky - 1) * h + dy0;
         pz = x0 + (float)(kz - 1) * h + dz0;
       }
-      float4 gv[3][5];
-      if (ok) {
-        // each axis: overlap and gradient side by side, five 16-byte loads;
-        // all fifteen are issued before any is written to LDS, and the
-        // projectile term below runs while they are in flight
-        const int col[3] = {cx, cy, cz};
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const float4 *src = og + 5 * col[d];
-#pragma unroll
-          for (int a = 0; a < 5; ++a) gv[d][a] = src[a];
-        }
-      }
+      // the projectile term, independent of the contraction: it runs while
+      // the chunk's columns are in flight
       float fx2 = 0.f, fy2 = 0.f, fz2 = 0.f;
       if (mine && pa.coef != 0.f) {
         const float dx = pa.px - px, dy = pa.py - py, dz = pa.pz - pz;
@@ -566,40 +732,37 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
         saz += fz2;
         sae += pot;
       }
-      if (ok) {
-        float *row = tab + l * FT_STRIDE;
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-#pragma unroll
-          for (int a = 0; a < 5; ++a) {
-            row[20 * d + 4 * a] = gv[d][a].x;
-            row[20 * d + 4 * a + 1] = gv[d][a].y;
-            row[20 * d + 4 * a + 2] = gv[d][a].z;
-            row[20 * d + 4 * a + 3] = gv[d][a].w;
-          }
-      }
-      __syncthreads();
+      __syncthreads();  // the columns have landed
+      const bool more = base + 64 < hi;
       {
         const int sn = base + 64 + l;
         inow = inext;
-        rnext = vh_load_pos(P, inow);
+        rcur = vh_load_pos(P, inow);
         inext = perm[sn + 64 < hi ? sn + 64 : lo];
       }
-      // phase B: the contraction, one sub-group of 16 particles at a time
-      // (two at a time, ft_contract<2>, measured no faster at twice the
-      // registers)
+      // the contraction, one sub-group of 16 particles at a time; lane l
+      // keeps the field of particle l (lane group sg of sub-group sg)
+      ft_f4 fl = {0.f, 0.f, 0.f, 0.f};
       if (ok) {
         const int nsg = (nlive + 15) >> 4;
-        const float *tabj = tab + j16 * FT_STRIDE;
 #pragma unroll 1
-        for (int sg = 0; sg < nsg; ++sg) ft_contract<1>(A, tabj, sg, grp, j16, w, fld);
+        for (int sg = 0; sg < nsg; ++sg) {
+          const int r = ft_row(16 * sg + j16);
+          ft_f4 f;
+          if constexpr (S16)
+            f = ft_contract16(A16, ty + r, tz + r, tx + 512 * sg + 64 * grp + 4 * j16, grp, mx, myz);
+          else
+            f = ft_contract<1>(A, tx + r, ty + r, tz + r, grp, w);
+          fl = grp == sg ? f : fl;
+        }
+        // the next chunk's columns: every read of this chunk's is done
+        if (more) ft_stage<S16>(ft_columns(rcur, half, spacing, nbdt, ncol), og, xsplit, tx, ty, tz, l);
       }
-      __syncthreads();
-      // phase C: field minus projectile term, coalesced stores of the live
-      // lanes; with the push, the records and keys of a cell that fits (the
+      // field minus projectile term, coalesced stores of the live lanes;
+      // with the push, the records and keys of a cell that fits (the
       // particles of the others are pushed by the fallback)
-      const float f0 = ok ? fld[0][l] - fx2 : nan, f1 = ok ? fld[1][l] - fy2 : nan;
-      const float f2 = ok ? fld[2][l] - fz2 : nan;
+      const float f0 = ok ? fl[0] - fx2 : nan, f1 = ok ? fl[1] - fy2 : nan;
+      const float f2 = ok ? fl[2] - fz2 : nan;
       vh_bstore_f(vh_rsrc(force + base, 4 * nlive), l, f0);
       vh_bstore_f(vh_rsrc(force + npad + base, 4 * nlive), l, f1);
       vh_bstore_f(vh_rsrc(force + 2 * npad + base, 4 * nlive), l, f2);
@@ -1199,9 +1362,10 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
                        a->nk, (float)a->x0, (float)a->h, a->spacing, (float)a->nbdt,
                        (float)weight, a->ncol, pa, a->force, a->dpart);
   else
-    hipLaunchKernelGGL(k_field_mfma, dim3(grid), dim3(64), 0, st, a->parts[a->cur],
-                       vh_rec_perm(a), a->npad, fine->csol, a->n, (const float4 *)a->og, a->zero, a->units, a->counters,
-                       a->nk, (float)a->x0, (float)a->h, a->spacing, (float)a->nbdt,
+    hipLaunchKernelGGL(a->field_fp32 ? k_field_mfma<false> : k_field_mfma<true>, dim3(grid), dim3(64), 0, st,
+                       a->parts[a->cur], vh_rec_perm(a), a->npad, fine->csol, a->n, (const float4 *)a->og,
+                       (const ft_u4 *)a->xsplit, a->xs_inv[0], a->xs_inv[1], a->zero, a->units,
+                       a->counters, a->nk, (float)a->x0, (float)a->h, a->spacing, (float)a->nbdt,
                        (float)weight, a->ncol, pa, pu, a->force, a->dpart);
   vh_prof_end(c, tok);
   VH_LAUNCH_CHECK();

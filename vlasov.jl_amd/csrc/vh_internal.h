@@ -111,6 +111,30 @@ struct vh_mesh {
 // memory sectors) rather than twelve scattered words.
 #define VH_REC_WORDS 12   // int32 words per particle in a buffer (both arrays)
 
+// K slot k = 8 g + j (g = lane group, j = element) of the fp16 split
+// product in the matrix-core field kernel's first stage, where a
+// v_mfma_f32_16x16x32_f16 sums, for the ten x columns ii of a row of csol,
+// c_hi w_hi + c_hi w_lo + c_lo w_hi (hi = fp16(v), lo = fp16(v - hi) of the
+// scaled FP32 values): slot k pairs part cpart of c[ii] with part wpart of
+// w[ii] (0 hi, 1 lo, -1 an empty slot).  Lane group g needs c at ii = 2g,
+// 2g + 1 and 8 + g / 2 only.
+__host__ __device__ inline void vh_s16_slot(int k, int &ii, int &cpart, int &wpart) {
+  const int g = k >> 3, j = k & 7;
+  if (j < 6) {
+    ii = 2 * g + j / 3;
+    cpart = j % 3 == 2;
+    wpart = j % 3 == 1;
+  } else if (j == 6) {
+    ii = 8 + (g >> 1);
+    cpart = g & 1;
+    wpart = 0;
+  } else {
+    ii = 8 + (g >> 1);
+    cpart = (g & 1) ? -1 : 0;
+    wpart = (g & 1) ? -1 : 1;
+  }
+}
+
 struct vh_pos {
   int k[3];
   float d[3];
@@ -218,7 +242,10 @@ struct vh_accel {
   float spacing;
   int nbdt, ncol;
   float *ovl, *grad, *nodes;   // [col*10 + a], [col*8 + a]
-  float *og;                   // [col*20 + a]: overlap a < 10, gradient 10 + a (16-byte columns)
+  float *og;                   // [col*32 + a]: overlap a < 10, gradient 10 + a (128-byte columns)
+  uint16_t *xsplit;            // [col*64 + 32 t + k]: fp16 split of the overlap (t = 0) and
+                               // gradient (t = 1) column in the K order of vh_s16_slot
+  float xs_inv[2];             // 1 / the power-of-two scales of those splits
   float *zero;                 // 64 zero words (the target of padding loads)
   double *ovl64;               // overlap in FP64 (energy budget)
   double *knots64;             // fine knots in FP64
@@ -274,6 +301,7 @@ struct vh_accel {
   int nout_idx;                // counters[nout_idx]: the deposit's rejections of this sort
   int pushed_fused;            // host flag: the field kernel's push wrote the records
   int exact_field;             // VH_OPT_EXACT_FIELD: the reference's FMA order
+  int field_fp32;              // VH_OPT_FIELD_FP32: FP32 products in the first stage
   int step_graph;              // VH_OPT_STEP_GRAPH
   hipGraphExec_t graph[2];     // vh_step recorded with parts[cur] = parts[0], parts[1]
   double graph_key[15];        // the arguments they were recorded with (VH_GRAPH_KEY)

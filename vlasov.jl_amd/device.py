@@ -191,6 +191,12 @@ class DeviceAccelerator:
         first stage on the matrix cores."""
         _lib.call("vh_accel_set_option", self.handle, _lib.VH_OPT_EXACT_FIELD, 1 if exact else 0)
 
+    def set_field_fp32(self, fp32: bool):
+        """True: the matrix-core contraction's first stage in FP32 products;
+        False (default): in fp16 splits (hi*hi + hi*lo + lo*hi, FP32
+        accumulation, about 2^-21 of the largest term)."""
+        _lib.call("vh_accel_set_option", self.handle, _lib.VH_OPT_FIELD_FP32, 1 if fp32 else 0)
+
     def set_step_graph(self, on: bool = True):
         """vh_step recorded once into a HIP graph per particle buffer and
         replayed (VH_OPT_STEP_GRAPH)."""

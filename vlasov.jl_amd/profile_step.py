@@ -35,12 +35,15 @@ def main(argv=None):
     ap.add_argument("--nfine", type=int, default=64)
     ap.add_argument("--steps", type=int, default=25)
     ap.add_argument("--seed", type=int, default=-1)
+    ap.add_argument("--field-fp32", action="store_true",
+                    help="the field contraction's first stage in FP32 products (VH_OPT_FIELD_FP32)")
     args = ap.parse_args(argv)
     va = load_package()
     L = va._lib
     p = va.xenon_parameters(args.particles, args.nfine)
     sim = va.Simulation(p, va.read_radial_profile(), projectile=va.xenon_projectile(), seed=args.seed)
     try:
+        sim.acc.set_field_fp32(args.field_fp32)
         L.call("vh_profile_enable", sim.ctx.handle, L.VH_PROF_ALL)
         for _ in range(args.steps):
             sim.step()
