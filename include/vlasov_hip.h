@@ -187,8 +187,19 @@ int vh_accel_destroy(vh_accel *a);
  *                       stage multiplies fp16 splits of the csol block and
  *                       of the x columns (hi*hi + hi*lo + lo*hi, FP32
  *                       accumulation; about 2^-21 of the largest term);
- *                       1: FP32 products.  Ignored with VH_OPT_EXACT_FIELD. */
-enum { VH_OPT_EXACT_FIELD = 0, VH_OPT_STEP_GRAPH = 1, VH_OPT_UNIT_SLOTS = 2, VH_OPT_FIELD_FP32 = 3 };
+ *                       1: FP32 products.  Ignored with VH_OPT_EXACT_FIELD.
+ *   VH_OPT_POISSON_UNFUSED 0 (default): vh_step computes each level's
+ *                       spline coefficients straight from the density and
+ *                       the boundary, by linearity (the potential's interior
+ *                       is not formed: after vh_step phi holds the boundary
+ *                       faces only); FP32-rounding agreement with the entry
+ *                       points' order.  1: vh_step runs poisson!,
+ *                       spline_coefficients! and effective_potential! as the
+ *                       entry points do (bit-identical to calling them).
+ *                       Meshes in split BF16, or of more than 256 points a
+ *                       side, always take the second form. */
+enum { VH_OPT_EXACT_FIELD = 0, VH_OPT_STEP_GRAPH = 1, VH_OPT_UNIT_SLOTS = 2, VH_OPT_FIELD_FP32 = 3,
+       VH_OPT_POISSON_UNFUSED = 4 };
 int vh_accel_set_option(vh_accel *a, int option, int value);
 
 /* upload!/download! of the particle buffer (src/accelerator.jl:48, 64-68):

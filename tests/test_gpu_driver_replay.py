@@ -12,7 +12,8 @@ the total potential into phi -- then `_advance_cloud!`, the device step!
 Each call is the ABI entry point the Julia extension binds to that method.
 
 Against it, the same cloud stepped by vh_step (one call per step, the
-projectile integrated on the device, the fine coefficients as one transform):
+projectile integrated on the device, the fine coefficients as one transform,
+the solve in the driver's order: VH_OPT_POISSON_UNFUSED):
 the clouds and forces identical to the bit after six steps with the budget
 taken on two of them, the projectile's state to FP64 rounding (the device's
 pow in the jellium term)."""
@@ -72,6 +73,7 @@ def test_driver_call_order_matches_vh_step(config1):
             sims.append(va.Simulation(p, None, projectile=proj, tables=(fine, coarse, sm),
                                       positions=pos, momenta=mom, resident_projectile=resident))
         native, replay = sims
+        native.acc.set_poisson_unfused(True)  # vh_step solving in the driver's order
         for i in range(6):
             energy = i in (0, 3)
             native.step()

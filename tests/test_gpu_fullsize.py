@@ -82,6 +82,7 @@ def test_full_size_step_matches_separate_calls(full):
             s.acc.upload(rec)
             runs.append(s)
         a, b = runs
+        a.acc.set_poisson_unfused(True)  # vh_step solving in the entry points' order
         for _ in range(2):
             a.step()
             b.update_forces(advance=True)
@@ -232,3 +233,19 @@ def test_full_size_energy_and_enclosed(full, orc):
     c = np.array(sim.projectile.position, dtype=np.float64)
     for r in (8.0, 35.0):
         assert acc.enclosed_count(c, r) == orc.enclosed(rec, x0, h, c, r)
+
+
+def test_full_size_fused_step_coefficients_against_oracle(full, orc):
+    # vh_step's default solve (each level's coefficients by linearity, phi's
+    # interior never formed) on the full grids: the fine and coarse
+    # coefficients (Hartree plus mean field) against the oracle's FP64 chain
+    # on the step's own densities, at 1e-5 of the magnitude of the terms
+    p, sim = full
+    sim.step()
+    rho_f, rho_c = sim.dmf.rho.astype(np.float64), sim.dmc.rho.astype(np.float64)
+    wf, wc, _ = orc.poisson_nested(rho_f, rho_c, sim.fine, sim.coarse)
+    for mesh, dm, phi, rho in ((sim.fine, sim.dmf, wf, rho_f), (sim.coarse, sim.dmc, wc, rho_c)):
+        hartree = orc.spline_coefficients(phi, mesh)
+        extra = orc.spline_coefficients(orc.effective_extra(rho, mesh, sim.jellium), mesh)
+        scale = max(np.max(np.abs(hartree)), np.max(np.abs(extra)))
+        assert np.max(np.abs(dm.csol - (hartree + extra))) / scale < 1e-5

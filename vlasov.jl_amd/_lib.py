@@ -19,6 +19,7 @@ VH_OPT_EXACT_FIELD = 0
 VH_OPT_STEP_GRAPH = 1
 VH_OPT_UNIT_SLOTS = 2
 VH_OPT_FIELD_FP32 = 3
+VH_OPT_POISSON_UNFUSED = 4
 VH_MESH_OPT_BF16X3 = 0
 STAGES = ("sort", "deposit_fine", "deposit_cic", "poisson", "spline", "meanfield", "field",
           "verlet", "field_kernel", "deposit_kernel")

@@ -95,7 +95,8 @@ extern "C" int vh_context_create(int device, vh_context **out) {
   // the side stream at the default priority, like the main one
   if ((e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking)) != hipSuccess ||
       (e = hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming)) != hipSuccess ||
-      (e = hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming)) != hipSuccess) {
+      (e = hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming)) != hipSuccess ||
+      (e = hipEventCreateWithFlags(&c->ev_a, hipEventDisableTiming)) != hipSuccess) {
     vh_set_error("vh_context_create: %s", hipGetErrorString(e));
     hipStreamDestroy(c->stream);
     delete c;
@@ -115,6 +116,7 @@ extern "C" int vh_context_destroy(vh_context *ctx) {
   VH_CHECK_HIP(hipStreamSynchronize(ctx->side));
   VH_CHECK_HIP(hipEventDestroy(ctx->fork_ev));
   VH_CHECK_HIP(hipEventDestroy(ctx->join_ev));
+  VH_CHECK_HIP(hipEventDestroy(ctx->ev_a));
   VH_CHECK_HIP(hipStreamDestroy(ctx->side));
   VH_CHECK_HIP(hipStreamDestroy(ctx->stream));
   delete ctx;
@@ -167,7 +169,8 @@ extern "C" int vh_mesh_destroy(vh_mesh *m) {
   void *ptrs[] = {m->colloc, m->knots, m->colloc64, m->knots64, m->lap, m->Minv_op, m->M_op,
                   m->Sinv_op, m->invlsum, m->dual, m->dual32, m->duallen, m->loc_cells,
                   m->rho, m->phi, m->csol, m->extra, m->work, m->inner, m->work1, m->work2,
-                  m->partials, m->scal, m->acc64, m->Minv_b3, m->M_b3, m->Sinv_b3};
+                  m->partials, m->scal, m->acc64, m->Minv_b3, m->M_b3, m->Sinv_b3,
+                  m->Pt_op, m->ulr, m->Minv_t, m->face};
   for (void *p : ptrs) vh_free(p);
   delete m;
   return VH_OK;
@@ -212,6 +215,36 @@ extern "C" int vh_mesh_create(vh_context *ctx, int n, int nk, const double *coll
   MK(vh_gemm_pack_b3(ctx, &m->Minv_b3, Minv, ni));
   MK(vh_gemm_pack_b3(ctx, &m->M_b3, M, ni));
   MK(vh_gemm_pack_b3(ctx, &m->Sinv_b3, Sinv, n));
+  {  // the operators of the solve by linearity, in FP64, rounded once
+    // P~ = S^-1 M~: column 0 and n - 1 of S^-1, interior columns S^-1 E M
+    std::vector<double> pt((size_t)n * n, 0.0);
+    for (int row = 0; row < n; ++row) {
+      pt[row] = Sinv[row];
+      pt[row + (size_t)n * (n - 1)] = Sinv[row + (size_t)n * (n - 1)];
+      for (int cc = 0; cc < ni; ++cc) {
+        double acc = 0.0;
+        for (int q = 0; q < ni; ++q) acc += Sinv[row + (size_t)n * (q + 1)] * M[q + (size_t)ni * cc];
+        pt[row + (size_t)n * (cc + 1)] = acc;
+      }
+    }
+    MK(vh_gemm_pack(ctx, &m->Pt_op, pt.data(), n));
+    // u = M^-1 of the outer columns of D at the interior rows (the faces'
+    // lift into the right-hand side, src/poisson.jl:275-326)
+    std::vector<double> u(2 * (size_t)ni), mt((size_t)ni * ni);
+    for (int a = 0; a < ni; ++a) {
+      double ul = 0.0, ur = 0.0;
+      for (int q = 0; q < ni; ++q) {
+        ul += Minv[a + (size_t)ni * q] * lap[(q + 1) + (size_t)n * 0];
+        ur += Minv[a + (size_t)ni * q] * lap[(q + 1) + (size_t)n * (n - 1)];
+        mt[(size_t)q * ni + a] = Minv[a + (size_t)ni * q];  // transposed
+      }
+      u[a] = ul;
+      u[ni + a] = ur;
+    }
+    MK(upload_converted(ctx, &m->ulr, u.data(), u.size()));
+    MK(upload_converted(ctx, &m->Minv_t, mt.data(), mt.size()));
+    MK(vh_alloc((void **)&m->face, sizeof(float) * (12 * (size_t)ni * ni + 12 * (size_t)ni)));
+  }
   MK(upload_converted(ctx, &m->dual, dual, 3 * (size_t)n));
   MK(upload_converted(ctx, &m->dual32, dual, 3 * (size_t)n));
   MK(upload_converted(ctx, &m->duallen, duallen, n));
@@ -260,6 +293,7 @@ extern "C" int vh_mesh_create(vh_context *ctx, int n, int nk, const double *coll
   MK(vh_fill_f32(ctx, m->inner, (int64_t)gi, 0.f));
   MK(vh_fill_f32(ctx, m->work1, (int64_t)gi, 0.f));
   MK(vh_fill_f32(ctx, m->work2, (int64_t)gi, 0.f));
+  MK(vh_fill_f32(ctx, m->face, 12 * (int64_t)ni * ni + 12 * (int64_t)ni, 0.f));
   MK(vh_fill_u64(ctx, (uint64_t *)m->acc64, (int64_t)g, 0ull));
   MK(vh_fill_u64(ctx, (uint64_t *)m->partials, (int64_t)10 * n * n, 0ull));
   MK(vh_fill_u64(ctx, (uint64_t *)m->scal, 32, 0ull));
@@ -524,6 +558,14 @@ extern "C" int vh_accel_set_option(vh_accel *a, int option, int value) {
       VH_REQUIRE(value == 0 || value == 1, "vh_accel_set_option: FP32 field products take 0 or 1");
       if (value != a->field_fp32) {
         a->field_fp32 = value;
+        drop_graphs(a);
+        ++a->config_version;
+      }
+      return VH_OK;
+    case VH_OPT_POISSON_UNFUSED:
+      VH_REQUIRE(value == 0 || value == 1, "vh_accel_set_option: unfused Poisson takes 0 or 1");
+      if (value != a->poisson_unfused) {
+        a->poisson_unfused = value;
         drop_graphs(a);
         ++a->config_version;
       }
@@ -1424,6 +1466,44 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
     // mean field started 90 us late).  A stream wait binds to the event's
     // last record at the time of the call, so the fork event serves again
     // for the second join.
+    // The solve by linearity (vh_poisson_fused_*): the density part of each
+    // level before its boundary is known, three products after it, and phi's
+    // interior never formed; off with VH_OPT_POISSON_UNFUSED (and on meshes
+    // in split BF16 or of more than VH_FUSED_MAXN points a side), where the
+    // step solves as the entry points do.
+    const bool fused = !a->poisson_unfused && !fine->gemm_bf16x3 && !coarse->gemm_bf16x3 &&
+                       fine->n <= VH_FUSED_MAXN && coarse->n <= VH_FUSED_MAXN;
+    if (fused) {
+      // On the side stream the CIC, the coarse level, the fine boundary and
+      // then the coarse mean field; on the main stream the fine deposit, the
+      // fine mean field and the density part of the fine solve, which fill
+      // its wait for the boundary, then the fine level.  (Measured against
+      // other arrangements: the coarse mean field kept apart on the main
+      // stream during the wait, 1.483 ms a step; on the side stream after
+      // the boundary and summed on the main stream, 1.481; after the fine
+      // level on the main stream, 1.51; this one 1.475.  A launch queued
+      // behind a wait for the other, busy, stream started tens of us late.)
+      VH_CHECK_HIP(hipEventRecord(c->fork_ev, c->stream));
+      VH_TRY(vh_deposit_fine_impl(a, fine, weight));
+      if (a->denoiser) VH_TRY(vh_denoiser_apply_impl(a->denoiser, fine, a->denoise_tol));
+      VH_TRY(vh_mean_field_coefficients_impl(fine, jel_nions, jel_radius));
+      VH_TRY(vh_poisson_fused_prepare_impl(fine));
+      VH_CHECK_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
+      {
+        vh_on_side on(c);
+        VH_TRY(vh_deposit_cic_impl(a, coarse, weight));
+        VH_TRY(vh_poisson_multipole_boundary_impl(coarse));
+        VH_TRY(vh_poisson_fused_prepare_impl(coarse));
+        VH_TRY(vh_poisson_fused_finish_impl(coarse, false));
+        VH_TRY(vh_poisson_fine_boundary_impl(fine, coarse));
+        VH_CHECK_HIP(hipEventRecord(c->join_ev, c->side));
+        VH_TRY(vh_effective_potential_impl(coarse, jel_nions, jel_radius));
+        VH_CHECK_HIP(hipEventRecord(c->ev_a, c->side));
+      }
+      VH_CHECK_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
+      VH_TRY(vh_poisson_fused_finish_impl(fine, true));
+      VH_CHECK_HIP(hipStreamWaitEvent(c->stream, c->ev_a, 0));
+    } else {
     VH_CHECK_HIP(hipEventRecord(c->fork_ev, c->stream));
     VH_TRY(vh_deposit_fine_impl(a, fine, weight));
     if (a->denoiser) VH_TRY(vh_denoiser_apply_impl(a->denoiser, fine, a->denoise_tol));
@@ -1444,6 +1524,7 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
     VH_TRY(vh_poisson_fine_part_impl(fine));
     VH_TRY(vh_hartree_coefficients_add_impl(fine));
     VH_CHECK_HIP(hipStreamWaitEvent(c->stream, c->fork_ev, 0));
+    }
   }
   if (step_fuses_push(a)) {
     // the field and fallback kernels push the particles they computed the

@@ -60,7 +60,7 @@ struct vh_context {
   // a second stream for work of a step that is independent of the main
   // chain (the coarse mean field beside the fine solve), joined by events
   hipStream_t side;
-  hipEvent_t fork_ev, join_ev;
+  hipEvent_t fork_ev, join_ev, ev_a;
   int num_cus;
   // timing of regions with events on `stream`
   int prof_on;                          // mask of the stages timed
@@ -86,6 +86,13 @@ struct vh_mesh {
   // whether the mesh's products run in split-BF16 x3 (VH_MESH_OPT_BF16X3)
   uint32_t *Minv_b3, *M_b3, *Sinv_b3;
   int gemm_bf16x3;
+  // the solve and the coefficients in one (vh_step): P~ = S^-1 M~ (n x n,
+  // vh_gemm_pack order), M~ = M on the interior indices and the identity on
+  // 0 and n - 1; u = M^-1 D[1:n-1, 0] then M^-1 D[1:n-1, n-1] (2 ni); M^-1
+  // transposed ([q ni + c] = M^-1[c, q]); the six faces' 2-D transforms by
+  // M^-1 (6 ni^2), the twelve edges' 1-D transforms (12 ni), then the faces'
+  // first pass (6 ni^2)
+  float *Pt_op, *ulr, *Minv_t, *face;
   float *invlsum;    // ni^3, i fastest
   double *dual;      // 3 x n (order k at k*n), FP64
   float *dual32;     // same, narrowed
@@ -301,6 +308,7 @@ struct vh_accel {
   int nout_idx;                // counters[nout_idx]: the deposit's rejections of this sort
   int pushed_fused;            // host flag: the field kernel's push wrote the records
   int exact_field;             // VH_OPT_EXACT_FIELD: the reference's FMA order
+  int poisson_unfused;         // VH_OPT_POISSON_UNFUSED: vh_step forms phi (the entry points' order)
   int field_fp32;              // VH_OPT_FIELD_FP32: FP32 products in the first stage
   int step_graph;              // VH_OPT_STEP_GRAPH
   hipGraphExec_t graph[2];     // vh_step recorded with parts[cur] = parts[0], parts[1]
@@ -399,6 +407,16 @@ int vh_scan_exclusive(vh_context *c, uint32_t *data, int64_t n, uint32_t *tmp, i
 // A given in MFMA operand order by vh_gemm_pack (device copy allocated there)
 enum { EPI_STORE = 0, EPI_HADAMARD = 1, EPI_FILL_INTERIOR = 2, EPI_ACCUMULATE = 3 };
 int vh_gemm_pack(vh_context *c, float **dst, const double *colmajor, int n);
+// the solve of a level and its spline coefficients by linearity (vh_step):
+// prepare = M^-1 applied to the density part of the right-hand side (needs
+// the density only, kept in inner); finish = once phi's faces hold the
+// boundary, csol (=, or += with accumulate) = S^-1 of the solution with
+// those faces, without forming phi's interior
+#define VH_FUSED_MAXN 256   // the face passes stage one row of at most 256 points
+int vh_poisson_fused_prepare_impl(vh_mesh *m);
+int vh_poisson_fused_finish_impl(vh_mesh *m, bool accumulate);
+// the coarse level's multipole boundary on phi's faces
+int vh_poisson_multipole_boundary_impl(vh_mesh *m);
 int vh_gemm_pack_b3(vh_context *c, uint32_t **dst, const double *colmajor, int n);
 int vh_rotate_gemm(vh_context *c, const float *Aop, const float *src, float *out, int n, int64_t m,
                    int epi, const float *aux, int nfull);

@@ -994,12 +994,17 @@ static int solve_interior(vh_mesh *m) {
   return VH_OK;
 }
 
-int vh_poisson_single_impl(vh_mesh *m) {
+int vh_poisson_multipole_boundary_impl(vh_mesh *m) {
   vh_context *c = m->ctx;
   VH_TRY(vh_multipole_impl(m));
   hipLaunchKernelGGL(k_boundary_potential, dim3(512), dim3(256), 0, c->stream, m->phi, m->scal + 10,
                      m->colloc64, m->n);
   VH_LAUNCH_CHECK();
+  return VH_OK;
+}
+
+int vh_poisson_single_impl(vh_mesh *m) {
+  VH_TRY(vh_poisson_multipole_boundary_impl(m));
   VH_TRY(solve_interior(m));
   return VH_OK;
 }
@@ -1037,6 +1042,183 @@ int vh_poisson_fine_part_impl(vh_mesh *fine) { return solve_interior(fine); }
 int vh_poisson_nested_impl(vh_mesh *fine, vh_mesh *coarse) {
   VH_TRY(vh_poisson_coarse_part_impl(fine, coarse));
   VH_TRY(vh_poisson_fine_part_impl(fine));
+  return VH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The solve of a level and its spline coefficients by linearity.
+// With E the embedding of the interior (ni = n - 2 points a side) and B the
+// boundary values on phi's faces, solve_interior! then spline_coefficients!
+// compute
+//   csol = S^-1 (E (M Lambda^-1 M^-1 rhs) + B),  rhs = c rho_int - lift(B)
+// with the three-axis products written as operators.  Let M~ be M on the
+// interior indices and the identity on 0 and n - 1: then
+//   phi = M~ Z~,  csol = (S^-1 M~) Z~ = P~ Z~,
+// where Z~ = Lambda^-1 (M^-1 rhs) inside and, on the shell, the boundary
+// values transformed by M^-1 along their interior axes (faces by two, edges
+// by one, corners as they are), since M~ maps exactly those back to B.  The
+// lift of the faces into rhs is a sum of six terms d (x) face, so
+//   M^-1 rhs = M^-1 (c rho_int) - sum_f u_f (x) H_f,
+// u_f = M^-1 d_f (two vectors, built at mesh creation) and H_f the 2-D
+// transform of face f's interior by M^-1 -- the same H_f that fill the
+// shell.  Hence: prepare (density only) = M^-1 (c rho_int); finish (once the
+// faces are known) = the six H_f (two rotating products of the face stack),
+// Z~ in one pass, and three products by P~: three products after the
+// boundary instead of nine, and phi's interior is never formed.  P~ and u are
+// formed in FP64 and rounded once.
+// ---------------------------------------------------------------------------
+// c rho at the interior points, ni^3 (i fastest); block (j, k), thread i
+__global__ void k_rho_interior(float *__restrict__ inner, const float *__restrict__ rho, int n) {
+  const int ni = n - 2, i = threadIdx.x, j = blockIdx.x, k = blockIdx.y;
+  if (i >= ni) return;
+  const int64_t nn = n;
+  inner[i + (int64_t)ni * (j + (int64_t)ni * k)] =
+      (-4.0f * 3.14159265358979323846f) * rho[(i + 1) + nn * ((j + 1) + nn * (k + 1))];
+}
+
+// The faces' and edges' transforms by M^-1, read straight from phi's shell,
+// in two passes of one block per row.  Face f's interior values F(p, q)
+// (x faces (i = 0, n - 1): p = j, q = k; y faces p = i, q = k; z faces
+// p = i, q = j; interior indices, 0-based) go to H_f = M^-1 F M^-T:
+//   pass 1, block (p, f): T_f(p, q') = sum_q F(p, q) M^-1[q', q];
+//   pass 2, block (p', f): H_f(p', q') = sum_p M^-1[p', p] T_f(p, q'), at
+//   f + 6 p' + 6 ni q'; blocks past the faces' take the twelve edges
+//   (edge e: axis = e / 4 the interior axis, 0 i, 1 j, 2 k, the two other
+//   indices 0 or n - 1 by bits 0 and 1 of e % 4, the lower axis first):
+//   E(c, e) = sum_q M^-1[c, q] line(q), at c 12 + e.
+// The row a block contracts sits in LDS; thread q' (or c) reads
+// minvt = M^-1 transposed ([q ni + c]) at consecutive words.
+__device__ __forceinline__ int64_t face_point(int f, int p, int q, int n) {
+  const int64_t nn = n;
+  const int side = (f & 1) ? n - 1 : 0;
+  if (f < 2) return side + nn * ((p + 1) + nn * (q + 1));
+  if (f < 4) return (p + 1) + nn * (side + nn * (q + 1));
+  return (p + 1) + nn * ((q + 1) + nn * side);
+}
+
+// sum_q row[q] m[q ni + c], eight independent partial sums
+__device__ __forceinline__ float row_dot(const float *row, const float *__restrict__ m, int ni, int c) {
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  int q = 0;
+  for (; q + 8 <= ni; q += 8) {
+#pragma unroll
+    for (int v = 0; v < 8; ++v) acc[v] = __builtin_fmaf(row[q + v], m[(int64_t)(q + v) * ni + c], acc[v]);
+  }
+  for (; q < ni; ++q) acc[0] = __builtin_fmaf(row[q], m[(int64_t)q * ni + c], acc[0]);
+  return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+}
+
+__global__ void __launch_bounds__(256)
+    k_face_pass1(float *__restrict__ T, const float *__restrict__ phi, const float *__restrict__ minvt, int n) {
+  __shared__ float row[256];
+  const int ni = n - 2, p = blockIdx.x, f = blockIdx.y, t = threadIdx.x;
+  for (int q = t; q < ni; q += blockDim.x) row[q] = phi[face_point(f, p, q, n)];
+  __syncthreads();
+  for (int c = t; c < ni; c += blockDim.x)
+    T[(int64_t)ni * ((int64_t)ni * f + p) + c] = row_dot(row, minvt, ni, c);
+}
+
+__global__ void __launch_bounds__(256)
+    k_face_pass2(float *__restrict__ H, float *__restrict__ E, const float *__restrict__ T,
+                 const float *__restrict__ phi, const float *__restrict__ minvt, int n) {
+  __shared__ float row[256];
+  const int ni = n - 2, t = threadIdx.x;
+  const int64_t nn = n;
+  if (blockIdx.y == 6) {
+    // an edge
+    const int e = blockIdx.x;
+    if (e >= 12) return;
+    const int axis = e >> 2, s0 = (e & 1) ? n - 1 : 0, s1 = (e & 2) ? n - 1 : 0;
+    int64_t base, step;
+    if (axis == 0) { base = nn * (s0 + nn * s1); step = 1; }          // j = s0, k = s1
+    else if (axis == 1) { base = s0 + nn * nn * s1; step = nn; }      // i = s0, k = s1
+    else { base = s0 + nn * s1; step = nn * nn; }                     // i = s0, j = s1
+    for (int q = t; q < ni; q += blockDim.x) row[q] = phi[base + step * (q + 1)];
+    __syncthreads();
+    for (int c = t; c < ni; c += blockDim.x) E[12 * c + e] = row_dot(row, minvt, ni, c);
+    return;
+  }
+  // H_f(p', q') = sum_p M^-1[p', p] T_f(p, q'): M^-1's row p' in LDS, T read
+  // at consecutive q'
+  const int pp = blockIdx.x, f = blockIdx.y;
+  for (int q = t; q < ni; q += blockDim.x) row[q] = minvt[(int64_t)q * ni + pp];
+  __syncthreads();
+  const float *Tf = T + (int64_t)ni * ni * f;
+  for (int c = t; c < ni; c += blockDim.x) H[f + 6 * pp + 6 * (int64_t)ni * c] = row_dot(row, Tf, ni, c);
+}
+
+// Z~ on the n^3 grid (i fastest): Y = M^-1 (c rho_int) in `inner`, H the
+// face transforms (H_f(p, q) at f + 6 p + 6 ni q), E the edges' (E(c, e) at
+// c 12 + e); block (j, k), thread i
+__global__ void k_ztilde(float *__restrict__ Z, const float *__restrict__ Y,
+                         const float *__restrict__ invl, const float *__restrict__ H,
+                         const float *__restrict__ u, const float *__restrict__ E,
+                         const float *__restrict__ phi, int n) {
+  const int ni = n - 2, i = threadIdx.x, j = blockIdx.x, k = blockIdx.y;
+  if (i >= n) return;
+  const int64_t nn = n, hq = 6 * (int64_t)ni;
+  const int64_t t = i + nn * (j + nn * k);
+  const float *ul = u, *ur = u + ni;
+  const bool bi = i == 0 || i == n - 1, bj = j == 0 || j == n - 1, bk = k == 0 || k == n - 1;
+  const int nb = (int)bi + (int)bj + (int)bk;
+  const int a = i - 1, b = j - 1, c = k - 1;
+  float z;
+  if (nb == 0) {
+    const int64_t g = a + (int64_t)ni * (b + (int64_t)ni * c);
+    // the six lifted faces, as M^-1 carries them: u_f (x) H_f
+    const float2 hx = *(const float2 *)(H + 6 * b + hq * c);
+    const float2 hy = *(const float2 *)(H + 2 + 6 * a + hq * c);
+    const float2 hz = *(const float2 *)(H + 4 + 6 * a + hq * b);
+    float lift = ul[a] * hx.x;
+    lift = __builtin_fmaf(ur[a], hx.y, lift);
+    lift = __builtin_fmaf(ul[b], hy.x, lift);
+    lift = __builtin_fmaf(ur[b], hy.y, lift);
+    lift = __builtin_fmaf(ul[c], hz.x, lift);
+    lift = __builtin_fmaf(ur[c], hz.y, lift);
+    z = invl[g] * (Y[g] - lift);
+  } else if (nb == 1) {
+    if (bi) z = H[(i ? 1 : 0) + 6 * b + hq * c];
+    else if (bj) z = H[(j ? 3 : 2) + 6 * a + hq * c];
+    else z = H[(k ? 5 : 4) + 6 * a + hq * b];
+  } else if (nb == 2) {
+    // an edge: M^-1 along its interior axis
+    const int axis = !bi ? 0 : !bj ? 1 : 2;
+    const int lo = axis == 0 ? j : i, hi = axis == 2 ? j : k;
+    const int e = 4 * axis + (lo ? 1 : 0) + (hi ? 2 : 0);
+    z = E[12 * (axis == 0 ? a : axis == 1 ? b : c) + e];
+  } else {
+    z = phi[t];
+  }
+  Z[t] = z;
+}
+
+int vh_poisson_fused_prepare_impl(vh_mesh *m) {
+  vh_context *c = m->ctx;
+  VH_REQUIRE(m->n <= 256, "fused solve: n = %d beyond 256", m->n);
+  hipLaunchKernelGGL(k_rho_interior, dim3(m->ni, m->ni), dim3((m->ni + 63) / 64 * 64), 0, c->stream,
+                     m->inner, m->rho, m->n);
+  VH_LAUNCH_CHECK();
+  // inner -> work1 -> work2 -> inner
+  VH_TRY(apply_all(c, vh_op{m->Minv_op, nullptr}, m->ni, m->inner, m->work1, m->work2, m->inner, EPI_STORE,
+                   nullptr, 0));
+  return VH_OK;
+}
+
+int vh_poisson_fused_finish_impl(vh_mesh *m, bool accumulate) {
+  vh_context *c = m->ctx;
+  const int n = m->n, ni = m->ni;
+  VH_REQUIRE(n <= 256 && ni >= 12, "fused solve: n = %d outside [14, 256]", n);
+  float *H = m->face, *E = H + 6 * (int64_t)ni * ni, *T = E + 12 * (int64_t)ni;
+  hipLaunchKernelGGL(k_face_pass1, dim3(ni, 6), dim3(128), 0, c->stream, T, m->phi, m->Minv_t, n);
+  VH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_face_pass2, dim3(ni, 7), dim3(128), 0, c->stream, H, E, T, m->phi, m->Minv_t, n);
+  VH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_ztilde, dim3(n, n), dim3((n + 63) / 64 * 64), 0, c->stream, m->extra, m->inner,
+                     m->invlsum, H, m->ulr, E, m->phi, n);
+  VH_LAUNCH_CHECK();
+  // extra -> work -> extra -> csol
+  VH_TRY(apply_all(c, vh_op{m->Pt_op, nullptr}, n, m->extra, m->work, m->extra, m->csol,
+                   accumulate ? EPI_ACCUMULATE : EPI_STORE, nullptr, 0));
   return VH_OK;
 }
 

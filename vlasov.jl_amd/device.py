@@ -197,6 +197,12 @@ class DeviceAccelerator:
         accumulation, about 2^-21 of the largest term)."""
         _lib.call("vh_accel_set_option", self.handle, _lib.VH_OPT_FIELD_FP32, 1 if fp32 else 0)
 
+    def set_poisson_unfused(self, unfused: bool):
+        """True: vh_step solves and transforms as the entry points do
+        (bit-identical to calling them); False (default): each level's
+        coefficients straight from the density and the boundary."""
+        _lib.call("vh_accel_set_option", self.handle, _lib.VH_OPT_POISSON_UNFUSED, 1 if unfused else 0)
+
     def set_step_graph(self, on: bool = True):
         """vh_step recorded once into a HIP graph per particle buffer and
         replayed (VH_OPT_STEP_GRAPH)."""
