@@ -1,6 +1,7 @@
 // Context, meshes, accelerators, buffers and the public ABI wrappers.
 #include "vh_internal.h"
 
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -193,6 +194,8 @@ extern "C" int vh_mesh_create(vh_context *ctx, int n, int nk, const double *coll
   VH_CHECK_HIP(hipSetDevice(ctx->device));
   vh_mesh *m = new vh_mesh();
   memset(m, 0, sizeof *m);
+  static std::atomic<uint64_t> mesh_serial{0};
+  m->serial = ++mesh_serial;
   m->ctx = ctx;
   m->n = n;
   m->nk = nk;
@@ -391,7 +394,7 @@ extern "C" int vh_accel_destroy(vh_accel *a) {
   void *ptrs[] = {a->ovl, a->grad, a->nodes, a->og, a->xsplit, a->ovl64, a->knots64, a->parts[0], a->parts[1], a->force,
                   a->keys[0], a->keys[1], a->vals[0], a->vals[1], a->counts, a->scan_tmp,
                   a->occupied, a->bounds, a->cellmap, a->fitlist, a->nonfit, a->nonfit_prefix,
-                  a->outlist, a->counters, a->cellrun, a->tiles, a->cicwins, a->dpart, a->dscal, a->cellscratch,
+                  a->outlist, a->counters, a->cellrun, a->tiles, a->cicwins, a->cicwtab, a->dpart, a->dscal, a->cellscratch,
                   a->dproj, a->units, a->unitstart, a->zero, a->ident, a->pos64, a->cloud_perm,
                   a->host_csol64, a->host_knots64, a->force64};
   for (void *p : ptrs) vh_free(p);
@@ -526,6 +529,8 @@ extern "C" int vh_accel_create(vh_context *ctx, int64_t npart, int nk, const dou
   MK(vh_alloc((void **)&a->cellrun, sizeof(int32_t) * 2 * a->ncell));
   MK(vh_alloc((void **)&a->tiles, sizeof(float) * 512 * (size_t)a->maxunits));
   MK(vh_alloc((void **)&a->cicwins, sizeof(long long) * 64 * (size_t)a->maxunits));
+  MK(vh_alloc((void **)&a->cicwtab, sizeof(vh_cicwin) * (size_t)nk));
+  a->cicwtab_serial = 0;
   a->dpart_len = 1 << 16;
   MK(vh_alloc((void **)&a->dpart, sizeof(double) * 8 * a->dpart_len));
   MK(vh_alloc((void **)&a->dscal, sizeof(double) * 64));

@@ -352,48 +352,93 @@ __device__ __forceinline__ void fx_add(unsigned long long *acc, int64_t idx, flo
   if (v != 0.f) atomicAdd(acc + idx, fx_of(v, scale));
 }
 
+// The window of every fine cell index of an axis (vh_cicwin): the coarse
+// interval of the cell's lower corner by locate_cell, as the windows of the
+// deposit are anchored.  The three axes share the fine and coarse axes.
+__global__ void k_cic_window_table(int nkf, float x0f, float hf, const float *__restrict__ gt, int n,
+                                   float x0t, float iwt, const int32_t *__restrict__ cells,
+                                   int nbin, vh_cicwin *__restrict__ tab) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nkf) return;
+  float w;
+  const float xl = x0f + ((float)t - 0.5f) * hf;   // fine cell index kc = t + 1
+  vh_cicwin e;
+  e.a = locate_cell(gt, n, x0t, iwt, cells, nbin, xl, w);
+  const int a = e.a > 0 ? e.a : 1;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) e.g[j] = gt[min(a - 1 + j, n - 1)];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) e.r[j] = e.g[j + 1] > e.g[j] ? 1.f / (e.g[j + 1] - e.g[j]) : 0.f;
+  tab[t] = e;
+}
+
+// Position of a particle in its cell's window along one axis: the linear
+// weights X[0..3] on the window's four points (left point weight
+// (g[q + 1] - p) / (g[q + 1] - g[q]) at q, the rest at q + 1, q the interval
+// of p); in: g[0] < p <= g[3], i.e. the interval locate_cell finds for p is
+// one of the window's three.  Outside, X holds finite values to be ignored.
+// g = the window's points, r = (r[0], r[1], r[2], -): the entry of
+// vh_cicwin after its anchor, in registers.
+__device__ __forceinline__ void cic_axis(float p, float4 g, float4 r, float (&X)[4], bool &in) {
+  in = p > g.x && p <= g.w;
+  const bool s1 = p > g.y, s2 = p > g.z;
+  const float hi = s2 ? g.w : (s1 ? g.z : g.y);
+  const float rr = s2 ? r.z : (s1 ? r.y : r.x);
+  const float a = in ? (hi - p) * rr : 0.f;
+  const float b = 1.f - a;
+  X[0] = s1 ? 0.f : a;
+  X[1] = s2 ? 0.f : (s1 ? a : b);
+  X[2] = s2 ? a : (s1 ? b : 0.f);
+  X[3] = s2 ? b : 0.f;
+}
+
+// a window entry (vh_cicwin, 32 bytes) as anchor, points, reciprocals
+struct cic_win {
+  int a;
+  float4 g, r;
+};
+__device__ __forceinline__ cic_win cic_load(const vh_cicwin *__restrict__ tab, int i) {
+  const float4 *q = reinterpret_cast<const float4 *>(tab + i);
+  const float4 lo = q[0], hi = q[1];
+  cic_win w;
+  w.a = __float_as_int(lo.x);
+  w.g = make_float4(lo.y, lo.z, lo.w, hi.x);
+  w.r = make_float4(hi.y, hi.z, hi.w, 0.f);
+  return w;
+}
+
 // One wave per work unit (at most VH_UNIT particles of one occupied fine
-// cell), CIC_WAVES waves per block sharing the coarse axis's collocation
-// points and locate table in LDS.  The unit's particles fall into a 4^3 window of coarse points
-// anchored at the coarse interval of the cell's lower corner; the window's
-// sums are a rank-P update
+// cell), CIC_WAVES waves per block.  The unit's particles fall into a 4^3
+// window of coarse points anchored at the coarse interval of the cell's
+// lower corner (vh_cicwin, one table entry per axis); the window's sums are
+// a rank-P update
 //   W[x + 4 y][z] = sum_p (X_p[x] Y_p[y]) Z_p[z],
 // X_p = the particle's two linear weights at its window offsets along x (zero
 // elsewhere), run on v_mfma_f32_16x16x4_f32: A = the 16 products X Y of four
 // particles, B = their z weights, each particle in its own four-column group
 // (B[k][4 g + z] = Z_k[z] when g = k), so that a product takes four particles
-// and the four groups are four partial sums.  Every 64 particles the FP32
-// sums are converted to 64-bit fixed point and added in registers; at the
-// end of the unit the four groups are added (integers: exact).  The window
+// and the four groups are four partial sums.  The unit's FP32 sums are added
+// across the groups and converted to 64-bit fixed point once.  The window
 // of a cell's only unit is committed with global 64-bit atomics; a cell cut
 // into several units writes one window per unit, and k_cic_flush adds them
 // and commits once per cell, which keeps the atomics on the hot points of
-// the cluster's core few.  Particles whose window offset falls outside the
-// window go straight to the global accumulator.  Integer sums do not depend
-// on order, so the result is reproducible.
+// the cluster's core few.  Particles outside their window go straight to
+// the global accumulator.  Integer sums do not depend on order, and the
+// FP32 sums of a unit are formed in a fixed order: the result is
+// reproducible.
 #define CIC_STRIDE 21   // 16 products + 4 z weights, odd row stride
-#define CIC_WAVES 4     // waves per block, each on its own units; the block shares the tables
-
-// dynamic LDS of k_cic_tiles: per wave 64 staging rows, then the collocation
-// points and the locate table of the coarse axis
-static inline size_t cic_lds_bytes(int n, int nbin) {
-  return sizeof(float) * ((size_t)CIC_WAVES * 64 * CIC_STRIDE + n + nbin);
-}
+#define CIC_WAVES 4     // waves per block, each on its own units
 
 __global__ void __launch_bounds__(64 * CIC_WAVES)
     k_cic_tiles(const int32_t *__restrict__ P, const uint32_t *__restrict__ perm,
                 const vh_unit *__restrict__ units,
                 const int32_t *__restrict__ unitstart, const int32_t *__restrict__ cnt, int nkf,
-                float x0f, float hf, const float *__restrict__ gt_g, int n, float x0t, float iwt,
-                const int32_t *__restrict__ cells_g, int nbin, double scale,
+                float x0f, float hf, const vh_cicwin *__restrict__ wtab,
+                const float *__restrict__ gt, int n, float x0t, float iwt,
+                const int32_t *__restrict__ cells, int nbin, double scale,
                 unsigned long long *__restrict__ acc, long long *__restrict__ wins) {
-  extern __shared__ float lds_cic[];
-  float *gt = lds_cic + CIC_WAVES * 64 * CIC_STRIDE;
-  int32_t *cells = (int32_t *)(gt + n);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) gt[i] = gt_g[i];
-  for (int i = threadIdx.x; i < nbin; i += blockDim.x) cells[i] = cells_g[i];
-  __syncthreads();
-  const int wv = threadIdx.x >> 6;
+  __shared__ float lds_cic[CIC_WAVES * 64 * CIC_STRIDE];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   float *st = lds_cic + wv * 64 * CIC_STRIDE;
   const int l = threadIdx.x & 63;
   const int kq = l >> 4, xy = l & 15, grp = (l & 15) >> 2, zc = l & 3;
@@ -401,17 +446,10 @@ __global__ void __launch_bounds__(64 * CIC_WAVES)
   for (int u = blockIdx.x * CIC_WAVES + wv; u < nunits; u += gridDim.x * CIC_WAVES) {
     const vh_unit un = units[u];
     const int key = un.key;
-    const int kc[3] = {key % nkf + 1, (key / nkf) % nkf + 1, key / (nkf * nkf) + 1};
-    // window anchor: coarse interval of the cell's lower corner, per axis
-    int anc[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      float w;
-      const float xl = x0f + ((float)(kc[d] - 1) - 0.5f) * hf;
-      anc[d] = locate_cell(gt, n, x0t, iwt, cells, nbin, xl, w);  // 0: every particle escapes
-    }
-    const bool anchored = anc[0] > 0 && anc[1] > 0 && anc[2] > 0;
-    long long sum[4] = {0, 0, 0, 0};
+    const cic_win wx = cic_load(wtab, key % nkf), wy = cic_load(wtab, (key / nkf) % nkf),
+                  wz = cic_load(wtab, key / (nkf * nkf));
+    const bool anchored = wx.a > 0 && wy.a > 0 && wz.a > 0;
+    vh_f4d d0 = {0.f, 0.f, 0.f, 0.f}, d1 = {0.f, 0.f, 0.f, 0.f};
     // slot s's record is record perm[s]; the index is loaded a chunk ahead
     // of the record
     vh_pos rn = vh_load_pos(P, perm[un.lo + l < un.hi ? un.lo + l : un.lo]);
@@ -424,27 +462,27 @@ __global__ void __launch_bounds__(64 * CIC_WAVES)
         rn = vh_load_pos(P, inext);
         inext = perm[sn < un.hi ? sn : un.lo];
       }
-      float X[4] = {0.f, 0.f, 0.f, 0.f}, Y[4] = {0.f, 0.f, 0.f, 0.f}, Z[4] = {0.f, 0.f, 0.f, 0.f};
-      int ix[3];
-      float ax[3];
-      bool inside = live;
+      float p[3];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const float p = x0f + (float)(r.k[d] - 1) * hf + r.d[d];
-        ix[d] = locate_cell(gt, n, x0t, iwt, cells, nbin, p, ax[d]);
-        inside = inside && ix[d] > 0;
-      }
-      if (inside) {
-        const int q0 = ix[0] - anc[0], q1 = ix[1] - anc[1], q2 = ix[2] - anc[2];
-        const bool fits = anchored && q0 >= 0 && q0 <= 2 && q1 >= 0 && q1 <= 2 && q2 >= 0 && q2 <= 2;
-        if (fits) {
+      for (int d = 0; d < 3; ++d) p[d] = x0f + (float)(r.k[d] - 1) * hf + r.d[d];
+      float X[4], Y[4], Z[4];
+      bool inx, iny, inz;
+      cic_axis(p[0], wx.g, wx.r, X, inx);
+      cic_axis(p[1], wy.g, wy.r, Y, iny);
+      cic_axis(p[2], wz.g, wz.r, Z, inz);
+      const bool fit = live && anchored && inx && iny && inz;
+      if (live && !fit) {
+        // outside the window: the particle's eight points by the full
+        // locate (src/kernels.jl:1207-1273), straight to the accumulator
+        int ix[3];
+        float ax[3];
+        bool inside = true;
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            X[e] = e == q0 ? ax[0] : (e == q0 + 1 ? 1.f - ax[0] : 0.f);
-            Y[e] = e == q1 ? ax[1] : (e == q1 + 1 ? 1.f - ax[1] : 0.f);
-            Z[e] = e == q2 ? ax[2] : (e == q2 + 1 ? 1.f - ax[2] : 0.f);
-          }
-        } else {
+        for (int d = 0; d < 3; ++d) {
+          ix[d] = locate_cell(gt, n, x0t, iwt, cells, nbin, p[d], ax[d]);
+          inside = inside && ix[d] > 0;
+        }
+        if (inside) {
           const float bx = 1.f - ax[0], by = 1.f - ax[1], bz = 1.f - ax[2];
           const int64_t i0 = ix[0] - 1, j0 = ix[1] - 1, k0 = ix[2] - 1;
           const int64_t nn = n;
@@ -458,42 +496,43 @@ __global__ void __launch_bounds__(64 * CIC_WAVES)
           fx_add(acc, (i0 + 1) + nn * ((j0 + 1) + nn * (k0 + 1)), bx * by * bz, scale);
         }
       }
-      // the particle's row: X[x] Y[y] at x + 4 y, then Z; the wave's own rows
-      // only, so a wave-level order suffices between the writes and the reads
+      // the particle's row: X[x] Y[y] at x + 4 y, then Z (zero unless the
+      // particle is in its window); the wave's own rows only, so a
+      // wave-level order suffices between the writes and the reads
       float *row = st + CIC_STRIDE * l;
 #pragma unroll
       for (int y = 0; y < 4; ++y)
 #pragma unroll
         for (int x = 0; x < 4; ++x) row[x + 4 * y] = X[x] * Y[y];
 #pragma unroll
-      for (int z = 0; z < 4; ++z) row[16 + z] = Z[z];
+      for (int z = 0; z < 4; ++z) row[16 + z] = fit ? Z[z] : 0.f;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
       // groups of four particles, alternately into two accumulators (two
-      // independent product chains); rows past the chunk are zero
+      // independent product chains); rows past the chunk have zero z weights
       const int ng = (min(64, un.hi - c0) + 3) >> 2;
-      vh_f4d d0 = {0.f, 0.f, 0.f, 0.f}, d1 = {0.f, 0.f, 0.f, 0.f};
       for (int m = 0; m < ng; m += 2) {
         const float *pa = st + CIC_STRIDE * (4 * m + kq), *pb = pa + 4 * CIC_STRIDE;
         const float a0 = pa[xy], z0 = pa[16 + zc];
-        const float a1 = pb[xy], z1 = pb[16 + zc];   // group m + 1 (zero rows past ng)
+        const float a1 = pb[xy], z1 = pb[16 + zc];   // group m + 1 (zero weights past ng)
         d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, grp == kq ? z0 : 0.f, d0, 0, 0, 0);
         d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, grp == kq ? z1 : 0.f, d1, 0, 0, 0);
       }
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-        sum[v] += (long long)rint((double)d0[v] * scale) + (long long)rint((double)d1[v] * scale);
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
-    // lane l holds W[4 (l >> 4) + v][4 grp + zc]: rows x = v, y = l >> 4; add the
-    // four groups (lanes l ^ 4, l ^ 8, l ^ 12 share y and z)
+    // lane l holds W[4 (l >> 4) + v][4 grp + zc]: rows x = v, y = l >> 4; the
+    // four groups (lanes l ^ 4, l ^ 8, l ^ 12 share y and z) added in FP32,
+    // then the window point in fixed point
+    long long sum[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      sum[v] += __shfl_xor(sum[v], 4, 64);
-      sum[v] += __shfl_xor(sum[v], 8, 64);
+      float s = d0[v] + d1[v];
+      s += __shfl_xor(s, 4, 64);
+      s += __shfl_xor(s, 8, 64);
+      sum[v] = (long long)rint((double)s * scale);
     }
     if (unitstart[un.g + 1] - unitstart[un.g] > 1) {
       // window point x + 4 y + 16 z of this unit, for k_cic_flush
@@ -502,10 +541,10 @@ __global__ void __launch_bounds__(64 * CIC_WAVES)
         for (int v = 0; v < 4; ++v) wins[(int64_t)u * 64 + v + 4 * kq + 16 * zc] = sum[v];
       }
     } else if (grp == 0 && anchored) {
-      const int64_t j = anc[1] - 1 + kq, k = anc[2] - 1 + zc;
+      const int64_t j = wy.a - 1 + kq, k = wz.a - 1 + zc;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int64_t i = anc[0] - 1 + v;
+        const int64_t i = wx.a - 1 + v;
         if (sum[v] != 0 && i < n && j < n && k < n)
           atomicAdd(acc + i + (int64_t)n * (j + (int64_t)n * k), (unsigned long long)sum[v]);
       }
@@ -517,28 +556,19 @@ __global__ void __launch_bounds__(64 * CIC_WAVES)
 // committed once.  One wave per occupied cell, a window point per lane.
 __global__ void __launch_bounds__(64)
     k_cic_flush(const int32_t *__restrict__ occupied, const int32_t *__restrict__ unitstart,
-                const int32_t *__restrict__ cnt, int nkf, float x0f, float hf,
-                const float *__restrict__ gt, int n, float x0t, float iwt,
-                const int32_t *__restrict__ cells, int nbin, const long long *__restrict__ wins,
-                unsigned long long *__restrict__ acc) {
+                const int32_t *__restrict__ cnt, int nkf, const vh_cicwin *__restrict__ wtab,
+                int n, const long long *__restrict__ wins, unsigned long long *__restrict__ acc) {
   const int l = threadIdx.x;
   const int nocc = cnt[CNT_NOCC];
   for (int g = blockIdx.x; g < nocc; g += gridDim.x) {
     const int u0 = unitstart[g], u1 = unitstart[g + 1];
     if (u1 - u0 < 2) continue;
     const int key = occupied[g];
-    const int kc[3] = {key % nkf + 1, (key / nkf) % nkf + 1, key / (nkf * nkf) + 1};
-    int anc[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      float w;
-      const float xl = x0f + ((float)(kc[d] - 1) - 0.5f) * hf;
-      anc[d] = locate_cell(gt, n, x0t, iwt, cells, nbin, xl, w);
-    }
-    if (!(anc[0] > 0 && anc[1] > 0 && anc[2] > 0)) continue;  // nothing was windowed
+    const int ax = wtab[key % nkf].a, ay = wtab[(key / nkf) % nkf].a, az = wtab[key / (nkf * nkf)].a;
+    if (!(ax > 0 && ay > 0 && az > 0)) continue;  // nothing was windowed
     long long sum = 0;
     for (int u = u0; u < u1; ++u) sum += wins[(int64_t)u * 64 + l];
-    const int64_t i = anc[0] - 1 + (l & 3), j = anc[1] - 1 + ((l >> 2) & 3), k = anc[2] - 1 + (l >> 4);
+    const int64_t i = ax - 1 + (l & 3), j = ay - 1 + ((l >> 2) & 3), k = az - 1 + (l >> 4);
     if (sum != 0 && i < n && j < n && k < n)
       atomicAdd(acc + i + (int64_t)n * (j + (int64_t)n * k), (unsigned long long)sum);
   }
@@ -562,22 +592,30 @@ int vh_deposit_cic_impl(vh_accel *a, vh_mesh *m, double charge) {
   vh_context *c = a->ctx;
   hipStream_t st = c->stream;
   const double scale = ldexp(1.0, a->cic_shift);
+  // the windows of the fine cells on this mesh: built once per mesh (in
+  // every replay of a graph captured before they were)
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  VH_CHECK_HIP(hipStreamIsCapturing(st, &cs));
+  if (a->cicwtab_serial != m->serial || cs != hipStreamCaptureStatusNone) {
+    hipLaunchKernelGGL(k_cic_window_table, dim3((a->nk + 63) / 64), dim3(64), 0, st, a->nk,
+                       (float)a->x0, (float)a->h, m->colloc, m->n, m->loc_x0, m->loc_iw,
+                       m->loc_cells, m->nbin, a->cicwtab);
+    VH_LAUNCH_CHECK();
+    if (cs == hipStreamCaptureStatusNone) a->cicwtab_serial = m->serial;
+  }
   // the fine deposit's tiles (2 KiB per unit, consumed by then) hold the
   // 512-byte windows of units of multi-unit cells
   long long *wins = a->cicwins;
-  const size_t lds = cic_lds_bytes(m->n, m->nbin);
-  VH_REQUIRE(lds <= 65536, "vh_deposit_cic: coarse locate table too large (%d bins)", m->nbin);
   const int grid = (int)((a->maxunits + CIC_WAVES - 1) / CIC_WAVES < 16384
                              ? (a->maxunits + CIC_WAVES - 1) / CIC_WAVES : 16384);
-  hipLaunchKernelGGL(k_cic_tiles, dim3(grid), dim3(64 * CIC_WAVES), lds, st, a->parts[a->cur], vh_rec_perm(a),
-                     a->units, a->unitstart, a->counters, a->nk, (float)a->x0, (float)a->h,
-                     m->colloc, m->n, m->loc_x0, m->loc_iw, m->loc_cells, m->nbin, scale,
-                     (unsigned long long *)m->acc64, wins);
+  hipLaunchKernelGGL(k_cic_tiles, dim3(grid), dim3(64 * CIC_WAVES), 0, st, a->parts[a->cur],
+                     vh_rec_perm(a), a->units, a->unitstart, a->counters, a->nk, (float)a->x0,
+                     (float)a->h, a->cicwtab, m->colloc, m->n, m->loc_x0, m->loc_iw, m->loc_cells,
+                     m->nbin, scale, (unsigned long long *)m->acc64, wins);
   VH_LAUNCH_CHECK();
   const int fgrid = (int)(a->maxocc < 65536 ? a->maxocc : 65536);
   hipLaunchKernelGGL(k_cic_flush, dim3(fgrid), dim3(64), 0, st, a->occupied, a->unitstart,
-                     a->counters, a->nk, (float)a->x0, (float)a->h, m->colloc, m->n, m->loc_x0,
-                     m->loc_iw, m->loc_cells, m->nbin, wins, (unsigned long long *)m->acc64);
+                     a->counters, a->nk, a->cicwtab, m->n, wins, (unsigned long long *)m->acc64);
   VH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_cic_convert, dim3(2048), dim3(256), 0, st, (unsigned long long *)m->acc64,
                      m->rho, m->duallen, m->n, 1.0 / scale, (float)charge);

@@ -108,7 +108,21 @@ struct vh_mesh {
   double *partials;  // 10 x (n*n) moments / charge partials
   double *scal;      // [0..9] moments, [10..19] multipole, [20] total charge
   int64_t *acc64;    // n^3 fixed-point accumulator of the CIC deposit
+  uint64_t serial;   // distinct for every mesh the process creates
 };
+
+// Window of a fine cell on the coarse collocation points along one axis (the
+// coarse deposit): the 1-based coarse interval a (left point) holding the
+// cell's lower corner, 0 when it is off the coarse grid; the four
+// collocation points a - 1 .. a + 2 (0-based; past the grid, the last point
+// again); the reciprocal lengths of the three intervals between them (0 for
+// an empty interval).  One entry per fine cell index of an axis.
+struct alignas(16) vh_cicwin {
+  int32_t a;
+  float g[4];
+  float r[3];
+};
+static_assert(sizeof(vh_cicwin) == 32, "vh_cicwin: two 16-byte words");
 
 // Particle buffer: two arrays of 24-byte position records, npad each: the
 // current positions (kx, ky, kz, dx, dy, dz; int32 and float words) of slot s
@@ -285,6 +299,8 @@ struct vh_accel {
                                // cloud (first -1: no particle; reset after each sort)
   float *tiles;                // 512 per work unit (fine deposit)
   long long *cicwins;          // 64 per work unit: the CIC windows of multi-unit cells
+  vh_cicwin *cicwtab;          // nk windows of the fine cells on a coarse mesh's points
+  uint64_t cicwtab_serial;     // serial of the mesh cicwtab was built for (0: none)
   double *dpart;               // per-block FP64 partials of the reductions
   int64_t dpart_len;
   double *dscal;               // [0..4] verlet diag, [8..11] field reduction, [16] q, [17] enclosed
