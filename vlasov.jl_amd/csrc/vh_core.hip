@@ -106,7 +106,52 @@ extern "C" int vh_context_create(int device, vh_context **out) {
   hipDeviceProp_t prop;
   VH_CHECK_HIP(hipGetDeviceProperties(&prop, device));
   c->num_cus = prop.multiProcessorCount;
+  if (vh_alloc((void **)&c->flags, sizeof(uint32_t) * VH_FLAG_WORDS) != VH_OK ||
+      vh_fill_i32(c, (int32_t *)c->flags, VH_FLAG_WORDS, 0) != VH_OK ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    vh_free(c->flags);
+    hipEventDestroy(c->fork_ev);
+    hipEventDestroy(c->join_ev);
+    hipEventDestroy(c->ev_a);
+    hipStreamDestroy(c->side);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return VH_EHIP;
+  }
+  c->flag_seq = 0;
   *out = c;
+  return VH_OK;
+}
+
+__global__ void k_flag_signal(uint32_t *__restrict__ f, uint32_t v) {
+  __hip_atomic_store(f, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One lane polls the flag; gives up after two seconds of the 100 MHz
+// constant clock (or 2^24 polls) and counts the timeout.  The work queued
+// behind the wait starts when this kernel ends: a kernel's start acquires
+// what the signalling stream's kernels released at their ends.
+__global__ void k_flag_wait(const uint32_t *__restrict__ f, uint32_t v, uint32_t *__restrict__ timeouts) {
+  const uint64_t t0 = wall_clock64();
+  for (uint32_t i = 0; i < (1u << 24); ++i) {
+    const uint32_t cur = __hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    if ((int32_t)(cur - v) >= 0) return;
+    if (wall_clock64() - t0 > 200000000ull) break;
+    __builtin_amdgcn_s_sleep(8);
+  }
+  atomicAdd(timeouts, 1u);
+}
+
+int vh_stream_signal(vh_context *c, hipStream_t st, int f, uint32_t v) {
+  hipLaunchKernelGGL(k_flag_signal, dim3(1), dim3(1), 0, st, c->flags + 16 * f, v);
+  VH_LAUNCH_CHECK();
+  return VH_OK;
+}
+
+int vh_stream_wait(vh_context *c, hipStream_t st, int f, uint32_t v) {
+  hipLaunchKernelGGL(k_flag_wait, dim3(1), dim3(1), 0, st, c->flags + 16 * f, v,
+                     c->flags + 16 * VH_FLAG_TIMEOUTS);
+  VH_LAUNCH_CHECK();
   return VH_OK;
 }
 
@@ -115,6 +160,7 @@ extern "C" int vh_context_destroy(vh_context *ctx) {
   VH_CHECK_HIP(hipSetDevice(ctx->device));
   VH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   VH_CHECK_HIP(hipStreamSynchronize(ctx->side));
+  vh_free(ctx->flags);
   VH_CHECK_HIP(hipEventDestroy(ctx->fork_ev));
   VH_CHECK_HIP(hipEventDestroy(ctx->join_ev));
   VH_CHECK_HIP(hipEventDestroy(ctx->ev_a));
@@ -127,6 +173,11 @@ extern "C" int vh_context_destroy(vh_context *ctx) {
 extern "C" int vh_sync(vh_context *ctx) {
   VH_REQUIRE(ctx != nullptr, "vh_sync: null context");
   VH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  VH_CHECK_HIP(hipStreamSynchronize(ctx->side));
+  uint32_t timeouts = 0;
+  VH_CHECK_HIP(hipMemcpy(&timeouts, ctx->flags + 16 * VH_FLAG_TIMEOUTS, sizeof timeouts,
+                         hipMemcpyDeviceToHost));
+  VH_REQUIRE(timeouts == 0, "vh_sync: %u wait(s) between the step's streams timed out", timeouts);
   return VH_OK;
 }
 
@@ -1488,12 +1539,24 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
       // the boundary and summed on the main stream, 1.481; after the fine
       // level on the main stream, 1.51; this one 1.475.  A launch queued
       // behind a wait for the other, busy, stream started tens of us late.)
-      VH_CHECK_HIP(hipEventRecord(c->fork_ev, c->stream));
+      // Outside a graph capture the order between the streams is kept by
+      // device flags (vh_stream_signal / vh_stream_wait), every wait queued
+      // after its signal; in a capture by events.
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      VH_CHECK_HIP(hipStreamIsCapturing(c->stream, &cs));
+      const bool flags = cs == hipStreamCaptureStatusNone;
+      const uint32_t seq = flags ? ++c->flag_seq : 0;
+      if (flags) {
+        VH_TRY(vh_stream_signal(c, c->stream, VH_FLAG_FORK, seq));
+        VH_TRY(vh_stream_wait(c, c->side, VH_FLAG_FORK, seq));
+      } else {
+        VH_CHECK_HIP(hipEventRecord(c->fork_ev, c->stream));
+        VH_CHECK_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
+      }
       VH_TRY(vh_deposit_fine_impl(a, fine, weight));
       if (a->denoiser) VH_TRY(vh_denoiser_apply_impl(a->denoiser, fine, a->denoise_tol));
       VH_TRY(vh_mean_field_coefficients_impl(fine, jel_nions, jel_radius));
       VH_TRY(vh_poisson_fused_prepare_impl(fine));
-      VH_CHECK_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
       {
         vh_on_side on(c);
         VH_TRY(vh_deposit_cic_impl(a, coarse, weight));
@@ -1501,13 +1564,17 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
         VH_TRY(vh_poisson_fused_prepare_impl(coarse));
         VH_TRY(vh_poisson_fused_finish_impl(coarse, false));
         VH_TRY(vh_poisson_fine_boundary_impl(fine, coarse));
-        VH_CHECK_HIP(hipEventRecord(c->join_ev, c->side));
+        if (flags) VH_TRY(vh_stream_signal(c, c->side, VH_FLAG_JOIN, seq));
+        else VH_CHECK_HIP(hipEventRecord(c->join_ev, c->side));
         VH_TRY(vh_effective_potential_impl(coarse, jel_nions, jel_radius));
-        VH_CHECK_HIP(hipEventRecord(c->ev_a, c->side));
+        if (flags) VH_TRY(vh_stream_signal(c, c->side, VH_FLAG_SIDE, seq));
+        else VH_CHECK_HIP(hipEventRecord(c->ev_a, c->side));
       }
-      VH_CHECK_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
+      if (flags) VH_TRY(vh_stream_wait(c, c->stream, VH_FLAG_JOIN, seq));
+      else VH_CHECK_HIP(hipStreamWaitEvent(c->stream, c->join_ev, 0));
       VH_TRY(vh_poisson_fused_finish_impl(fine, true));
-      VH_CHECK_HIP(hipStreamWaitEvent(c->stream, c->ev_a, 0));
+      if (flags) VH_TRY(vh_stream_wait(c, c->stream, VH_FLAG_SIDE, seq));
+      else VH_CHECK_HIP(hipStreamWaitEvent(c->stream, c->ev_a, 0));
     } else {
     VH_CHECK_HIP(hipEventRecord(c->fork_ev, c->stream));
     VH_TRY(vh_deposit_fine_impl(a, fine, weight));

@@ -408,7 +408,9 @@ __device__ __forceinline__ cic_win cic_load(const vh_cicwin *__restrict__ tab, i
 }
 
 // One wave per work unit (at most VH_UNIT particles of one occupied fine
-// cell), CIC_WAVES waves per block.  The unit's particles fall into a 4^3
+// cell), one wave per block: beside the fine deposit's one-wave blocks,
+// larger blocks of this kernel waited for four free wave slots on one CU
+// and started late.  The unit's particles fall into a 4^3
 // window of coarse points anchored at the coarse interval of the cell's
 // lower corner (vh_cicwin, one table entry per axis); the window's sums are
 // a rank-P update
@@ -427,7 +429,7 @@ __device__ __forceinline__ cic_win cic_load(const vh_cicwin *__restrict__ tab, i
 // FP32 sums of a unit are formed in a fixed order: the result is
 // reproducible.
 #define CIC_STRIDE 21   // 16 products + 4 z weights, odd row stride
-#define CIC_WAVES 4     // waves per block, each on its own units
+#define CIC_WAVES 1     // waves per block, each on its own units
 
 __global__ void __launch_bounds__(64 * CIC_WAVES)
     k_cic_tiles(const int32_t *__restrict__ P, const uint32_t *__restrict__ perm,

@@ -61,6 +61,14 @@ struct vh_context {
   // chain (the coarse mean field beside the fine solve), joined by events
   hipStream_t side;
   hipEvent_t fork_ev, join_ev, ev_a;
+  // Order between the two streams inside vh_step: a small kernel on one
+  // stream stores the step's sequence number into a flag word, a one-wave
+  // kernel on the other spins (bounded) until it reads it.  A stream that
+  // waits at an event while the other runs a chain of short kernels is
+  // resumed by the command processor tens to hundreds of microseconds late;
+  // a kernel already spinning on the device ends within a few.
+  uint32_t *flags;      // VH_FLAG_WORDS words: flag f at f * 16 (own 64-byte line)
+  uint32_t flag_seq;    // sequence number of the last step's signals
   int num_cus;
   // timing of regions with events on `stream`
   int prof_on;                          // mask of the stages timed
@@ -456,6 +464,13 @@ int vh_poisson_coarse_level_impl(vh_mesh *coarse);   // solve and coefficients
 int vh_poisson_fine_boundary_impl(vh_mesh *fine, vh_mesh *coarse);
 int vh_poisson_fine_part_impl(vh_mesh *fine);
 // launches of the scope go to the context's side stream
+enum { VH_FLAG_FORK = 0, VH_FLAG_JOIN, VH_FLAG_SIDE, VH_FLAG_TIMEOUTS, VH_FLAG_COUNT };
+#define VH_FLAG_WORDS (16 * VH_FLAG_COUNT)
+// flag f := v once the work queued on st before it is complete
+int vh_stream_signal(vh_context *c, hipStream_t st, int f, uint32_t v);
+// the work queued on st after this waits until flag f has reached v
+int vh_stream_wait(vh_context *c, hipStream_t st, int f, uint32_t v);
+
 struct vh_on_side {
   vh_context *c;
   hipStream_t saved;
