@@ -967,14 +967,14 @@ extern "C" int vh_columns(vh_accel *a, int32_t *cols, int32_t *nout) {
   VH_CHECK_HIP(hipSetDevice(a->ctx->device));
   VH_TRY(vh_rec_settle(a));
   int32_t *dcols = (int32_t *)a->parts[1 - a->cur];
-  VH_TRY(vh_fill_i32(a->ctx, a->counters + 32, 1, 0));
+  VH_TRY(vh_fill_i32(a->ctx, a->counters + CNT_COLUMNS_OUT, 1, 0));
   hipLaunchKernelGGL(k_columns, dim3(2048), dim3(256), 0, a->ctx->stream, a->parts[a->cur],
                      a->npad, a->npart, (float)a->x0, (float)a->h, a->lo, a->hi,
-                     a->spacing, (float)a->nbdt, a->ncol, dcols, a->counters + 32);
+                     a->spacing, (float)a->nbdt, a->ncol, dcols, a->counters + CNT_COLUMNS_OUT);
   VH_LAUNCH_CHECK();
   VH_CHECK_HIP(hipMemcpyAsync(cols, dcols, (size_t)a->npart * 12, hipMemcpyDeviceToHost, a->ctx->stream));
   int32_t hn = 0;
-  VH_CHECK_HIP(hipMemcpyAsync(&hn, a->counters + 32, 4, hipMemcpyDeviceToHost, a->ctx->stream));
+  VH_CHECK_HIP(hipMemcpyAsync(&hn, a->counters + CNT_COLUMNS_OUT, 4, hipMemcpyDeviceToHost, a->ctx->stream));
   VH_CHECK_HIP(hipStreamSynchronize(a->ctx->stream));
   if (nout) *nout = hn;
   return VH_OK;
@@ -1546,14 +1546,21 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
       VH_CHECK_HIP(hipStreamIsCapturing(c->stream, &cs));
       const bool flags = cs == hipStreamCaptureStatusNone;
       const uint32_t seq = flags ? ++c->flag_seq : 0;
+      // The fork's flag is stored as the fine deposit's first kernel starts
+      // and the join's as the coarse mean field's does (the work before
+      // them on their streams is then complete).  The side stream's wait is
+      // queued first, so that it is spinning on the device before the sort
+      // ends (a launch on an idle stream behind a busy one is taken up late);
+      // should the deposit's launch fail on the host, that wait ends at its
+      // time limit and counts a timeout.
       if (flags) {
-        VH_TRY(vh_stream_signal(c, c->stream, VH_FLAG_FORK, seq));
         VH_TRY(vh_stream_wait(c, c->side, VH_FLAG_FORK, seq));
       } else {
         VH_CHECK_HIP(hipEventRecord(c->fork_ev, c->stream));
         VH_CHECK_HIP(hipStreamWaitEvent(c->side, c->fork_ev, 0));
       }
-      VH_TRY(vh_deposit_fine_impl(a, fine, weight));
+      VH_TRY(vh_deposit_fine_impl(a, fine, weight,
+                                  flags ? vh_flag_sig(c, VH_FLAG_FORK, seq) : vh_sig{nullptr, 0}));
       if (a->denoiser) VH_TRY(vh_denoiser_apply_impl(a->denoiser, fine, a->denoise_tol));
       VH_TRY(vh_mean_field_coefficients_impl(fine, jel_nions, jel_radius));
       VH_TRY(vh_poisson_fused_prepare_impl(fine));
@@ -1564,9 +1571,9 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
         VH_TRY(vh_poisson_fused_prepare_impl(coarse));
         VH_TRY(vh_poisson_fused_finish_impl(coarse, false));
         VH_TRY(vh_poisson_fine_boundary_impl(fine, coarse));
-        if (flags) VH_TRY(vh_stream_signal(c, c->side, VH_FLAG_JOIN, seq));
-        else VH_CHECK_HIP(hipEventRecord(c->join_ev, c->side));
-        VH_TRY(vh_effective_potential_impl(coarse, jel_nions, jel_radius));
+        if (!flags) VH_CHECK_HIP(hipEventRecord(c->join_ev, c->side));
+        VH_TRY(vh_effective_potential_impl(coarse, jel_nions, jel_radius,
+                                           flags ? vh_flag_sig(c, VH_FLAG_JOIN, seq) : vh_sig{nullptr, 0}));
         if (flags) VH_TRY(vh_stream_signal(c, c->side, VH_FLAG_SIDE, seq));
         else VH_CHECK_HIP(hipEventRecord(c->ev_a, c->side));
       }
@@ -1612,14 +1619,14 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
     {
       vh_region r(a->ctx, VH_ST_FIELD);
       if (a->has_dproj) {
+        // the projectile stepped by the kernel that sums its forces
         const double proj[5] = {0.0, 0.0, 0.0, a->proj_charge, a->proj_scale};
         VH_TRY(vh_forces_impl(a, fine, coarse, weight, proj, a->proj_soft_mode,
-                              a->proj_ball_radius, a->dproj, fa));
+                              a->proj_ball_radius, a->dproj, fa, true, dt));
       } else {
         VH_TRY(vh_forces_impl(a, fine, coarse, weight, nullptr, VH_SOFT_DEVICE, 0.0, nullptr, fa));
       }
     }
-    if (a->has_dproj) VH_TRY(vh_projectile_step(a, dt));
     return VH_OK;
   }
   if (a->has_dproj) {

@@ -41,8 +41,9 @@ __global__ void __launch_bounds__(64)
     k_dep_tiles(const int32_t *__restrict__ P, const uint32_t *__restrict__ perm,
                 const float *__restrict__ nodes,
                 const vh_unit *__restrict__ units, const int32_t *__restrict__ cnt, int nk, int n,
-                float spacing, float nbdt, int ncol, float *__restrict__ tiles) {
+                float spacing, float nbdt, int ncol, float *__restrict__ tiles, vh_sig sig) {
   __shared__ float vals[DEP_ROW * DEP_STAGE];
+  vh_sig_store(sig);
   const int l = threadIdx.x;
   const int row = l & 15, kq = l >> 4;       // A: row kk, particle kq of the group
   const int ii = l & 7, jh = (l >> 3) & 1;   // B: column l & 15 -> (ii, jj = 2 t + jh)
@@ -290,7 +291,7 @@ __global__ void k_dep_scale(float *__restrict__ rho, int64_t len, const double *
 }
 
 
-int vh_deposit_fine_impl(vh_accel *a, vh_mesh *fine, double charge) {
+int vh_deposit_fine_impl(vh_accel *a, vh_mesh *fine, double charge, vh_sig sig) {
   vh_context *c = a->ctx;
   hipStream_t st = c->stream;
   // one wave per work unit up to 65536 units, then a stride over units
@@ -298,7 +299,7 @@ int vh_deposit_fine_impl(vh_accel *a, vh_mesh *fine, double charge) {
   int tok = vh_prof_begin(c, VH_ST_DEPOSIT_KERNEL);
   hipLaunchKernelGGL(k_dep_tiles, dim3(grid), dim3(64), 0, st, a->parts[a->cur], vh_rec_perm(a), a->nodes,
                      a->units, a->counters, a->nk, a->n, a->spacing, (float)a->nbdt, a->ncol,
-                     a->tiles);
+                     a->tiles, sig);
   vh_prof_end(c, tok);
   VH_LAUNCH_CHECK();
   const int mgrid = (int)(a->maxocc < 16384 ? a->maxocc : 16384);

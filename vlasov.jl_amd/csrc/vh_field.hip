@@ -932,18 +932,56 @@ __global__ void k_fallback(const int32_t *__restrict__ P, const uint32_t *__rest
   if (vh_lane() == 0 && rejected) atomicAdd(pu.nout, rejected);
 }
 
-// First pass of a column sum over many blocks: block (s, v) sums slice s
-// (of gridDim.x equal slices) of column v into out[v gridDim.x + s]; a
-// k_reduce_cols over those partials ends it (a fixed order either way).
-__global__ void k_reduce_slices(const double *__restrict__ part, int nparts,
-                                double *__restrict__ out) {
+// The field kernel's four columns of FP64 partials (projectile force and
+// potential) to dscal[8..11] in one launch: block (s, v) sums slice s of
+// column v; the last block to finish (a ticket at
+// *ticket, reset by it) sums the slices of each column in a fixed order and,
+// with dproj, steps the projectile (as k_projectile_step).  Grid (ns, 4).
+__global__ void k_reduce_field(const double *__restrict__ part, int nparts,
+                               double *__restrict__ slices, double *__restrict__ out,
+                               uint32_t *__restrict__ ticket, double *__restrict__ dproj, double dt) {
   __shared__ double lds[16];
+  __shared__ bool last;
   const int s = blockIdx.x, v = blockIdx.y, ns = gridDim.x;
   const int len = (nparts + ns - 1) / ns;
   const int lo = s * len < nparts ? s * len : nparts;
   const int hi = lo + len < nparts ? lo + len : nparts;
   const double r = vh_block_sum_d(vh_thread_sum(part + (int64_t)v * nparts + lo, hi - lo), lds);
-  if (threadIdx.x == 0) out[(int64_t)v * ns + s] = r;
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(slices + (int64_t)v * ns + s, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = t == (uint32_t)(ns * gridDim.y) - 1u;
+  }
+  __syncthreads();
+  if (!last) return;
+  if (threadIdx.x < (int)gridDim.y) {
+    const int c = threadIdx.x;
+    double sum = 0.0;
+    for (int k = 0; k < ns; ++k)
+      sum += __hip_atomic_load(slices + (int64_t)c * ns + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    out[c] = sum;
+    lds[c] = sum;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    *ticket = 0u;
+    if (dproj) {
+      // src/projectile.jl:298-304, as k_projectile_step
+      double *st = dproj;
+      const double p[3] = {st[0], st[1], st[2]}, prev[3] = {st[3], st[4], st[5]};
+      const double q = st[10], R = st[12];
+      const double r2 = (p[0] * p[0] + p[1] * p[1]) + p[2] * p[2];
+      const double modf = r2 > R * R ? st[11] * q / pow(r2, 1.5) : q / 64.0;  // WIGNER_SEITZ_NA^3
+      const double cfac = dt * dt / st[9];
+      for (int d = 0; d < 3; ++d) {
+        const double f = modf * p[d] + lds[d];
+        const double nw = (2.0 * p[d] - prev[d]) + cfac * f;
+        st[6 + d] = (nw - prev[d]) / (2.0 * dt);
+        st[3 + d] = p[d];
+        st[d] = nw;
+      }
+    }
+  }
 }
 
 // Sums each of `nvals` columns of `nparts` partials in a fixed order; one
@@ -1305,7 +1343,7 @@ int vh_energy_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, d
 
 int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, const double *proj,
                    int soft_mode, double ball_radius, const double *dproj, double push_a,
-                   bool fallback) {
+                   bool fallback, double proj_dt) {
   vh_context *c = a->ctx;
   hipStream_t st = c->stream;
   // the exact-order kernel reads the records in slot order
@@ -1369,17 +1407,6 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
                        (float)weight, a->ncol, pa, pu, a->force, a->dpart);
   vh_prof_end(c, tok);
   VH_LAUNCH_CHECK();
-  {
-    // 65536 partials of 4 columns: slices over 32 blocks, then their sums
-    // (4 blocks alone took 13 us); the slices' sums go after the partials
-    constexpr int nsl = FIELD_SLICES;
-    double *sl = a->dpart + 4 * (int64_t)grid;
-    VH_REQUIRE(4 * (int64_t)grid + 4 * nsl <= 8 * a->dpart_len, "forces: partials beyond the buffer");
-    hipLaunchKernelGGL(k_reduce_slices, dim3(nsl, 4), dim3(1024), 0, st, a->dpart, grid, sl);
-    VH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_cols, dim3(4), dim3(64), 0, st, sl, nsl, 4, a->dscal + 8);
-    VH_LAUNCH_CHECK();
-  }
   vh_fallback_args fa;
   fa.px = proj ? proj[0] : 0.0;
   fa.py = proj ? proj[1] : 0.0;
@@ -1398,6 +1425,20 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
     VH_LAUNCH_CHECK();
   } else {
     VH_REQUIRE(!pu.on, "forces: the push needs the fallback");
+  }
+  {
+    // 65536 partials of 4 columns: slices over 32 blocks, then their sums by
+    // the last block (4 blocks alone took 13 us); the slices' sums go after
+    // the partials.  (After the fallback, which reads the projectile's
+    // position before the step.)
+    constexpr int nsl = FIELD_SLICES;
+    double *sl = a->dpart + 4 * (int64_t)grid;
+    VH_REQUIRE(4 * (int64_t)grid + 4 * nsl <= 8 * a->dpart_len, "forces: partials beyond the buffer");
+    const bool step_proj = proj_dt > 0 && dproj != nullptr;
+    hipLaunchKernelGGL(k_reduce_field, dim3(nsl, 4), dim3(1024), 0, st, a->dpart, grid, sl,
+                       a->dscal + 8, (uint32_t *)(a->counters + CNT_REDUCE_TICKET),
+                       step_proj ? (double *)dproj : (double *)nullptr, proj_dt);
+    VH_LAUNCH_CHECK();
   }
   if (pu.on) {
     a->cur = 1 - a->cur;  // the pushed records, in slot order

@@ -54,6 +54,18 @@ struct vh_prof_rec {
   hipEvent_t start, stop;
 };
 
+// A flag store folded into the start of a kernel (thread 0 of block 0, no
+// flag when f is null): the kernel has started, so the work queued on its
+// stream before it is complete.
+struct vh_sig {
+  uint32_t *f;
+  uint32_t v;
+};
+__device__ __forceinline__ void vh_sig_store(vh_sig s) {
+  if (s.f && blockIdx.x == 0 && threadIdx.x == 0)
+    __hip_atomic_store(s.f, s.v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 struct vh_context {
   int device;
   hipStream_t stream;
@@ -239,7 +251,9 @@ enum {
   // into parts[b] (slot CNT_PUSH_NOUT + b), counted by the push for the
   // deposit of the next step
   CNT_PUSH_NOUT = CNT_COUNT,
-  CNT_READ = CNT_PUSH_NOUT + 2  // the counters a host read copies
+  CNT_READ = CNT_PUSH_NOUT + 2,  // the counters a host read copies
+  CNT_COLUMNS_OUT = 32,          // vh_columns' rejection count
+  CNT_REDUCE_TICKET = 40         // k_reduce_field's ticket (reset by its last block)
 };
 
 // Work units of the per-cell kernels: an occupied cell's particles in runs of
@@ -453,7 +467,7 @@ int vh_rec_settle(vh_accel *a);
 static inline const uint32_t *vh_rec_perm(const vh_accel *a) {
   return a->rec_pending ? a->rec_perm : a->ident;
 }
-int vh_deposit_fine_impl(vh_accel *a, vh_mesh *fine, double charge);
+int vh_deposit_fine_impl(vh_accel *a, vh_mesh *fine, double charge, vh_sig sig = vh_sig{nullptr, 0});
 int vh_deposit_cic_impl(vh_accel *a, vh_mesh *coarse, double charge);
 int vh_poisson_single_impl(vh_mesh *m);
 int vh_poisson_nested_impl(vh_mesh *fine, vh_mesh *coarse);
@@ -465,6 +479,7 @@ int vh_poisson_fine_boundary_impl(vh_mesh *fine, vh_mesh *coarse);
 int vh_poisson_fine_part_impl(vh_mesh *fine);
 // launches of the scope go to the context's side stream
 enum { VH_FLAG_FORK = 0, VH_FLAG_JOIN, VH_FLAG_SIDE, VH_FLAG_TIMEOUTS, VH_FLAG_COUNT };
+static inline vh_sig vh_flag_sig(vh_context *c, int f, uint32_t v) { return vh_sig{c->flags + 16 * f, v}; }
 #define VH_FLAG_WORDS (16 * VH_FLAG_COUNT)
 // flag f := v once the work queued on st before it is complete
 int vh_stream_signal(vh_context *c, hipStream_t st, int f, uint32_t v);
@@ -478,7 +493,8 @@ struct vh_on_side {
   ~vh_on_side() { c->stream = saved; }
 };
 int vh_spline_coefficients_impl(vh_mesh *m, const float *src, float *dst);
-int vh_effective_potential_impl(vh_mesh *m, double nions, double radius);
+int vh_effective_potential_impl(vh_mesh *m, double nions, double radius,
+                                vh_sig sig = vh_sig{nullptr, 0});
 int vh_potential_coefficients_impl(vh_mesh *m, double nions, double radius);
 // its two halves: csol = S^-1 V(rho) (needs the density only), csol += S^-1 phi
 int vh_mean_field_coefficients_impl(vh_mesh *m, double nions, double radius);
@@ -494,9 +510,11 @@ vh_context *vh_denoiser_context(const vh_denoiser *d);
 // matrix-core field kernel and vh_verlet_tiles
 // fallback = false leaves the out-of-stencil slots of the force buffer to the
 // caller (vh_forces_host resolves them in FP64 from the host positions)
+// proj_dt > 0 (with dproj): the projectile is stepped on the device as well
+// (vh_projectile_step_impl), by the kernel that sums its forces
 int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, const double *proj,
                    int soft_mode, double ball_radius, const double *dproj, double push_a = 0.0,
-                   bool fallback = true);
+                   bool fallback = true, double proj_dt = 0.0);
 int vh_projectile_step_impl(vh_accel *a, double dt);
 int vh_verlet_impl(vh_accel *a, double dt, double mass, double rcmax);
 // the push leaves the next sort's keys and first-digit histograms

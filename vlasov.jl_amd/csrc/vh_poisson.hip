@@ -1227,7 +1227,8 @@ int vh_poisson_fused_finish_impl(vh_mesh *m, bool accumulate) {
 // ---------------------------------------------------------------------------
 __global__ void k_effective(float *__restrict__ extra, const float *__restrict__ rho,
                             const float *__restrict__ g, int n, float nions, float radius,
-                            const float *__restrict__ add) {
+                            const float *__restrict__ add, vh_sig sig) {
+  vh_sig_store(sig);
   const float xc_ex = (float)(-0.98474502184269641);          // -cbrt(3/pi)
   const float xc_co = (float)(-0.0333);
   const float xc_sc = (float)(11.4 * 1.6119919540164696);     // 11.4 cbrt(4 pi / 3)
@@ -1244,10 +1245,10 @@ __global__ void k_effective(float *__restrict__ extra, const float *__restrict__
   }
 }
 
-int vh_effective_potential_impl(vh_mesh *m, double nions, double radius) {
+int vh_effective_potential_impl(vh_mesh *m, double nions, double radius, vh_sig sig) {
   vh_context *c = m->ctx;
   hipLaunchKernelGGL(k_effective, dim3(2048), dim3(256), 0, c->stream, m->extra, m->rho, m->colloc,
-                     m->n, (float)nions, (float)radius, (const float *)nullptr);
+                     m->n, (float)nions, (float)radius, (const float *)nullptr, sig);
   VH_LAUNCH_CHECK();
   // extra -> work -> extra -> csol (+=)
   VH_TRY(apply_all(c, mop(m, m->Sinv_op, m->Sinv_b3), m->n, m->extra, m->work, m->extra, m->csol, EPI_ACCUMULATE,
@@ -1262,7 +1263,7 @@ int vh_effective_potential_impl(vh_mesh *m, double nions, double radius) {
 int vh_mean_field_coefficients_impl(vh_mesh *m, double nions, double radius) {
   vh_context *c = m->ctx;
   hipLaunchKernelGGL(k_effective, dim3(2048), dim3(256), 0, c->stream, m->extra, m->rho, m->colloc,
-                     m->n, (float)nions, (float)radius, (const float *)nullptr);
+                     m->n, (float)nions, (float)radius, (const float *)nullptr, vh_sig{nullptr, 0});
   VH_LAUNCH_CHECK();
   // extra -> work -> extra -> csol
   VH_TRY(apply_all(c, mop(m, m->Sinv_op, m->Sinv_b3), m->n, m->extra, m->work, m->extra, m->csol, EPI_STORE, nullptr, 0));
