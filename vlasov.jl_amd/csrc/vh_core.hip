@@ -1492,6 +1492,9 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
                       double mass, double rcmax, double jel_nions, double jel_radius) {
   // _update_forces_resident! (src/simulation.jl:472-531), then step!
   vh_context *c = a->ctx;
+  // the push's Verlet factor, and whether the fallback ran beside the fine level
+  const double push_factor = (double)(float)(dt * dt / mass);
+  bool early_fallback = false;
   const int staged = (1 << VH_ST_DEPOSIT_FINE) | (1 << VH_ST_DEPOSIT_CIC) | (1 << VH_ST_POISSON) |
                      (1 << VH_ST_SPLINE) | (1 << VH_ST_MEANFIELD);
   if (c->prof_on & staged) {
@@ -1529,6 +1532,7 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
     // step solves as the entry points do.
     const bool fused = !a->poisson_unfused && !fine->gemm_bf16x3 && !coarse->gemm_bf16x3 &&
                        fine->n <= VH_FUSED_MAXN && coarse->n <= VH_FUSED_MAXN;
+    early_fallback = fused && step_fuses_push(a) && a->sorted_once;
     if (fused) {
       // On the side stream the CIC, the coarse level, the fine boundary and
       // then the coarse mean field; on the main stream the fine deposit, the
@@ -1574,6 +1578,15 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
         if (!flags) VH_CHECK_HIP(hipEventRecord(c->join_ev, c->side));
         VH_TRY(vh_effective_potential_impl(coarse, jel_nions, jel_radius,
                                            flags ? vh_flag_sig(c, VH_FLAG_JOIN, seq) : vh_sig{nullptr, 0}));
+        // the out-of-stencil fallback (with the push) needs only the coarse
+        // coefficients and the sort: here, beside the fine level, rather
+        // than after the field kernel
+        if (early_fallback) {
+          const double proj[5] = {0.0, 0.0, 0.0, a->proj_charge, a->proj_scale};
+          VH_TRY(vh_fallback_impl(a, coarse, weight, a->has_dproj ? proj : nullptr,
+                                  a->has_dproj ? a->proj_ball_radius : 0.0,
+                                  a->has_dproj ? a->dproj : nullptr, push_factor));
+        }
         if (flags) VH_TRY(vh_stream_signal(c, c->side, VH_FLAG_SIDE, seq));
         else VH_CHECK_HIP(hipEventRecord(c->ev_a, c->side));
       }
@@ -1615,16 +1628,17 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
       vh_set_error("vh_step: the cloud has not been sorted");
       return VH_ESTATE;
     }
-    const double fa = (double)(float)(dt * dt / mass);
+    const int fb = early_fallback ? VH_FALLBACK_DONE : VH_FALLBACK_AFTER;
     {
       vh_region r(a->ctx, VH_ST_FIELD);
       if (a->has_dproj) {
         // the projectile stepped by the kernel that sums its forces
         const double proj[5] = {0.0, 0.0, 0.0, a->proj_charge, a->proj_scale};
         VH_TRY(vh_forces_impl(a, fine, coarse, weight, proj, a->proj_soft_mode,
-                              a->proj_ball_radius, a->dproj, fa, true, dt));
+                              a->proj_ball_radius, a->dproj, push_factor, fb, dt));
       } else {
-        VH_TRY(vh_forces_impl(a, fine, coarse, weight, nullptr, VH_SOFT_DEVICE, 0.0, nullptr, fa));
+        VH_TRY(vh_forces_impl(a, fine, coarse, weight, nullptr, VH_SOFT_DEVICE, 0.0, nullptr,
+                              push_factor, fb));
       }
     }
     return VH_OK;

@@ -761,11 +761,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
       // field minus projectile term, coalesced stores of the live lanes;
       // with the push, the records and keys of a cell that fits (the
       // particles of the others are pushed by the fallback)
+      // (a cell that does not fit stores nothing: its forces are the
+      // fallback's, which may run before this kernel)
       const float f0 = ok ? fl[0] - fx2 : nan, f1 = ok ? fl[1] - fy2 : nan;
       const float f2 = ok ? fl[2] - fz2 : nan;
-      vh_bstore_f(vh_rsrc(force + base, 4 * nlive), l, f0);
-      vh_bstore_f(vh_rsrc(force + npad + base, 4 * nlive), l, f1);
-      vh_bstore_f(vh_rsrc(force + 2 * npad + base, 4 * nlive), l, f2);
+      const int nf = ok ? nlive : 0;
+      vh_bstore_f(vh_rsrc(force + base, 4 * nf), l, f0);
+      vh_bstore_f(vh_rsrc(force + npad + base, 4 * nf), l, f1);
+      vh_bstore_f(vh_rsrc(force + 2 * npad + base, 4 * nf), l, f2);
       {
         const int np = pu.on && ok ? nlive : 0;
         vh_pos nw;
@@ -1341,13 +1344,7 @@ int vh_energy_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, d
   return VH_OK;
 }
 
-int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, const double *proj,
-                   int soft_mode, double ball_radius, const double *dproj, double push_a,
-                   bool fallback, double proj_dt) {
-  vh_context *c = a->ctx;
-  hipStream_t st = c->stream;
-  // the exact-order kernel reads the records in slot order
-  if (a->exact_field) VH_TRY(vh_rec_settle(a));
+static vh_push_args push_args(vh_accel *a, double push_a) {
   vh_push_args pu;
   pu.on = 0;
   pu.a = pu.h = 0.f;
@@ -1358,8 +1355,7 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
   pu.lo = a->lo;
   pu.hi = a->hi;
   pu.nout = nullptr;
-  if (push_a > 0.0) {
-    VH_REQUIRE(!a->exact_field, "forces with the push: the matrix-core field kernel only");
+  if (push_a > 0.0) {   // (the matrix-core field kernel only: checked by the callers)
     pu.on = 1;
     pu.a = (float)push_a;
     pu.h = (float)a->h;
@@ -1369,6 +1365,42 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
     pu.dst = a->parts[1 - a->cur];
     pu.nout = a->counters + CNT_PUSH_NOUT + (1 - a->cur);
   }
+  return pu;
+}
+
+// The out-of-stencil fallback of the particles of cells whose stencil does
+// not fit (FP64; with push_a > 0 it pushes them), on the context's current
+// stream.  It reads the projectile's position before the step.
+int vh_fallback_impl(vh_accel *a, vh_mesh *coarse, double weight, const double *proj,
+                     double ball_radius, const double *dproj, double push_a) {
+  VH_REQUIRE(push_a <= 0.0 || !a->exact_field, "fallback with the push: the matrix-core field kernel only");
+  vh_fallback_args fa;
+  fa.px = proj ? proj[0] : 0.0;
+  fa.py = proj ? proj[1] : 0.0;
+  fa.pz = proj ? proj[2] : 0.0;
+  fa.q = proj ? proj[3] : 0.0;
+  fa.ball = ball_radius > 0;
+  fa.radius = ball_radius > 0 ? ball_radius : (proj ? proj[4] : 1.0);
+  fa.has_proj = proj != nullptr;
+  fa.w = weight;
+  fa.escaped = 0.0;
+  fa.dev = dproj;
+  hipLaunchKernelGGL(k_fallback, dim3(1024), dim3(256), 0, a->ctx->stream, a->parts[a->cur],
+                     vh_rec_perm(a), a->npad, a->outlist, a->counters, a->x0, a->h, coarse->csol,
+                     coarse->knots64, coarse->nk, coarse->n, fa, push_args(a, push_a), a->force);
+  VH_LAUNCH_CHECK();
+  return VH_OK;
+}
+
+int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, const double *proj,
+                   int soft_mode, double ball_radius, const double *dproj, double push_a,
+                   int fallback, double proj_dt) {
+  vh_context *c = a->ctx;
+  hipStream_t st = c->stream;
+  // the exact-order kernel reads the records in slot order
+  if (a->exact_field) VH_TRY(vh_rec_settle(a));
+  VH_REQUIRE(push_a <= 0.0 || !a->exact_field, "forces with the push: the matrix-core field kernel only");
+  const vh_push_args pu = push_args(a, push_a);
   vh_proj_args pa;
   pa.dev = dproj;
   pa.px = pa.py = pa.pz = 0.f;
@@ -1407,24 +1439,10 @@ int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, c
                        (float)weight, a->ncol, pa, pu, a->force, a->dpart);
   vh_prof_end(c, tok);
   VH_LAUNCH_CHECK();
-  vh_fallback_args fa;
-  fa.px = proj ? proj[0] : 0.0;
-  fa.py = proj ? proj[1] : 0.0;
-  fa.pz = proj ? proj[2] : 0.0;
-  fa.q = proj ? proj[3] : 0.0;
-  fa.ball = ball_radius > 0;
-  fa.radius = ball_radius > 0 ? ball_radius : (proj ? proj[4] : 1.0);
-  fa.has_proj = proj != nullptr;
-  fa.w = weight;
-  fa.escaped = 0.0;
-  fa.dev = dproj;
-  if (fallback) {
-    hipLaunchKernelGGL(k_fallback, dim3(1024), dim3(256), 0, st, a->parts[a->cur], vh_rec_perm(a),
-                       a->npad, a->outlist, a->counters, a->x0, a->h, coarse->csol, coarse->knots64,
-                       coarse->nk, coarse->n, fa, pu, a->force);
-    VH_LAUNCH_CHECK();
+  if (fallback == VH_FALLBACK_AFTER) {
+    VH_TRY(vh_fallback_impl(a, coarse, weight, proj, ball_radius, dproj, push_a));
   } else {
-    VH_REQUIRE(!pu.on, "forces: the push needs the fallback");
+    VH_REQUIRE(!pu.on || fallback == VH_FALLBACK_DONE, "forces: the push needs the fallback");
   }
   {
     // 65536 partials of 4 columns: slices over 32 blocks, then their sums by
@@ -1769,7 +1787,7 @@ extern "C" int vh_forces_host(vh_accel *a, vh_mesh *fine, const double *csol_f,
   VH_TRY(vh_rec_settle(a));
   const int tok = vh_prof_begin(a->ctx, VH_ST_FIELD);
   const int rc = vh_forces_impl(a, fine, fine, weight, proj, VH_SOFT_DEVICE, ball_radius, nullptr,
-                                0.0, false);
+                                0.0, VH_FALLBACK_NONE);
   vh_prof_end(a->ctx, tok);
   if (rc != VH_OK) return rc;
   vh_fallback_args fa;

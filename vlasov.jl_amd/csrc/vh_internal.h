@@ -508,13 +508,20 @@ vh_context *vh_denoiser_context(const vh_denoiser *d);
 // push_a > 0: the field and fallback kernels also push their particles
 // (Verlet factor dt^2 / mass) and leave the next sort's keys; requires the
 // matrix-core field kernel and vh_verlet_tiles
-// fallback = false leaves the out-of-stencil slots of the force buffer to the
+// VH_FALLBACK_NONE leaves the out-of-stencil slots of the force buffer to the
 // caller (vh_forces_host resolves them in FP64 from the host positions)
+// fallback: VH_FALLBACK_AFTER launches vh_fallback_impl after the field
+// kernel; VH_FALLBACK_DONE: the caller has launched it (with the same push),
+// ordered before this call's reduction (which steps the projectile);
+// VH_FALLBACK_NONE leaves those slots alone.
 // proj_dt > 0 (with dproj): the projectile is stepped on the device as well
 // (vh_projectile_step_impl), by the kernel that sums its forces
+enum { VH_FALLBACK_NONE = 0, VH_FALLBACK_AFTER = 1, VH_FALLBACK_DONE = 2 };
 int vh_forces_impl(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight, const double *proj,
                    int soft_mode, double ball_radius, const double *dproj, double push_a = 0.0,
-                   bool fallback = true, double proj_dt = 0.0);
+                   int fallback = VH_FALLBACK_AFTER, double proj_dt = 0.0);
+int vh_fallback_impl(vh_accel *a, vh_mesh *coarse, double weight, const double *proj,
+                     double ball_radius, const double *dproj, double push_a);
 int vh_projectile_step_impl(vh_accel *a, double dt);
 int vh_verlet_impl(vh_accel *a, double dt, double mass, double rcmax);
 // the push leaves the next sort's keys and first-digit histograms
