@@ -123,18 +123,23 @@ extern "C" int vh_context_create(int device, vh_context **out) {
   return VH_OK;
 }
 
+// The flags carry no data: the work before a signal has completed and
+// released its writes at its kernels' ends, and the work after a wait
+// acquires them at its kernels' starts.  Relaxed device-scope accesses
+// therefore suffice, and they keep the L2 caches intact (a release or
+// acquire at that scope writes back or invalidates the L2 of the XCD the
+// wave runs on, each time: while the other stream works).
 __global__ void k_flag_signal(uint32_t *__restrict__ f, uint32_t v) {
-  __hip_atomic_store(f, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // One lane polls the flag; gives up after two seconds of the 100 MHz
 // constant clock (or 2^24 polls) and counts the timeout.  The work queued
-// behind the wait starts when this kernel ends: a kernel's start acquires
-// what the signalling stream's kernels released at their ends.
+// behind the wait starts when this kernel ends.
 __global__ void k_flag_wait(const uint32_t *__restrict__ f, uint32_t v, uint32_t *__restrict__ timeouts) {
   const uint64_t t0 = wall_clock64();
   for (uint32_t i = 0; i < (1u << 24); ++i) {
-    const uint32_t cur = __hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t cur = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if ((int32_t)(cur - v) >= 0) return;
     if (wall_clock64() - t0 > 200000000ull) break;
     __builtin_amdgcn_s_sleep(8);

@@ -56,14 +56,14 @@ struct vh_prof_rec {
 
 // A flag store folded into the start of a kernel (thread 0 of block 0, no
 // flag when f is null): the kernel has started, so the work queued on its
-// stream before it is complete.
+// stream before it is complete (relaxed, as k_flag_signal's).
 struct vh_sig {
   uint32_t *f;
   uint32_t v;
 };
 __device__ __forceinline__ void vh_sig_store(vh_sig s) {
   if (s.f && blockIdx.x == 0 && threadIdx.x == 0)
-    __hip_atomic_store(s.f, s.v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(s.f, s.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 struct vh_context {
