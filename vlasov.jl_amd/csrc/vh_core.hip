@@ -1575,7 +1575,8 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
       VH_TRY(vh_poisson_fused_prepare_impl(fine));
       {
         vh_on_side on(c);
-        VH_TRY(vh_deposit_cic_impl(a, coarse, weight));
+        // the coarse density formed in the multipole's moments pass
+        VH_TRY(vh_deposit_cic_impl(a, coarse, weight, true));
         VH_TRY(vh_poisson_multipole_boundary_impl(coarse));
         VH_TRY(vh_poisson_fused_prepare_impl(coarse));
         VH_TRY(vh_poisson_fused_finish_impl(coarse, false));

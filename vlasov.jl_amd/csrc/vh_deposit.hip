@@ -591,10 +591,16 @@ __global__ void k_cic_convert(unsigned long long *__restrict__ acc, float *__res
   }
 }
 
-int vh_deposit_cic_impl(vh_accel *a, vh_mesh *m, double charge) {
+int vh_deposit_cic_impl(vh_accel *a, vh_mesh *m, double charge, bool defer) {
   vh_context *c = a->ctx;
   hipStream_t st = c->stream;
   const double scale = ldexp(1.0, a->cic_shift);
+  if (m->acc_pending) {   // an earlier deferred deposit never formed: clear its sums
+    hipLaunchKernelGGL(k_cic_convert, dim3(2048), dim3(256), 0, st, (unsigned long long *)m->acc64,
+                       m->rho, m->duallen, m->n, m->acc_inv_scale, m->acc_charge);
+    VH_LAUNCH_CHECK();
+    m->acc_pending = 0;
+  }
   // the windows of the fine cells on this mesh: built once per mesh (in
   // every replay of a graph captured before they were)
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -620,6 +626,12 @@ int vh_deposit_cic_impl(vh_accel *a, vh_mesh *m, double charge) {
   hipLaunchKernelGGL(k_cic_flush, dim3(fgrid), dim3(64), 0, st, a->occupied, a->unitstart,
                      a->counters, a->nk, a->cicwtab, m->n, wins, (unsigned long long *)m->acc64);
   VH_LAUNCH_CHECK();
+  if (defer) {
+    m->acc_pending = 1;
+    m->acc_inv_scale = 1.0 / scale;
+    m->acc_charge = (float)charge;
+    return VH_OK;
+  }
   hipLaunchKernelGGL(k_cic_convert, dim3(2048), dim3(256), 0, st, (unsigned long long *)m->acc64,
                      m->rho, m->duallen, m->n, 1.0 / scale, (float)charge);
   VH_LAUNCH_CHECK();

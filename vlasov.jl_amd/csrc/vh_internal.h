@@ -128,6 +128,11 @@ struct vh_mesh {
   double *partials;  // 10 x (n*n) moments / charge partials
   double *scal;      // [0..9] moments, [10..19] multipole, [20] total charge
   int64_t *acc64;    // n^3 fixed-point accumulator of the CIC deposit
+  // a coarse deposit whose density is still in acc64 (vh_step): the
+  // multipole's moments pass forms rho and clears the accumulator
+  int acc_pending;
+  double acc_inv_scale;
+  float acc_charge;
   uint64_t serial;   // distinct for every mesh the process creates
 };
 
@@ -468,7 +473,8 @@ static inline const uint32_t *vh_rec_perm(const vh_accel *a) {
   return a->rec_pending ? a->rec_perm : a->ident;
 }
 int vh_deposit_fine_impl(vh_accel *a, vh_mesh *fine, double charge, vh_sig sig = vh_sig{nullptr, 0});
-int vh_deposit_cic_impl(vh_accel *a, vh_mesh *coarse, double charge);
+// defer: leave the sums in the accumulator for vh_multipole_impl (acc_pending)
+int vh_deposit_cic_impl(vh_accel *a, vh_mesh *coarse, double charge, bool defer = false);
 int vh_poisson_single_impl(vh_mesh *m);
 int vh_poisson_nested_impl(vh_mesh *fine, vh_mesh *coarse);
 // the nested solve in two parts: the coarse level, its coefficients and the

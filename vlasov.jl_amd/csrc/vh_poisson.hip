@@ -586,13 +586,26 @@ static int apply_all(vh_context *c, const vh_op &A, int n, const float *src, flo
 // (lane, then wave, then block, then the fixed-order reduction of the
 // block partials).
 #define MOM_MAXN 288
-template <int Q>
+// With the coarse deposit's accumulator (cic.acc non-null), the density is
+// formed here from the 64-bit fixed-point sums as k_cic_convert forms it,
+// stored, and the accumulator cleared, in the same pass as the moments.
+struct vh_cic_src {
+  unsigned long long *acc;
+  float *rho;
+  const float *wl;   // dual lengths
+  double inv_scale;
+  float charge;
+};
+template <int Q, bool CIC>
 __global__ void __launch_bounds__(256)
     k_moments(const float *__restrict__ c, const double *__restrict__ dual, int n,
-              double *__restrict__ part, int nparts) {
+              double *__restrict__ part, int nparts, vh_cic_src cic) {
   __shared__ double pd[3 * MOM_MAXN];
+  __shared__ float pw[MOM_MAXN];
   __shared__ double lds[10][4];
   for (int i = threadIdx.x; i < 3 * n; i += blockDim.x) pd[i] = dual[i];
+  if (CIC)
+    for (int i = threadIdx.x; i < n; i += blockDim.x) pw[i] = cic.wl[i];
   __syncthreads();
   const double *p0 = pd, *p1 = pd + n, *p2 = pd + 2 * n;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -602,11 +615,37 @@ __global__ void __launch_bounds__(256)
   const int njk = n * n;
   for (int t0 = 4 * (blockIdx.x * 4 + w); t0 < njk; t0 += 16 * gridDim.x) {
     float val[4][Q];
+    if (CIC) {
+      unsigned long long raw[4][Q];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const float *col = c + (int64_t)min(t0 + u, njk - 1) * n;
+      for (int u = 0; u < 4; ++u) {
+        const unsigned long long *col = cic.acc + (int64_t)min(t0 + u, njk - 1) * n;
 #pragma unroll
-      for (int q = 0; q < Q; ++q) val[u][q] = col[min(l + 64 * q, n - 1)];
+        for (int q = 0; q < Q; ++q) raw[u][q] = col[min(l + 64 * q, n - 1)];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int t = min(t0 + u, njk - 1), j = t % n, k = t / n;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          const int i = min(l + 64 * q, n - 1);
+          // k_cic_convert's expression, in its order
+          const float r = (float)((double)(long long)raw[u][q] * cic.inv_scale);
+          val[u][q] = r * (cic.charge / (pw[i] * pw[j] * pw[k]));
+          if (t0 + u < njk && l + 64 * q < n) {
+            const int64_t idx = (int64_t)t * n + i;
+            cic.rho[idx] = val[u][q];
+            cic.acc[idx] = 0ull;
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float *col = c + (int64_t)min(t0 + u, njk - 1) * n;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) val[u][q] = col[min(l + 64 * q, n - 1)];
+      }
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -722,15 +761,25 @@ static int moments_blocks(int n) {
 // the ten moments of rho into mom[0..9] and the multipole into mp[0..9],
 // through 10 x moments_blocks(n) FP64 partials
 static int multipole_launch(hipStream_t st, const float *rho, const double *dual, int n,
-                            double *partials, double *mom, double *mp) {
+                            double *partials, double *mom, double *mp,
+                            vh_cic_src cic = vh_cic_src{nullptr, nullptr, nullptr, 0.0, 0.f}) {
   VH_REQUIRE(n >= 2 && n <= MOM_MAXN, "multipole: n = %d outside [2, %d]", n, MOM_MAXN);
   const int nb = moments_blocks(n);
-  if (n <= 128)
-    hipLaunchKernelGGL(k_moments<2>, dim3(nb), dim3(256), 0, st, rho, dual, n, partials, nb);
-  else if (n <= 192)
-    hipLaunchKernelGGL(k_moments<3>, dim3(nb), dim3(256), 0, st, rho, dual, n, partials, nb);
-  else
-    hipLaunchKernelGGL(k_moments<5>, dim3(nb), dim3(256), 0, st, rho, dual, n, partials, nb);
+  if (cic.acc) {
+    if (n <= 128)
+      hipLaunchKernelGGL((k_moments<2, true>), dim3(nb), dim3(256), 0, st, rho, dual, n, partials, nb, cic);
+    else if (n <= 192)
+      hipLaunchKernelGGL((k_moments<3, true>), dim3(nb), dim3(256), 0, st, rho, dual, n, partials, nb, cic);
+    else
+      hipLaunchKernelGGL((k_moments<5, true>), dim3(nb), dim3(256), 0, st, rho, dual, n, partials, nb, cic);
+  } else {
+    if (n <= 128)
+      hipLaunchKernelGGL((k_moments<2, false>), dim3(nb), dim3(256), 0, st, rho, dual, n, partials, nb, cic);
+    else if (n <= 192)
+      hipLaunchKernelGGL((k_moments<3, false>), dim3(nb), dim3(256), 0, st, rho, dual, n, partials, nb, cic);
+    else
+      hipLaunchKernelGGL((k_moments<5, false>), dim3(nb), dim3(256), 0, st, rho, dual, n, partials, nb, cic);
+  }
   VH_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_moments_finish, dim3(1), dim3(640), 0, st, partials, nb, mom, mp);
   VH_LAUNCH_CHECK();
@@ -738,8 +787,16 @@ static int multipole_launch(hipStream_t st, const float *rho, const double *dual
 }
 
 int vh_multipole_impl(vh_mesh *m) {
+  // a coarse deposit that left its sums in the accumulator (vh_step): the
+  // density is formed in the moments' pass
+  vh_cic_src cic{nullptr, nullptr, nullptr, 0.0, 0.f};
+  if (m->acc_pending) {
+    cic = vh_cic_src{(unsigned long long *)m->acc64, m->rho, m->duallen, m->acc_inv_scale,
+                     m->acc_charge};
+    m->acc_pending = 0;
+  }
   return multipole_launch(m->ctx->stream, m->rho, m->dual, m->n, m->partials, m->scal,
-                          m->scal + 10);
+                          m->scal + 10, cic);
 }
 
 // --- the multipole without allocation --------------------------------------
