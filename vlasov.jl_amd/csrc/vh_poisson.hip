@@ -762,7 +762,8 @@ static int moments_blocks(int n) {
 // through 10 x moments_blocks(n) FP64 partials
 static int multipole_launch(hipStream_t st, const float *rho, const double *dual, int n,
                             double *partials, double *mom, double *mp,
-                            vh_cic_src cic = vh_cic_src{nullptr, nullptr, nullptr, 0.0, 0.f}) {
+                            vh_cic_src cic = vh_cic_src{nullptr, nullptr, nullptr, 0.0, 0.f},
+                            bool finish = true) {
   VH_REQUIRE(n >= 2 && n <= MOM_MAXN, "multipole: n = %d outside [2, %d]", n, MOM_MAXN);
   const int nb = moments_blocks(n);
   if (cic.acc) {
@@ -781,12 +782,15 @@ static int multipole_launch(hipStream_t st, const float *rho, const double *dual
       hipLaunchKernelGGL((k_moments<5, false>), dim3(nb), dim3(256), 0, st, rho, dual, n, partials, nb, cic);
   }
   VH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_moments_finish, dim3(1), dim3(640), 0, st, partials, nb, mom, mp);
-  VH_LAUNCH_CHECK();
+  if (finish) {
+    hipLaunchKernelGGL(k_moments_finish, dim3(1), dim3(640), 0, st, partials, nb, mom, mp);
+    VH_LAUNCH_CHECK();
+  }
   return VH_OK;
 }
 
-int vh_multipole_impl(vh_mesh *m) {
+// finish = false: the block partials only (k_boundary_multipole ends the sums)
+static int multipole_moments(vh_mesh *m, bool finish) {
   // a coarse deposit that left its sums in the accumulator (vh_step): the
   // density is formed in the moments' pass
   vh_cic_src cic{nullptr, nullptr, nullptr, 0.0, 0.f};
@@ -796,8 +800,10 @@ int vh_multipole_impl(vh_mesh *m) {
     m->acc_pending = 0;
   }
   return multipole_launch(m->ctx->stream, m->rho, m->dual, m->n, m->partials, m->scal,
-                          m->scal + 10, cic);
+                          m->scal + 10, cic, finish);
 }
+
+int vh_multipole_impl(vh_mesh *m) { return multipole_moments(m, true); }
 
 // --- the multipole without allocation --------------------------------------
 // The partials, moments and multipole of vh_multipole_noalloc live in this
@@ -919,9 +925,34 @@ __device__ __forceinline__ void face_point(int64_t t, int n, int &i, int &j, int
   }
 }
 
-// multipole potential (src/poisson.jl:198-206) on the faces
-__global__ void k_boundary_potential(float *__restrict__ phi, const double *__restrict__ mp,
-                                     const double *__restrict__ g, int n) {
+// The multipole potential (src/poisson.jl:198-206) on phi's faces, with
+// k_moments_finish in the same launch: every block sums the moments' block
+// partials (wave v value v, k_moments_finish's order, so every block holds
+// the same bits) and forms the multipole, block 0 stores both, and the
+// blocks then fill the faces.
+__global__ void __launch_bounds__(640)
+    k_boundary_multipole(float *__restrict__ phi, const double *__restrict__ part, int nparts,
+                         double *__restrict__ mom, double *__restrict__ mpout,
+                         const double *__restrict__ g, int n) {
+  __shared__ double sm[10], mp[10];
+  const int v = threadIdx.x >> 6, l = threadIdx.x & 63;
+  double q[4] = {0.0, 0.0, 0.0, 0.0};
+  int i0 = l;
+  for (; i0 + 192 < nparts; i0 += 256)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) q[u] += part[(int64_t)v * nparts + i0 + 64 * u];
+#pragma unroll
+  for (int u = 0; u < 3; ++u)
+    if (i0 + 64 * u < nparts) q[u] += part[(int64_t)v * nparts + i0 + 64 * u];
+  const double sv = vh_wave_sum((q[0] + q[1]) + (q[2] + q[3]));
+  if (l == 0) sm[v] = sv;
+  __syncthreads();
+  if (threadIdx.x == 0) multipole_of(sm, mp);
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x < 10) {
+    mom[threadIdx.x] = sm[threadIdx.x];
+    mpout[threadIdx.x] = mp[threadIdx.x];
+  }
   const int64_t nf = nface_of(n);
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nf;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -1053,9 +1084,9 @@ static int solve_interior(vh_mesh *m) {
 
 int vh_poisson_multipole_boundary_impl(vh_mesh *m) {
   vh_context *c = m->ctx;
-  VH_TRY(vh_multipole_impl(m));
-  hipLaunchKernelGGL(k_boundary_potential, dim3(512), dim3(256), 0, c->stream, m->phi, m->scal + 10,
-                     m->colloc64, m->n);
+  VH_TRY(multipole_moments(m, false));
+  hipLaunchKernelGGL(k_boundary_multipole, dim3(160), dim3(640), 0, c->stream, m->phi, m->partials,
+                     moments_blocks(m->n), m->scal, m->scal + 10, m->colloc64, m->n);
   VH_LAUNCH_CHECK();
   return VH_OK;
 }
