@@ -78,6 +78,41 @@ static int fill_iota(vh_context *c, uint32_t *p, int64_t n) {
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
+// A flag wait is a kernel that spins: it holds the hardware queue its stream
+// runs on, and the packets behind it in that queue, until the flag is set.
+// When the runtime has more streams than hardware queues (four per process
+// by default) it puts several streams on one queue, and a wait on the side
+// stream whose signal is queued behind it on the same queue would only end
+// at its time limit; so would a cycle through the queues of two contexts
+// whose steps interleave.  Flags therefore order a step's streams only when
+// (a) the context is the only one alive in the process and (b) at creation a
+// wait on the side stream ended on a signal queued after it on the main one,
+// i.e. the two streams do not share a queue.  Otherwise events order them.
+static std::atomic<int> g_live_contexts{0};
+
+__global__ void k_flag_signal(uint32_t *__restrict__ f, uint32_t v);
+__global__ void k_flag_wait(const uint32_t *__restrict__ f, uint32_t v, uint32_t *__restrict__ timeouts,
+                            uint64_t limit);
+
+static int probe_stream_queues(vh_context *c) {
+  uint32_t *f = c->flags + 16 * VH_FLAG_PROBE;
+  // 20 ms of the 100 MHz constant clock: a shared queue costs the creation that
+  hipLaunchKernelGGL(k_flag_wait, dim3(1), dim3(1), 0, c->side, f, 1u, f + 4, (uint64_t)2000000);
+  VH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_flag_signal, dim3(1), dim3(1), 0, c->stream, f, 1u);
+  VH_LAUNCH_CHECK();
+  VH_CHECK_HIP(hipStreamSynchronize(c->side));
+  VH_CHECK_HIP(hipStreamSynchronize(c->stream));
+  uint32_t missed = 1;
+  VH_CHECK_HIP(hipMemcpy(&missed, f + 4, sizeof missed, hipMemcpyDeviceToHost));
+  c->flags_ok = missed == 0;
+  return VH_OK;
+}
+
+bool vh_stream_flags_usable(const vh_context *c) {
+  return c->flags_ok && g_live_contexts.load() == 1;
+}
+
 extern "C" int vh_context_create(int device, vh_context **out) {
   VH_REQUIRE(out != nullptr, "vh_context_create: null output");
   *out = nullptr;
@@ -119,6 +154,14 @@ extern "C" int vh_context_create(int device, vh_context **out) {
     return VH_EHIP;
   }
   c->flag_seq = 0;
+  c->flags_ok = false;
+  c->live = false;
+  if (probe_stream_queues(c) != VH_OK) {
+    vh_context_destroy(c);
+    return VH_EHIP;
+  }
+  g_live_contexts.fetch_add(1);
+  c->live = true;
   *out = c;
   return VH_OK;
 }
@@ -133,15 +176,16 @@ __global__ void k_flag_signal(uint32_t *__restrict__ f, uint32_t v) {
   __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// One lane polls the flag; gives up after two seconds of the 100 MHz
+// One lane polls the flag; gives up after `limit` ticks of the 100 MHz
 // constant clock (or 2^24 polls) and counts the timeout.  The work queued
 // behind the wait starts when this kernel ends.
-__global__ void k_flag_wait(const uint32_t *__restrict__ f, uint32_t v, uint32_t *__restrict__ timeouts) {
+__global__ void k_flag_wait(const uint32_t *__restrict__ f, uint32_t v, uint32_t *__restrict__ timeouts,
+                            uint64_t limit) {
   const uint64_t t0 = wall_clock64();
   for (uint32_t i = 0; i < (1u << 24); ++i) {
     const uint32_t cur = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if ((int32_t)(cur - v) >= 0) return;
-    if (wall_clock64() - t0 > 200000000ull) break;
+    if (wall_clock64() - t0 > limit) break;
     __builtin_amdgcn_s_sleep(8);
   }
   atomicAdd(timeouts, 1u);
@@ -154,14 +198,17 @@ int vh_stream_signal(vh_context *c, hipStream_t st, int f, uint32_t v) {
 }
 
 int vh_stream_wait(vh_context *c, hipStream_t st, int f, uint32_t v) {
+  // two seconds
   hipLaunchKernelGGL(k_flag_wait, dim3(1), dim3(1), 0, st, c->flags + 16 * f, v,
-                     c->flags + 16 * VH_FLAG_TIMEOUTS);
+                     c->flags + 16 * VH_FLAG_TIMEOUTS, (uint64_t)200000000);
   VH_LAUNCH_CHECK();
   return VH_OK;
 }
 
 extern "C" int vh_context_destroy(vh_context *ctx) {
   if (!ctx) return VH_OK;
+  if (ctx->live) g_live_contexts.fetch_sub(1);
+  ctx->live = false;
   VH_CHECK_HIP(hipSetDevice(ctx->device));
   VH_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   VH_CHECK_HIP(hipStreamSynchronize(ctx->side));
@@ -1553,7 +1600,7 @@ static int step_eager(vh_accel *a, vh_mesh *fine, vh_mesh *coarse, double weight
       // after its signal; in a capture by events.
       hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
       VH_CHECK_HIP(hipStreamIsCapturing(c->stream, &cs));
-      const bool flags = cs == hipStreamCaptureStatusNone;
+      const bool flags = cs == hipStreamCaptureStatusNone && vh_stream_flags_usable(c);
       const uint32_t seq = flags ? ++c->flag_seq : 0;
       // The fork's flag is stored as the fine deposit's first kernel starts
       // and the join's as the coarse mean field's does (the work before

@@ -81,6 +81,8 @@ struct vh_context {
   // a kernel already spinning on the device ends within a few.
   uint32_t *flags;      // VH_FLAG_WORDS words: flag f at f * 16 (own 64-byte line)
   uint32_t flag_seq;    // sequence number of the last step's signals
+  bool flags_ok;        // the two streams were seen to progress independently
+  bool live;            // counted among the process's live contexts
   int num_cus;
   // timing of regions with events on `stream`
   int prof_on;                          // mask of the stages timed
@@ -133,6 +135,7 @@ struct vh_mesh {
   int acc_pending;
   double acc_inv_scale;
   float acc_charge;
+  int inner_ready;   // the moments pass also wrote the interior's right-hand side
   uint64_t serial;   // distinct for every mesh the process creates
 };
 
@@ -484,13 +487,16 @@ int vh_poisson_coarse_level_impl(vh_mesh *coarse);   // solve and coefficients
 int vh_poisson_fine_boundary_impl(vh_mesh *fine, vh_mesh *coarse);
 int vh_poisson_fine_part_impl(vh_mesh *fine);
 // launches of the scope go to the context's side stream
-enum { VH_FLAG_FORK = 0, VH_FLAG_JOIN, VH_FLAG_SIDE, VH_FLAG_TIMEOUTS, VH_FLAG_COUNT };
+enum { VH_FLAG_FORK = 0, VH_FLAG_JOIN, VH_FLAG_SIDE, VH_FLAG_TIMEOUTS, VH_FLAG_PROBE, VH_FLAG_COUNT };
 static inline vh_sig vh_flag_sig(vh_context *c, int f, uint32_t v) { return vh_sig{c->flags + 16 * f, v}; }
 #define VH_FLAG_WORDS (16 * VH_FLAG_COUNT)
 // flag f := v once the work queued on st before it is complete
 int vh_stream_signal(vh_context *c, hipStream_t st, int f, uint32_t v);
 // the work queued on st after this waits until flag f has reached v
 int vh_stream_wait(vh_context *c, hipStream_t st, int f, uint32_t v);
+// whether a step may order its streams by flags: this context is the only
+// one alive in the process and its streams passed the probe at creation
+bool vh_stream_flags_usable(const vh_context *c);
 
 struct vh_on_side {
   vh_context *c;

@@ -595,6 +595,7 @@ struct vh_cic_src {
   const float *wl;   // dual lengths
   double inv_scale;
   float charge;
+  float *inner;      // non-null: also the interior's right-hand side, as k_rho_interior
 };
 template <int Q, bool CIC>
 __global__ void __launch_bounds__(256)
@@ -636,6 +637,10 @@ __global__ void __launch_bounds__(256)
             const int64_t idx = (int64_t)t * n + i;
             cic.rho[idx] = val[u][q];
             cic.acc[idx] = 0ull;
+            const int ni = n - 2;
+            if (cic.inner && i >= 1 && j >= 1 && k >= 1 && i <= ni && j <= ni && k <= ni)
+              cic.inner[(i - 1) + (int64_t)ni * ((j - 1) + (int64_t)ni * (k - 1))] =
+                  (-4.0f * 3.14159265358979323846f) * val[u][q];
           }
         }
       }
@@ -762,7 +767,7 @@ static int moments_blocks(int n) {
 // through 10 x moments_blocks(n) FP64 partials
 static int multipole_launch(hipStream_t st, const float *rho, const double *dual, int n,
                             double *partials, double *mom, double *mp,
-                            vh_cic_src cic = vh_cic_src{nullptr, nullptr, nullptr, 0.0, 0.f},
+                            vh_cic_src cic = vh_cic_src{nullptr, nullptr, nullptr, 0.0, 0.f, nullptr},
                             bool finish = true) {
   VH_REQUIRE(n >= 2 && n <= MOM_MAXN, "multipole: n = %d outside [2, %d]", n, MOM_MAXN);
   const int nb = moments_blocks(n);
@@ -793,11 +798,15 @@ static int multipole_launch(hipStream_t st, const float *rho, const double *dual
 static int multipole_moments(vh_mesh *m, bool finish) {
   // a coarse deposit that left its sums in the accumulator (vh_step): the
   // density is formed in the moments' pass
-  vh_cic_src cic{nullptr, nullptr, nullptr, 0.0, 0.f};
+  vh_cic_src cic{nullptr, nullptr, nullptr, 0.0, 0.f, nullptr};
   if (m->acc_pending) {
+    // with the interior's right-hand side for the solve by linearity
+    // (vh_poisson_fused_prepare_impl then skips k_rho_interior)
+    const bool inner = m->n <= VH_FUSED_MAXN;
     cic = vh_cic_src{(unsigned long long *)m->acc64, m->rho, m->duallen, m->acc_inv_scale,
-                     m->acc_charge};
+                     m->acc_charge, inner ? m->inner : nullptr};
     m->acc_pending = 0;
+    m->inner_ready = inner;
   }
   return multipole_launch(m->ctx->stream, m->rho, m->dual, m->n, m->partials, m->scal,
                           m->scal + 10, cic, finish);
@@ -1283,9 +1292,12 @@ __global__ void k_ztilde(float *__restrict__ Z, const float *__restrict__ Y,
 int vh_poisson_fused_prepare_impl(vh_mesh *m) {
   vh_context *c = m->ctx;
   VH_REQUIRE(m->n <= 256, "fused solve: n = %d beyond 256", m->n);
-  hipLaunchKernelGGL(k_rho_interior, dim3(m->ni, m->ni), dim3((m->ni + 63) / 64 * 64), 0, c->stream,
-                     m->inner, m->rho, m->n);
-  VH_LAUNCH_CHECK();
+  if (!m->inner_ready) {
+    hipLaunchKernelGGL(k_rho_interior, dim3(m->ni, m->ni), dim3((m->ni + 63) / 64 * 64), 0, c->stream,
+                       m->inner, m->rho, m->n);
+    VH_LAUNCH_CHECK();
+  }
+  m->inner_ready = 0;
   // inner -> work1 -> work2 -> inner
   VH_TRY(apply_all(c, vh_op{m->Minv_op, nullptr}, m->ni, m->inner, m->work1, m->work2, m->inner, EPI_STORE,
                    nullptr, 0));
