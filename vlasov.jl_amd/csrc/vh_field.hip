@@ -437,12 +437,12 @@ __device__ __forceinline__ ft_f4 ft_contract(const float A[7][3], const float *r
       fy = __builtin_fmaf(z[u][m], uy[u][m], fy);
       fz = __builtin_fmaf(z[u][3 + m], uz[u][m], fz);
     }
-    fx += __shfl_xor(fx, 16, 64);
-    fy += __shfl_xor(fy, 16, 64);
-    fz += __shfl_xor(fz, 16, 64);
-    fx += __shfl_xor(fx, 32, 64);
-    fy += __shfl_xor(fy, 32, 64);
-    fz += __shfl_xor(fz, 32, 64);
+    fx = vh_sum_xor16(fx);
+    fy = vh_sum_xor16(fy);
+    fz = vh_sum_xor16(fz);
+    fx = vh_sum_xor32(fx);
+    fy = vh_sum_xor32(fy);
+    fz = vh_sum_xor32(fz);
     return ft_f4{-w * fx, -w * fy, -w * fz, 0.f};
   }
 }
@@ -549,12 +549,12 @@ __device__ __forceinline__ ft_f4 ft_contract16(const ft_u4 A[7], const float *ry
     fy = __builtin_fmaf(z[m], uy[m], fy);
     fz = __builtin_fmaf(z[3 + m], uz[m], fz);
   }
-  fx += __shfl_xor(fx, 16, 64);
-  fy += __shfl_xor(fy, 16, 64);
-  fz += __shfl_xor(fz, 16, 64);
-  fx += __shfl_xor(fx, 32, 64);
-  fy += __shfl_xor(fy, 32, 64);
-  fz += __shfl_xor(fz, 32, 64);
+  fx = vh_sum_xor16(fx);
+  fy = vh_sum_xor16(fy);
+  fz = vh_sum_xor16(fz);
+  fx = vh_sum_xor32(fx);
+  fy = vh_sum_xor32(fy);
+  fz = vh_sum_xor32(fz);
   return ft_f4{mx * fx, myz * fy, myz * fz, 0.f};
 }
 

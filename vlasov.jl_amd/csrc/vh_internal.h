@@ -397,6 +397,20 @@ __device__ __forceinline__ uint32_t cell_key0(int kx, int ky, int kz, int nk) {
 
 __device__ __forceinline__ int vh_lane() { return __lane_id(); }
 
+// x[l] + x[l ^ 16] and x[l] + x[l ^ 32] in every lane l by the gfx950
+// row swaps (v_permlane16_swap / v_permlane32_swap, two registers holding
+// x: the pair returned is x at l and at the partner lane, in some order) on
+// the VALU, where __shfl_xor across rows goes through LDS (ds_bpermute).
+// FP32 addition commutes, so the sums equal x + __shfl_xor(x, 16 or 32).
+__device__ __forceinline__ float vh_sum_xor16(float x) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float vh_sum_xor32(float x) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
 template <typename T>
 __device__ __forceinline__ T vh_wave_sum(T v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
